@@ -1,0 +1,171 @@
+"""Benchmark: real-time RK4 steps/s of the dipolar GPE on one MI355X.
+
+Workload (configurable with --grid): 256^3 one-component dipolar GPE, box
+16 x 16 x 32 (oscillator units), cigar trap omega = (1, 1, 0.25), g = 200,
+eps_dd = 1.3 with dipoles along z, dt = 5e-4; psi0 = seeded Thomas-Fermi
+profile + 1e-3 complex noise.  Synthetic data generated in-process.
+
+`value` is the plain FP64 form's steps/s with psi and V resident in HBM; the
+mixed-precision form is timed the same way and reported beside it with its
+error against the FP64 form.  The roofline object covers the kernel class
+with the largest share of the FP64 step time, from HIP events recorded on
+the stream the kernels run on, over a second timed region of the same length.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def run_form(torch, w, psi, V, precision, steps, warmup):
+    from fgpe_amd import Context
+    g = w.grid
+    kx, ky, kz = g.kaxes()
+    ctx = Context(g.nx, g.ny, g.nz, precision)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    ctx.set_kspace(kx, ky, kz, g.dv)
+    ctx.set_hamiltonian(a=w.a, a_dd=w.a_dd, dip=w.dip)
+    ctx.upload_psi(psi)
+    ctx.upload_vext(V)
+    t = 0.0
+    ctx.rk4_steps(w.dt, warmup, t)
+    t += warmup * w.dt
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    ctx.rk4_steps(w.dt, steps, t)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    t += steps * w.dt
+    # second timed region: per-kernel-class HIP events on the same stream
+    ctx.timing_enable(True)
+    ctx.rk4_steps(w.dt, steps, t)
+    kt = ctx.timing_read()
+    ctx.timing_enable(False)
+    t += steps * w.dt
+    obs = ctx.observables(t)
+    out = ctx.download_psi()
+    ctx.close()
+    return ms / steps, kt, obs, out
+
+
+def cpu_baseline(w, psi, V, nsteps, threads):
+    from oracle import oracle as orc
+    orc.set_threads(threads)
+    g = w.grid
+    kx, ky, kz = g.kaxes()
+    og = orc.Grid(g.nx, g.ny, g.nz, kx, ky, kz, g.dv)
+    h = orc.Ham(1, w.a, vtilde=orc.vdd(og, w.a_dd, w.dip))
+    t0 = time.perf_counter()
+    orc.rk4(og, h, psi, V, w.dt, nsteps)
+    dt = time.perf_counter() - t0
+    return nsteps / dt, orc.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", default="256,256,256")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is "
+              "single-GPU by design, see DESIGN.md for how it would shard", file=sys.stderr)
+        return 2
+
+    import torch
+    from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, configs, roofline
+
+    nx, ny, nz = (int(v) for v in args.grid.split(","))
+    w = configs.dipolar_real(nx, ny, nz, nsteps=args.steps)
+    psi, V = w.initial_state(), w.trap()
+    torch.cuda.set_device(0)
+    print(f"# workload {w.name}: {nx}x{ny}x{nz}, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+
+    res = {}
+    for name, prec in (("f64", FGPE_PREC_F64), ("mixed", FGPE_PREC_MIXED)):
+        ms, kt, obs, out = run_form(torch, w, psi, V, prec, args.steps, args.warmup)
+        res[name] = dict(ms=ms, kt=kt, obs=obs, psi=out)
+        print(f"# {name}: {ms:.3f} ms/step, {1000.0 / ms:.2f} steps/s, kernels {kt}",
+              file=sys.stderr, flush=True)
+
+    N = nx * ny * nz
+    nch = 2
+    f64 = res["f64"]
+    mixed = res["mixed"]
+    steps_s = 1000.0 / f64["ms"]
+    # dominant kernel class of the FP64 form
+    kt = {k: v for k, v in f64["kt"].items() if v[1] > 0}
+    dom = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0] / kt[dom][1]
+    if dom == "xstage":
+        per_pt = sum(roofline.xstage_bytes(s, False, nch) for s in (1, 2, 3, 4)) / 4.0
+    else:
+        per_pt = roofline.pass_bytes(False, nch)
+    achieved = per_pt * N / (avg_ms * 1e-3) / 1e9
+    step_gbs = roofline.step_bytes_per_point(False, nch) * N * steps_s / 1e9
+    mixed_steps = 1000.0 / mixed["ms"]
+    l2 = float(np.sqrt(np.sum(np.abs(mixed["psi"] - f64["psi"]) ** 2) / np.sum(np.abs(f64["psi"]) ** 2)))
+    de = float(abs(mixed["obs"][2] - f64["obs"][2]) / abs(f64["obs"][2]))
+
+    line = {
+        "metric": "RK4 steps/s at 256^3 dipolar GPE (FP64 plain form; mixed form beside it)",
+        "value": round(steps_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Thomas-Fermi psi0 + noise, harmonic trap)",
+        "config": {"workload": f"dipolar real-time RK4 {nx}x{ny}x{nz}", "grid": [nx, ny, nz],
+                   "box": [16.0, 16.0, 32.0], "g": 200.0, "eps_dd": 1.3, "omega": [1, 1, 0.25],
+                   "dt": w.dt},
+        "mixed": {"steps_per_s": round(mixed_steps, 3), "ms_per_step": round(mixed["ms"], 4),
+                  "speedup_vs_f64": round(mixed_steps / steps_s, 3), "l2_psi_err_vs_f64": l2,
+                  "rel_energy_err_vs_f64": de, "steps_compared": args.warmup + 2 * args.steps},
+        "step_traffic": {"bytes_per_point_f64": roofline.step_bytes_per_point(False, nch),
+                         "bytes_per_point_mixed": roofline.step_bytes_per_point(True, nch),
+                         "f64_GBs": round(step_gbs, 1),
+                         "f64_frac_of_peak": round(step_gbs / roofline.HBM_PEAK_GBS, 4)},
+        "kernels_f64_ms_avg": {k: round(v[0] / v[1], 4) for k, v in kt.items()},
+        "kernels_mixed_ms_avg": {k: round(v[0] / v[1], 4) for k, v in mixed["kt"].items() if v[1] > 0},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_point": per_pt, "avg_launch_ms": round(avg_ms, 4)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and args.cpu_steps > 0:
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        sps, used = cpu_baseline(w, psi, V, args.cpu_steps, threads)
+        line["cpu_baseline"] = {"value": round(sps, 5), "unit": "steps/s", "cores": used,
+                                "kind": "port",
+                                "sample": f"{args.cpu_steps} FP64 RK4 steps of the same {nx}x{ny}x{nz} "
+                                          f"dipolar workload, oracle/fgpe_oracle.c (own FFT), "
+                                          f"{used} OpenMP threads"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,200 @@
+"""ctypes binding of libfgpe.so (include/fgpe_c.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfgpe.so")
+
+FGPE_PREC_F64 = 0
+FGPE_PREC_MIXED = 1
+FGPE_IMAG_RK4_RENORM = 0
+FGPE_IMAG_HEAVY_BALL = 1
+
+# every symbol include/fgpe_c.h declares
+EXPORTS = (
+    "fgpe_ctx_create", "fgpe_ctx_destroy", "fgpe_last_error", "fgpe_ctx_set_stream",
+    "fgpe_ctx_sync", "fgpe_ctx_device_bytes", "fgpe_set_kspace", "fgpe_set_hamiltonian",
+    "fgpe_upload_psi", "fgpe_download_psi", "fgpe_upload_vext", "fgpe_psi_device_ptr",
+    "fgpe_fft3d_c2c_f64", "fgpe_fft3d_c2c_f32", "fgpe_laplacian_f64", "fgpe_apply_h",
+    "fgpe_observables", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_timing_enable",
+    "fgpe_timing_read",
+)
+KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
+
+
+class HParams(C.Structure):
+    _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
+                ("quench", C.c_double * 6), ("a_dd", C.c_double), ("dip", C.c_double * 3),
+                ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int)]
+
+
+class FgpeError(RuntimeError):
+    pass
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+def load() -> C.CDLL:
+    """Load libfgpe.so and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FgpeError(f"{LIB_PATH} is missing: build it with `python -m fgpe_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.fgpe_last_error.restype = C.c_char_p
+    L.fgpe_ctx_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]
+    L.fgpe_ctx_destroy.argtypes = [_vp]
+    L.fgpe_ctx_set_stream.argtypes = [_vp, _vp]
+    L.fgpe_ctx_sync.argtypes = [_vp]
+    L.fgpe_ctx_device_bytes.argtypes = [_vp, C.POINTER(C.c_size_t)]
+    L.fgpe_set_kspace.argtypes = [_vp, _dp, _dp, _dp, C.c_double]
+    L.fgpe_set_hamiltonian.argtypes = [_vp, C.POINTER(HParams)]
+    L.fgpe_upload_psi.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_download_psi.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_upload_vext.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_psi_device_ptr.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
+    L.fgpe_fft3d_c2c_f64.argtypes = [_vp, _vp, _vp, C.c_int]
+    L.fgpe_fft3d_c2c_f32.argtypes = [_vp, _vp, _vp, C.c_int]
+    L.fgpe_laplacian_f64.argtypes = [_vp, _vp, _vp]
+    L.fgpe_apply_h.argtypes = [_vp, C.c_double, _vp, _vp]
+    L.fgpe_observables.argtypes = [_vp, C.c_double, _dp]
+    L.fgpe_rk4_steps.argtypes = [_vp, C.c_double, C.c_double, C.c_int]
+    L.fgpe_imag_steps.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, _dp, C.POINTER(C.c_int)]
+    L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
+    L.fgpe_timing_read.argtypes = [_vp, _dp]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise FgpeError(f"fgpe error {rc}: {load().fgpe_last_error().decode()}")
+
+
+def _dptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.float64, np.complex128)
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """Owns one fgpe_ctx (device buffers + stream) for an nx*ny*nz grid."""
+
+    def __init__(self, nx, ny, nz, precision=FGPE_PREC_F64):
+        L = load()
+        h = _vp()
+        check(L.fgpe_ctx_create(nx, ny, nz, precision, C.byref(h)))
+        self.h = h
+        self.shape = (nz, ny, nx)
+        self.N = nx * ny * nz
+        self.precision = precision
+        self.ncomp = 1
+
+    def close(self):
+        if self.h:
+            load().fgpe_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_stream(self, stream_ptr: int | None):
+        check(load().fgpe_ctx_set_stream(self.h, stream_ptr))
+
+    def sync(self):
+        check(load().fgpe_ctx_sync(self.h))
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(load().fgpe_ctx_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def set_kspace(self, kx, ky, kz, dv):
+        kx, ky, kz = (np.ascontiguousarray(k, dtype=np.float64) for k in (kx, ky, kz))
+        check(load().fgpe_set_kspace(self.h, _dptr(kx), _dptr(ky), _dptr(kz), float(dv)))
+
+    def set_hamiltonian(self, ncomp=1, a=(0.0,), quench=None, a_dd=0.0, dip=(0.0, 0.0, 1.0),
+                        sc_U=0.0, sc_R=0.0, sc_dim=3):
+        aa = tuple(a) + (0.0,) * (3 - len(a))
+        q = (0.0,) * 6 if quench is None else tuple(quench)
+        hp = HParams(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
+                     (C.c_double * 6)(*q), a_dd, (C.c_double * 3)(*dip), sc_U, sc_R, sc_dim)
+        check(load().fgpe_set_hamiltonian(self.h, C.byref(hp)))
+        self.ncomp = ncomp
+
+    def upload_psi(self, psi, comp=0):
+        a = np.ascontiguousarray(psi, dtype=np.complex128).reshape(-1)
+        assert a.size == self.N
+        check(load().fgpe_upload_psi(self.h, comp, _dptr(a)))
+
+    def download_psi(self, comp=0):
+        out = np.empty(self.shape, dtype=np.complex128)
+        check(load().fgpe_download_psi(self.h, comp, _dptr(out)))
+        return out
+
+    def upload_vext(self, V, comp=0):
+        a = np.ascontiguousarray(V, dtype=np.complex128).reshape(-1)
+        assert a.size == self.N
+        check(load().fgpe_upload_vext(self.h, comp, _dptr(a)))
+
+    def psi_device_ptr(self, comp=0) -> int:
+        p = _vp()
+        check(load().fgpe_psi_device_ptr(self.h, comp, C.byref(p)))
+        return p.value
+
+    def fft3d_f64(self, dev_in: int, dev_out: int, direction: int):
+        check(load().fgpe_fft3d_c2c_f64(self.h, dev_in, dev_out, direction))
+
+    def fft3d_f32(self, dev_in: int, dev_out: int, direction: int):
+        check(load().fgpe_fft3d_c2c_f32(self.h, dev_in, dev_out, direction))
+
+    def laplacian_f64(self, dev_in: int, dev_out: int):
+        check(load().fgpe_laplacian_f64(self.h, dev_in, dev_out))
+
+    def apply_h(self, t: float, dev_hmu: int, dev_hen: int | None):
+        check(load().fgpe_apply_h(self.h, t, dev_hmu, dev_hen))
+
+    def observables(self, t: float = 0.0) -> np.ndarray:
+        out = np.zeros(3 * self.ncomp)
+        check(load().fgpe_observables(self.h, t, _dptr(out)))
+        return out
+
+    def rk4_steps(self, dt: float, nsteps: int, t0: float = 0.0):
+        check(load().fgpe_rk4_steps(self.h, dt, t0, nsteps))
+
+    def timing_enable(self, on=True):
+        check(load().fgpe_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self) -> dict:
+        out = np.zeros(2 * len(KCLASS_NAMES))
+        check(load().fgpe_timing_read(self.h, _dptr(out)))
+        return {n: (out[2 * i], int(out[2 * i + 1])) for i, n in enumerate(KCLASS_NAMES)}
+
+    def imag_steps(self, dt, nsteps, scheme=FGPE_IMAG_RK4_RENORM, beta=0.0, npart=1.0, eps=0.0,
+                   history=False):
+        hist = np.zeros((max(nsteps, 1), 3)) if history else None
+        done = C.c_int(0)
+        check(load().fgpe_imag_steps(self.h, scheme, dt, beta, npart, eps, nsteps,
+                                     _dptr(hist) if history else _dp(), C.byref(done)))
+        return (done.value, hist[:done.value]) if history else done.value

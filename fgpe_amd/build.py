@@ -1,0 +1,48 @@
+"""Build recipe of the HIP library libfgpe.so (gfx950) and the CPU oracle.
+
+    python -m fgpe_amd.build          # or __graft_entry__.build()
+
+Every call recompiles.  The library is built in-tree so that it travels with
+the source tree; it is git-ignored.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libfgpe.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 arithmetic stays off (v_pk_* FP32 forms are avoided)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+SOURCES = [os.path.join(HERE, "csrc", "fgpe_capi.hip")]
+
+
+def build_lib(verbose: bool = False) -> str:
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), *SOURCES, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def main() -> None:
+    build_lib(verbose=True)
+    build_oracle()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

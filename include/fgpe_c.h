@@ -1,0 +1,146 @@
+/*
+ * fgpe_c.h -- C ABI of the MI355X-native fGPE time-stepping library
+ * (libfgpe.so, hand-written HIP kernels for gfx950).
+ *
+ * The reference has no C ABI: its hot path is C++ member functions of the
+ * class chain Cuda <- BaseHamiltonian <- OneComponentGPSolver /
+ * TwoComponentGPSolver <- plugin.  Each entry point below replaces one of
+ * those members (cited per function); the reference-shaped C++ classes
+ * (fgpe_amd/host) and the Python bindings call only this header.
+ *
+ * Conventions
+ *   - Return value: 0 on success, a negative FGPE_E* code on failure; the
+ *     message of the last failure on the calling thread is fgpe_last_error().
+ *   - Complex arrays are interleaved (re, im) doubles in the reference's
+ *     layout ID(x,y,z) = z*NX*NY + y*NX + x (hpp/header.hpp:20).
+ *   - Host pointers are named host_*; dev_* arguments are device pointers
+ *     (HIP allocations on the context's device), used on the context's
+ *     stream.
+ *   - A context owns its device memory and one stream; it is not
+ *     thread-safe.  The caller owns the context handle.
+ */
+#ifndef FGPE_C_H
+#define FGPE_C_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGPE_OK 0
+#define FGPE_EINVAL -1   /* bad argument                                   */
+#define FGPE_EUNSUP -2   /* unsupported grid size / feature                */
+#define FGPE_EHIP -3     /* HIP runtime error                              */
+#define FGPE_ESTATE -4   /* call out of order (e.g. no k-space set)        */
+
+/* precision of the bulk (FFT passes, right-hand side, stage states) */
+#define FGPE_PREC_F64 0     /* everything FP64 (the plain drop-in form)       */
+#define FGPE_PREC_MIXED 1   /* FP32 complex bulk; FP64 state, twiddles and
+                               reductions                                    */
+
+/* imaginary-time schemes */
+#define FGPE_IMAG_RK4_RENORM 0  /* RK4 with c = -dt, renormalised each step  */
+#define FGPE_IMAG_HEAVY_BALL 1  /* the reference's scheme,
+                                   src/oneComponentGPSolver.cu:26-104         */
+
+typedef struct fgpe_ctx fgpe_ctx;
+
+/* Hamiltonian of one or two components.
+     H psi_c = -1/2 lap psi_c + (V_c + sum_c' 4 pi a_cc' n_c' + Phi) psi_c
+   Phi = F^-1[ Vt(k) F(n_total) ] / N with
+     Vt = 12 pi a_dd ((k.d/|k|)^2 - 1/3)        (src/parallel.cu:326-352)
+        + soft-core Vt of dimension sc_dim    (src/parallel.cu:371-477)
+   The energy functional uses 2 pi a and Phi/2. */
+typedef struct fgpe_hparams {
+    int ncomp;          /* 1 or 2                                              */
+    double a[3];        /* 1 comp: a[0]; 2 comp: a11, a22, a12 (code units)    */
+    int quench_on;      /* 1: a(t) of d_quench_as (BEC_oneComponent.cu:167)   */
+    double quench[6];   /* a0, a1, a2, t1, t2, t3                              */
+    double a_dd;        /* dipolar length; 0 disables                          */
+    double dip[3];      /* dipole direction                                    */
+    double sc_U;        /* soft-core strength; 0 disables                      */
+    double sc_R;        /* soft-core radius                                    */
+    int sc_dim;         /* 1, 2 or 3: which soft-core transform                */
+} fgpe_hparams;
+
+/* ---- context (replaces Cuda::Cuda, src/parallel.cu:14-113) ------------- */
+int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out);
+int fgpe_ctx_destroy(fgpe_ctx* ctx);
+const char* fgpe_last_error(void);
+/* Run on the given HIP stream from now on (NULL selects the HIP null
+   stream).  A context starts on a non-blocking stream of its own. */
+int fgpe_ctx_set_stream(fgpe_ctx* ctx, void* hip_stream);
+int fgpe_ctx_sync(fgpe_ctx* ctx);
+/* Bytes of device memory the context holds. */
+int fgpe_ctx_device_bytes(fgpe_ctx* ctx, size_t* bytes);
+
+/* k axes in cycles per length and the volume element
+   (BaseHamiltonian::InitializeKspace, src/baseHamiltonian.cu:53-82; DV from
+   src/params.cu:213). */
+int fgpe_set_kspace(fgpe_ctx* ctx, const double* host_kx, const double* host_ky,
+                    const double* host_kz, double dv);
+/* Hamiltonian parameters (the plugin constructor's d_h_params,
+   hamiltonians/BEC_oneComponent.cu:10-15, plus the dipolar/soft-core
+   kernels). */
+int fgpe_set_hamiltonian(fgpe_ctx* ctx, const fgpe_hparams* hp);
+
+/* State transfer (the H2D/D2H copies of runRealTimeEvol/runImagTimeEvol,
+   src/oneComponentGPSolver.cu:32-34,127-129,211). comp = 0 or 1. */
+int fgpe_upload_psi(fgpe_ctx* ctx, int comp, const double* host_psi);
+int fgpe_download_psi(fgpe_ctx* ctx, int comp, double* host_psi);
+int fgpe_upload_vext(fgpe_ctx* ctx, int comp, const double* host_vext);
+/* Device address of component comp's FP64 state (N complex doubles). */
+int fgpe_psi_device_ptr(fgpe_ctx* ctx, int comp, void** dev_psi);
+
+/* ---- building blocks on device arrays ---------------------------------- */
+/* 3D C2C transform of N complex values, unnormalised; direction -1 forward
+   (CUFFT_FORWARD), +1 inverse (cufftExecZ2Z, src/baseHamiltonian.cu:89,95).
+   in == out is allowed.  _f32 works on interleaved floats. */
+int fgpe_fft3d_c2c_f64(fgpe_ctx* ctx, const void* dev_in, void* dev_out, int direction);
+int fgpe_fft3d_c2c_f32(fgpe_ctx* ctx, const void* dev_in, void* dev_out, int direction);
+/* out = F^-1[(19.739208802/N) |k|^2 F psi] = -lap psi / 2
+   (alg_Laplace + Laplace, src/baseHamiltonian.cu:86-96, src/parallel.cu:194-205) */
+int fgpe_laplacian_f64(fgpe_ctx* ctx, const void* dev_psi, void* dev_out);
+
+/* H applied to the current state at time t (alg_calcHpsi,
+   hamiltonians/BEC_oneComponent.cu:92-103): dev_hmu, dev_hen receive
+   ncomp*N complex doubles each (dev_hen may be NULL). */
+int fgpe_apply_h(fgpe_ctx* ctx, double t, void* dev_hmu, void* dev_hen);
+/* norm = sum |psi_c|^2 DV, mu = sum Re(psi* H psi) DV / norm,
+   E = sum Re(psi* Hen psi) DV / norm, for each component c:
+   out[3*c + {0,1,2}] (alg_calcNorm + alg_calc2Observables,
+   src/baseHamiltonian.cu:173-179,230-236). */
+int fgpe_observables(fgpe_ctx* ctx, double t, double* host_out);
+
+/* ---- time stepping ----------------------------------------------------- */
+/* nsteps real-time RK4 steps from time t0 (runRealTimeEvol's loop body,
+   src/oneComponentGPSolver.cu:150-208; two components:
+   src/twoComponentGPSolver.cu:142-203). */
+int fgpe_rk4_steps(fgpe_ctx* ctx, double dt, double t0, int nsteps);
+/* nsteps imaginary-time steps (scheme FGPE_IMAG_*).  npart: target norm per
+   component.  eps > 0 stops heavy-ball at |E - E_prev| / E_prev < eps
+   (src/oneComponentGPSolver.cu:84-88).  host_hist (3*nsteps doubles per
+   component or NULL) gets norm, mu, E per step; *steps_done the number run. */
+int fgpe_imag_steps(fgpe_ctx* ctx, int scheme, double dt, double beta, double npart,
+                    double eps, int nsteps, double* host_hist, int* steps_done);
+
+/* ---- instrumentation (no counterpart in the reference, which prints only
+   an "Average iteration time", hamiltonians/BEC_oneComponent.cu:279) ---- */
+#define FGPE_KCLASS_XSTAGE 0   /* fused x pass (pointwise H + RK stage)      */
+#define FGPE_KCLASS_STRIDED 1  /* y (or z) forward/inverse pass               */
+#define FGPE_KCLASS_LAST 2     /* last axis forward + k multiply + inverse    */
+#define FGPE_KCLASS_FFTX 3     /* standalone x pass                           */
+#define FGPE_KCLASS_OTHER 4    /* reductions and small kernels                */
+#define FGPE_NKCLASS 5
+/* on != 0: bracket every launch with HIP events on the context's stream and
+   accumulate per kernel class; on == 0 stops.  Enabling clears the record. */
+int fgpe_timing_enable(fgpe_ctx* ctx, int on);
+/* Synchronises, then writes out[2*k] = total ms, out[2*k+1] = launches for
+   each class k < FGPE_NKCLASS, and clears the record. */
+int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

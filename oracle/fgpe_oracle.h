@@ -1,0 +1,72 @@
+/*
+ * fgpe_oracle.h -- CPU FP64 restatement of the fGPE time-stepping hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product library (libfgpe) never links or calls it.
+ *
+ * Every function restates one piece of the reference (file:line cited at the
+ * definition).  Arrays of complex numbers are interleaved (re, im) doubles,
+ * laid out as ID(x,y,z) = z*NX*NY + y*NX + x (hpp/header.hpp:20).
+ */
+#ifndef FGPE_ORACLE_H
+#define FGPE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid {
+    int nx, ny, nz;          /* points per axis; N = nx*ny*nz                 */
+    const double *kx, *ky, *kz; /* k axes in cycles per length (InitializeKspace) */
+    double dv;               /* volume element DX*DY*DZ                       */
+} orc_grid;
+
+typedef struct orc_ham {
+    int    ncomp;            /* 1 or 2                                          */
+    double a[3];             /* 1 comp: a[0]; 2 comp: a11, a22, a12 (code units) */
+    int    quench_on;        /* 1: a(t) from the d_quench_as ramp              */
+    double quench[6];        /* a0, a1, a2, t1, t2, t3                          */
+    const double *vtilde;    /* real k-space kernel Vt(k) (N doubles) or NULL   */
+} orc_ham;
+
+/* ---- building blocks ---------------------------------------------------- */
+void   orc_kaxis(int n, double dk, double kmax, int active, double *k);
+void   orc_twiddle(int n, int m, double *c, double *s);
+void   orc_fft1d(int n, double *data, int sign);
+void   orc_fft3d(int nx, int ny, int nz, double *data, int sign);
+void   orc_laplace(const orc_grid *g, const double *psi, double *out);
+void   orc_vdd(const orc_grid *g, double a_dd, const double d[3], double *vt);
+void   orc_vsoftcore(const orc_grid *g, int dim, double R, double U, double *vt);
+void   orc_convolve(const orc_grid *g, const double *vt, const double *dens, double *phi);
+double orc_quench_a(double t, const double prms[6]);
+
+/* ---- Hamiltonian application ------------------------------------------- */
+/* psi, V, hmu, hen: ncomp consecutive complex arrays of N points each.
+   hen may be NULL (the "MU" kernels). */
+void   orc_hpsi(const orc_grid *g, const orc_ham *h, double t,
+                const double *psi, const double *V, double *hmu, double *hen);
+
+/* ---- reductions --------------------------------------------------------- */
+double orc_norm(const orc_grid *g, const double *psi);
+void   orc_obs2(const orc_grid *g, double norm, const double *psi,
+                const double *o1psi, const double *o2psi, double *o1, double *o2);
+
+/* ---- time loops --------------------------------------------------------- */
+/* Real-time RK4 (coef = -i dt) or imaginary-time RK4 (coef = -dt, renorm) */
+void   orc_rk4(const orc_grid *g, const orc_ham *h, double *psi, const double *V,
+               double dt, double t0, int nsteps, int imaginary, double npart);
+/* Heavy-ball imaginary time of the one-component solver.  Returns the number
+   of iterations run; hist (3*niter doubles or NULL) gets norm, mu, E per
+   iteration. */
+int    orc_imag_heavy_ball(const orc_grid *g, const orc_ham *h, double *psi,
+                           const double *V, double dt, double beta, double npart,
+                           double eps, int niter, double *hist);
+
+int    orc_num_threads(void);
+void   orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,217 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/fgpe_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: this module is the checker.  Only tests/,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py import
+it; the product path (fgpe_amd) never does.
+
+Complex arrays are numpy complex128 of shape (nz, ny, nx) (x fastest, the
+reference's ID(x,y,z) layout, hpp/header.hpp:20); two-component states are
+shape (2, nz, ny, nx).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfgpe_oracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("kx", _dp), ("ky", _dp), ("kz", _dp), ("dv", C.c_double)]
+
+
+class OrcHam(C.Structure):
+    _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
+                ("quench", C.c_double * 6), ("vtilde", _dp)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_kaxis.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_twiddle.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_fft1d.argtypes = [C.c_int, _dp, C.c_int]
+        L.orc_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int]
+        L.orc_laplace.argtypes = [C.POINTER(OrcGrid), _dp, _dp]
+        L.orc_vdd.argtypes = [C.POINTER(OrcGrid), C.c_double, _dp, _dp]
+        L.orc_vsoftcore.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_convolve.argtypes = [C.POINTER(OrcGrid), _dp, _dp, _dp]
+        L.orc_quench_a.argtypes = [C.c_double, _dp]
+        L.orc_quench_a.restype = C.c_double
+        L.orc_hpsi.argtypes = [C.POINTER(OrcGrid), C.POINTER(OrcHam), C.c_double,
+                               _dp, _dp, _dp, _dp]
+        L.orc_norm.argtypes = [C.POINTER(OrcGrid), _dp]
+        L.orc_norm.restype = C.c_double
+        L.orc_obs2.argtypes = [C.POINTER(OrcGrid), C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.orc_rk4.argtypes = [C.POINTER(OrcGrid), C.POINTER(OrcHam), _dp, _dp,
+                              C.c_double, C.c_double, C.c_int, C.c_int, C.c_double]
+        L.orc_imag_heavy_ball.argtypes = [C.POINTER(OrcGrid), C.POINTER(OrcHam), _dp, _dp,
+                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_int, _dp]
+        L.orc_imag_heavy_ball.restype = C.c_int
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(n)
+
+
+class Grid:
+    """Grid of the reference: k axes from InitializeKspace, DV = DX*DY*DZ."""
+
+    def __init__(self, nx, ny, nz, kx, ky, kz, dv):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.kx = np.ascontiguousarray(kx, dtype=np.float64)
+        self.ky = np.ascontiguousarray(ky, dtype=np.float64)
+        self.kz = np.ascontiguousarray(kz, dtype=np.float64)
+        self.dv = float(dv)
+        self._s = OrcGrid(nx, ny, nz, _p(self.kx), _p(self.ky), _p(self.kz), self.dv)
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def N(self):
+        return self.nx * self.ny * self.nz
+
+
+class Ham:
+    def __init__(self, ncomp=1, a=(0.0, 0.0, 0.0), quench=None, vtilde=None):
+        self.ncomp = ncomp
+        self.vtilde = None if vtilde is None else np.ascontiguousarray(vtilde, dtype=np.float64)
+        q = (0.0,) * 6 if quench is None else tuple(quench)
+        aa = tuple(a) + (0.0,) * (3 - len(a))
+        self._s = OrcHam(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
+                         (C.c_double * 6)(*q),
+                         _p(self.vtilde) if self.vtilde is not None else _dp())
+
+
+def kaxis(n, dk, kmax, active=True):
+    out = np.zeros(n)
+    lib().orc_kaxis(n, dk, kmax, 1 if active else 0, _p(out))
+    return out
+
+
+def twiddle(n, m):
+    c, s = C.c_double(), C.c_double()
+    lib().orc_twiddle(n, m, C.byref(c), C.byref(s))
+    return c.value, s.value
+
+
+def fft1d(x, sign=-1):
+    a = _c128(x).copy()
+    lib().orc_fft1d(a.size, _p(a.view(np.float64)), sign)
+    return a
+
+
+def fft3d(x, sign=-1):
+    a = _c128(x).copy()
+    nz, ny, nx = a.shape
+    lib().orc_fft3d(nx, ny, nz, _p(a.view(np.float64)), sign)
+    return a
+
+
+def laplace(g: Grid, psi):
+    a = _c128(psi)
+    out = np.empty_like(a)
+    lib().orc_laplace(C.byref(g._s), _p(a.view(np.float64)), _p(out.view(np.float64)))
+    return out
+
+
+def vdd(g: Grid, a_dd, d=(0.0, 0.0, 1.0)):
+    out = np.empty(g.shape)
+    dd = np.ascontiguousarray(d, dtype=np.float64)
+    lib().orc_vdd(C.byref(g._s), a_dd, _p(dd), _p(out))
+    return out
+
+
+def vsoftcore(g: Grid, dim, R, U):
+    out = np.empty(g.shape)
+    lib().orc_vsoftcore(C.byref(g._s), dim, R, U, _p(out))
+    return out
+
+
+def convolve(g: Grid, vt, dens):
+    vt = np.ascontiguousarray(vt, dtype=np.float64)
+    dens = np.ascontiguousarray(dens, dtype=np.float64)
+    out = np.empty(g.shape)
+    lib().orc_convolve(C.byref(g._s), _p(vt), _p(dens), _p(out))
+    return out
+
+
+def quench_a(t, prms):
+    p = np.ascontiguousarray(prms, dtype=np.float64)
+    return lib().orc_quench_a(t, _p(p))
+
+
+def hpsi(g: Grid, h: Ham, t, psi, V, energy=True):
+    psi = _c128(psi)
+    V = _c128(V)
+    hmu = np.empty_like(psi)
+    hen = np.empty_like(psi) if energy else None
+    lib().orc_hpsi(C.byref(g._s), C.byref(h._s), t, _p(psi.view(np.float64)),
+                   _p(V.view(np.float64)), _p(hmu.view(np.float64)),
+                   _p(hen.view(np.float64)) if energy else _dp())
+    return (hmu, hen) if energy else hmu
+
+
+def norm(g: Grid, psi):
+    return lib().orc_norm(C.byref(g._s), _p(_c128(psi).view(np.float64)))
+
+
+def obs2(g: Grid, nrm, psi, o1psi, o2psi):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_obs2(C.byref(g._s), nrm, _p(_c128(psi).view(np.float64)),
+                   _p(_c128(o1psi).view(np.float64)), _p(_c128(o2psi).view(np.float64)),
+                   C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def rk4(g: Grid, h: Ham, psi, V, dt, nsteps, t0=0.0, imaginary=False, npart=1.0):
+    a = _c128(psi).copy()
+    V = _c128(V)
+    lib().orc_rk4(C.byref(g._s), C.byref(h._s), _p(a.view(np.float64)), _p(V.view(np.float64)),
+                  dt, t0, nsteps, 1 if imaginary else 0, npart)
+    return a
+
+
+def imag_heavy_ball(g: Grid, h: Ham, psi, V, dt, beta, npart, eps, niter):
+    a = _c128(psi).copy()
+    V = _c128(V)
+    hist = np.zeros((max(niter, 1), 3))
+    n = lib().orc_imag_heavy_ball(C.byref(g._s), C.byref(h._s), _p(a.view(np.float64)),
+                                  _p(V.view(np.float64)), dt, beta, npart, eps, niter,
+                                  _p(hist))
+    return a, n, hist[:n]

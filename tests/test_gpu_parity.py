@@ -1,0 +1,193 @@
+"""Parity of the HIP path (called through the C ABI) with the CPU oracle on
+seeded inputs.  FP64 form: 1e-12 relative or better per quantity; mixed
+form: the error bounds written next to each assertion."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.cases import Case, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, arr):
+    return torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+
+
+FFT_SHAPES = [(32, 32, 32), (64, 32, 96), (128, 64, 32), (96, 192, 32), (512, 32, 32),
+              (256, 32, 64), (64, 64, 1), (256, 128, 1)]
+
+
+@pytest.mark.parametrize("shape", FFT_SHAPES)
+def test_fft3d_f64(torch_cuda, shape):
+    torch = torch_cuda
+    nx, ny, nz = shape
+    c = Case(nx, ny, nz)
+    x = c.rand_psi(11)
+    ref_f = np.fft.fftn(x)
+    ref_b = np.fft.ifftn(x) * x.size
+    with c.ctx() as ctx:
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        xi = dev(torch, x)
+        out = torch.empty_like(xi)
+        ctx.fft3d_f64(xi.data_ptr(), out.data_ptr(), -1)
+        torch.cuda.synchronize()
+        err = rel_max(out.cpu().numpy(), ref_f)
+        assert err < 1e-14, err
+        ctx.fft3d_f64(xi.data_ptr(), out.data_ptr(), +1)
+        torch.cuda.synchronize()
+        err = rel_max(out.cpu().numpy(), ref_b)
+        assert err < 1e-14, err
+        # in place, and against the oracle's transform
+        y = xi.clone()
+        ctx.fft3d_f64(y.data_ptr(), y.data_ptr(), -1)
+        torch.cuda.synchronize()
+        err = rel_max(y.cpu().numpy(), orc.fft3d(x, -1))
+        assert err < 1e-14, err
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (128, 64, 32), (256, 32, 64)])
+def test_fft3d_f32(torch_cuda, shape):
+    torch = torch_cuda
+    c = Case(*shape)
+    x = c.rand_psi(12)
+    with c.ctx() as ctx:
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        xi = dev(torch, x.astype(np.complex64))
+        out = torch.empty_like(xi)
+        ctx.fft3d_f32(xi.data_ptr(), out.data_ptr(), -1)
+        torch.cuda.synchronize()
+        # FP32 FFT: error ~ eps32 * log2(N)
+        err = rel_max(out.cpu().numpy().astype(np.complex128), np.fft.fftn(x))
+        assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 128), (128, 128, 1)])
+def test_laplacian_f64(torch_cuda, shape):
+    torch = torch_cuda
+    c = Case(*shape, L=(16.0, 12.0, 20.0))
+    x = c.rand_psi(13)
+    ref = orc.laplace(c.grid, x)
+    with c.ctx() as ctx:
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        xi = dev(torch, x)
+        out = torch.empty_like(xi)
+        ctx.laplacian_f64(xi.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        err = rel_max(out.cpu().numpy(), ref)
+        assert err < 1e-13, err
+
+
+HAMS = {
+    "contact": dict(a=(0.7,)),
+    "quench": dict(a=(0.7,), quench=(0.5, 0.9, 0.3, 0.1, 0.2, 0.4)),
+    "dipolar": dict(a=(1.2,), a_dd=1.5, dip=(0.0, 0.6, 0.8)),
+    "softcore": dict(a=(0.4,), sc_U=5.0, sc_R=1.5, sc_dim=3),
+    "dip+sc": dict(a=(0.4,), a_dd=0.6, dip=(0.0, 0.0, 1.0), sc_U=2.0, sc_R=1.2, sc_dim=3),
+}
+
+
+def oracle_ham(c, h):
+    vt = None
+    if h.get("a_dd") or h.get("sc_U"):
+        vt = c.vtilde(h.get("a_dd", 0.0), h.get("dip", (0, 0, 1)),
+                      (h.get("sc_dim", 3), h.get("sc_R"), h.get("sc_U")) if h.get("sc_U") else None)
+    return orc.Ham(1, h["a"], quench=h.get("quench"), vtilde=vt)
+
+
+@pytest.mark.parametrize("name", list(HAMS))
+def test_apply_h_and_observables(torch_cuda, name):
+    torch = torch_cuda
+    h = HAMS[name]
+    c = Case(32, 64, 32, L=(12.0, 16.0, 12.0))
+    psi = c.rand_psi(21, 0.2)
+    V = c.rand_psi(22, 3.0).real + 1j * (0.1 * c.rand_psi(23).real if name == "quench" else 0.0)
+    t = 0.15
+    hmu_ref, hen_ref = orc.hpsi(c.grid, oracle_ham(c, h), t, psi, V)
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        hmu = torch.empty(c.shape, dtype=torch.complex128, device="cuda")
+        hen = torch.empty_like(hmu)
+        ctx.apply_h(t, hmu.data_ptr(), hen.data_ptr())
+        ctx.sync()
+        err = rel_max(hmu.cpu().numpy(), hmu_ref)
+        assert err < 1e-13, err
+        err = rel_max(hen.cpu().numpy(), hen_ref)
+        assert err < 1e-13, err
+        obs = ctx.observables(t)
+    nrm = orc.norm(c.grid, psi)
+    mu, en = orc.obs2(c.grid, nrm, psi, hmu_ref, hen_ref)
+    assert abs(obs[0] - nrm) < 1e-13 * nrm
+    assert abs(obs[1] - mu) < 1e-12 * abs(mu)
+    assert abs(obs[2] - en) < 1e-12 * abs(en)
+
+
+def run_both(c, h, psi, V, dt, nsteps, precision=0, imaginary=False):
+    from fgpe_amd import Context  # noqa: F401
+    with c.ctx(precision) as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        if imaginary:
+            ctx.imag_steps(dt, nsteps)
+        else:
+            ctx.rk4_steps(dt, nsteps)
+        gpu = ctx.download_psi()
+        obs = ctx.observables(nsteps * dt)
+    return gpu, obs
+
+
+@pytest.mark.parametrize("name", ["contact", "dipolar", "softcore", "quench"])
+def test_rk4_real_time_f64(name):
+    h = dict(HAMS[name])
+    c = Case(32, 32, 64, L=(10.0, 10.0, 20.0))
+    from fgpe_amd import configs
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25))
+    if name == "quench":
+        V = V + 0.05j * c.rand_psi(5).real
+    psi = configs.normalise(np.exp(-0.5 * np.abs(configs.harmonic_trap(c.spec, (1, 1, 0.5)))) +
+                            0.01 * c.rand_psi(4), c.dv)
+    dt, n = 5e-4, 12
+    gpu, obs = run_both(c, h, psi, V, dt, n)
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, dt, n)
+    err = rel_max(np.abs(gpu), np.abs(ref))
+    assert err < 1e-12, err
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
+    hmu, hen = orc.hpsi(c.grid, oracle_ham(c, h), n * dt, ref, V)
+    nrm = orc.norm(c.grid, ref)
+    mu, en = orc.obs2(c.grid, nrm, ref, hmu, hen)
+    assert abs(obs[0] - nrm) < 1e-12 * nrm
+    assert abs(obs[2] - en) < 1e-12 * abs(en)
+
+
+def test_rk4_imag_renorm_f64():
+    from fgpe_amd import configs
+    c = Case(32, 32, 32, L=(16.0, 16.0, 16.0))
+    h = dict(a=(100.0 / (4 * np.pi),))
+    V = configs.harmonic_trap(c.spec)
+    psi = configs.gaussian_noisy(c.spec, 1)
+    dt, n = 1e-3, 30
+    gpu, obs = run_both(c, h, psi, V, dt, n, imaginary=True)
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, dt, n, imaginary=True, npart=1.0)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
+    assert abs(obs[0] - 1.0) < 1e-13
+
+
+def test_mixed_precision_dipolar_error():
+    """Mixed form against the FP64 GPU form: relative energy error <= 1e-6."""
+    from fgpe_amd import configs
+    w = configs.dipolar_real(64, 64, 128, nsteps=200)
+    psi, V = w.initial_state(), w.trap()
+    c = Case(64, 64, 128, L=(16.0, 16.0, 32.0))
+    h = dict(a=w.a, a_dd=w.a_dd, dip=w.dip)
+    g64, o64 = run_both(c, h, psi, V, w.dt, w.nsteps, precision=0)
+    g32, o32 = run_both(c, h, psi, V, w.dt, w.nsteps, precision=1)
+    l2 = float(np.sqrt(np.sum(np.abs(g32 - g64) ** 2) / np.sum(np.abs(g64) ** 2)))
+    de = abs(o32[2] - o64[2]) / abs(o64[2])
+    print(f"mixed vs f64 after {w.nsteps} steps: L2 psi {l2:.3e}, rel energy {de:.3e}")
+    assert de < 1e-6
+    assert l2 < 1e-4
