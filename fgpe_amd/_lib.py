@@ -144,17 +144,31 @@ class Context:
         self.ncomp = ncomp
 
     def upload_psi(self, psi, comp=0):
-        a = np.ascontiguousarray(psi, dtype=np.complex128).reshape(-1)
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        if a.size == 2 * self.N and comp == 0:   # both components at once
+            for c in range(2):
+                self.upload_psi(a.reshape(2, -1)[c], c)
+            return
+        a = a.reshape(-1)
         assert a.size == self.N
         check(load().fgpe_upload_psi(self.h, comp, _dptr(a)))
 
-    def download_psi(self, comp=0):
+    def download_psi(self, comp=None):
+        if comp is None:
+            if self.ncomp == 2:
+                return np.stack([self.download_psi(0), self.download_psi(1)])
+            comp = 0
         out = np.empty(self.shape, dtype=np.complex128)
         check(load().fgpe_download_psi(self.h, comp, _dptr(out)))
         return out
 
     def upload_vext(self, V, comp=0):
-        a = np.ascontiguousarray(V, dtype=np.complex128).reshape(-1)
+        a = np.ascontiguousarray(V, dtype=np.complex128)
+        if a.size == 2 * self.N and comp == 0:
+            for c in range(2):
+                self.upload_vext(a.reshape(2, -1)[c], c)
+            return
+        a = a.reshape(-1)
         assert a.size == self.N
         check(load().fgpe_upload_vext(self.h, comp, _dptr(a)))
 
@@ -193,7 +207,7 @@ class Context:
 
     def imag_steps(self, dt, nsteps, scheme=FGPE_IMAG_RK4_RENORM, beta=0.0, npart=1.0, eps=0.0,
                    history=False):
-        hist = np.zeros((max(nsteps, 1), 3)) if history else None
+        hist = np.zeros((max(nsteps, 1), 3 * self.ncomp)) if history else None
         done = C.c_int(0)
         check(load().fgpe_imag_steps(self.h, scheme, dt, beta, npart, eps, nsteps,
                                      _dptr(hist) if history else _dp(), C.byref(done)))

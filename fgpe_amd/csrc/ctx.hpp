@@ -1,0 +1,123 @@
+// ctx.hpp -- the context behind fgpe_ctx* (device buffers, twiddle tables,
+// stream, launch timing) and the per-precision launcher interface shared by
+// the translation units of libfgpe.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fgpe_c.h"
+#include "gpe_kernels.hpp"
+
+namespace fgpe {
+
+int fail(int code, const std::string& msg);
+
+#define FGPE_HIPCHK(expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return ::fgpe::fail(FGPE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define FGPE_LAUNCHCHK(what)                                                                  \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            return ::fgpe::fail(FGPE_EHIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define FGPE_RET(expr)                    \
+    do {                                  \
+        int r_ = (expr);                  \
+        if (r_ != FGPE_OK) return r_;     \
+    } while (0)
+
+// Pencil lengths with a compiled plan.
+#define FGPE_FOR_EACH_N(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
+
+// Scalar slots of fgpe_ctx::scal (doubles) and ::flags (ints) on the device.
+enum {
+    SC_SCALE = 0,   // [2] pending renormalisation per component
+    SC_MU = 2,      // [2] heavy-ball mu of the previous iteration
+    SC_EPREV = 4,   // [2] heavy-ball E of the previous iteration
+    SC_OBS = 6,     // [6] observables (norm, mu, E) per component
+    SC_TMP = 12,    // [3] scratch output
+    SC_COUNT = 16
+};
+enum { FL_STOP = 0, FL_DONE = 1, FL_COUNT = 4 };
+
+}  // namespace fgpe
+
+struct fgpe_ctx {
+    int n[3] = {1, 1, 1};
+    long long ntot = 0;
+    int prec = FGPE_PREC_F64;
+    hipStream_t own = nullptr, stream = nullptr;
+    size_t bytes = 0;
+
+    double* k[3] = {nullptr, nullptr, nullptr};
+    double dv = 0.0;
+    bool kset = false;
+
+    fgpe_hparams hp{};
+    bool hset = false;
+    int nch = 0;
+
+    double2* P0[2] = {nullptr, nullptr};
+    void* PS[2] = {nullptr, nullptr};
+    void* K[2] = {nullptr, nullptr};
+    void* Vr[2] = {nullptr, nullptr};
+    void* Vi[2] = {nullptr, nullptr};
+    bool vcomplex[2] = {false, false};
+    void* spec = nullptr;
+    void* wtab = nullptr;
+    double* partials = nullptr;
+    size_t partials_n = 0;
+    double* scal = nullptr;
+    int* flags = nullptr;
+    double* hist = nullptr;
+    size_t hist_n = 0;
+    bool spec_valid = false;
+
+    void* tw64[3] = {nullptr, nullptr, nullptr};
+    void* tw32[3] = {nullptr, nullptr, nullptr};
+
+    // launch timing (fgpe_timing_enable)
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    std::vector<std::pair<int, size_t>> evrec;
+
+    bool mixed() const { return prec == FGPE_PREC_MIXED; }
+    size_t tsz() const { return mixed() ? 4 : 8; }
+};
+
+namespace fgpe {
+
+// ---- launch timing: HIP events around each launch when enabled -------------
+long t_begin(fgpe_ctx* c);
+void t_end(fgpe_ctx* c, long k, int cls);
+struct TScope {
+    fgpe_ctx* c;
+    int cls;
+    long k;
+    TScope(fgpe_ctx* c_, int cls_) : c(c_), cls(cls_), k(t_begin(c_)) {}
+    ~TScope() { t_end(c, k, cls); }
+};
+
+// ---- per-precision launchers (launch_f64.hip / launch_f32.hip) --------------
+// axis: 1 = y, 2 = z.  nch channels of ntot points at ctx->spec (or `data`).
+template <typename T> struct Launch {
+    static long long xblocks(fgpe_ctx* c);
+    static int fft_x(fgpe_ctx* c, const void* in, void* out, int dir);
+    static int strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp, const double* scale,
+                       const int* stop);
+    static int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter,
+                    const int* stop);
+    static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode);
+};
+
+}  // namespace fgpe

@@ -1,20 +1,70 @@
-// fgpe_capi.hip -- the C ABI of include/fgpe_c.h: context (device buffers,
-// twiddle tables, stream), launch dispatch over the supported pencil lengths,
-// and the time-stepping drivers.  gfx950 only.
+// fgpe_capi.hip -- the C ABI of include/fgpe_c.h: context set-up and the
+// time-stepping drivers (real-time RK4, imaginary-time RK4 with
+// renormalisation, heavy-ball imaginary time, H application, observables).
+// The pass kernels are launched through fgpe::Launch<T> (launch_f64.hip,
+// launch_f32.hip).  gfx950 only.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
-#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "../../include/fgpe_c.h"
-#include "gpe_kernels.hpp"
+#include "ctx.hpp"
 
 using namespace fgpe;
 
-namespace {
+namespace fgpe {
+
+__global__ void __launch_bounds__(256) k_finalize(FinArgs f)
+{
+    __shared__ double sh[6][256];
+    if (f.stop && *f.stop) return;
+    const int t = threadIdx.x;
+    const int nv = 3 * f.nc;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = t; b < f.nblocks; b += 256)
+        for (int q = 0; q < nv; ++q) s[q] += f.partials[(long long)b * nv + q];
+    for (int q = 0; q < 6; ++q) sh[q][t] = s[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 6; ++q) sh[q][t] += sh[q][t + w];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    bool conv = true;
+    for (int c = 0; c < f.nc; ++c) {
+        const double norm = sh[3 * c][0] * f.dv;
+        if (f.kind == FIN_OBS) {
+            f.out[3 * c] = norm;
+            f.out[3 * c + 1] = sh[3 * c + 1][0] * f.dv / norm;
+            f.out[3 * c + 2] = sh[3 * c + 2][0] * f.dv / norm;
+        } else if (f.kind == FIN_RENORM) {
+            f.out[3 * c] = norm;
+            f.scale[c] = sqrt(f.npart[c] / norm);
+        } else {
+            const double sc = sqrt(f.npart[c] / norm);
+            const double narg = f.nc == 1 ? f.npart[c] : norm;
+            const double mu = sc * sh[3 * c + 1][0] * f.dv / narg;
+            const double en = sc * sh[3 * c + 2][0] * f.dv / narg;
+            f.out[3 * c] = norm;
+            f.out[3 * c + 1] = mu;
+            f.out[3 * c + 2] = en;
+            f.scale[c] = sc;
+            f.mu[c] = mu;
+            const double ep = f.eprev[c];
+            if (f.nc == 1) conv = conv && (fabs(en - ep) / ep < f.eps);
+            else conv = conv && (fabs(en - ep) < f.eps);
+            f.eprev[c] = en;
+        }
+    }
+    if (f.kind == FIN_HB) {
+        f.done[0] += 1;
+        if (conv) f.stop[0] = 1;
+    }
+}
+
 
 thread_local std::string g_err;
 
@@ -24,21 +74,25 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
-#define HIPCHK(expr)                                                                   \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(FGPE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+long t_begin(fgpe_ctx* c)
+{
+    if (!c->timing || c->evused + 2 > c->evpool.size()) return -1;
+    const size_t k = c->evused;
+    c->evused += 2;
+    (void)hipEventRecord(c->evpool[k], c->stream);
+    return (long)k;
+}
 
-#define LAUNCHCHK(what)                                                                \
-    do {                                                                               \
-        hipError_t e_ = hipGetLastError();                                             \
-        if (e_ != hipSuccess)                                                          \
-            return fail(FGPE_EHIP, std::string(what) + ": " + hipGetErrorString(e_));  \
-    } while (0)
+void t_end(fgpe_ctx* c, long k, int cls)
+{
+    if (k < 0) return;
+    (void)hipEventRecord(c->evpool[k + 1], c->stream);
+    c->evrec.emplace_back(cls, (size_t)k);
+}
 
-#define FGPE_FOR_EACH_N(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
+}  // namespace fgpe
+
+namespace {
 
 bool supported_n(int n)
 {
@@ -64,55 +118,6 @@ template <typename T> std::vector<c2_t<T>> twiddle_table(int n)
     return w;
 }
 
-}  // namespace
-
-struct fgpe_ctx {
-    int n[3] = {1, 1, 1};
-    long long ntot = 0;
-    int prec = FGPE_PREC_F64;
-    hipStream_t own = nullptr, stream = nullptr;
-    size_t bytes = 0;
-
-    double* k[3] = {nullptr, nullptr, nullptr};
-    double dv = 0.0;
-    bool kset = false;
-
-    fgpe_hparams hp{};
-    bool hset = false;
-    int nch = 0;
-
-    double2* P0[2] = {nullptr, nullptr};
-    void* PS[2] = {nullptr, nullptr};
-    void* K[2] = {nullptr, nullptr};
-    void* Vr[2] = {nullptr, nullptr};
-    void* Vi[2] = {nullptr, nullptr};
-    bool vcomplex[2] = {false, false};
-    void* spec = nullptr;
-    void* wtab = nullptr;
-    double* partials = nullptr;
-    size_t partials_n = 0;
-    double* scal = nullptr;   // [0] scale, [1..15] observables scratch
-    double* hist = nullptr;
-    size_t hist_n = 0;
-    bool spec_valid = false;
-
-    void* tw64[3] = {nullptr, nullptr, nullptr};
-    void* tw32[3] = {nullptr, nullptr, nullptr};
-
-    // launch timing (fgpe_timing_enable)
-    bool timing = false;
-    std::vector<hipEvent_t> evpool;
-    size_t evused = 0;
-    std::vector<std::pair<int, size_t>> evrec;
-
-    bool mixed() const { return prec == FGPE_PREC_MIXED; }
-    size_t tsz() const { return mixed() ? 4 : 8; }
-};
-
-namespace {
-
-template <typename T> void* tw_of(fgpe_ctx* c, int axis) { return sizeof(T) == 8 ? c->tw64[axis] : c->tw32[axis]; }
-
 int dalloc(fgpe_ctx* c, void** p, size_t bytes)
 {
     hipError_t e = hipMalloc(p, bytes);
@@ -124,225 +129,22 @@ int dalloc(fgpe_ctx* c, void** p, size_t bytes)
     return FGPE_OK;
 }
 
-// ---- launch timing --------------------------------------------------------
-// Returns the event-pair slot to close with t_end, or -1 when not timing.
-long t_begin(fgpe_ctx* c)
+void dfree(fgpe_ctx* c, void** p, size_t bytes)
 {
-    if (!c->timing || c->evused + 2 > c->evpool.size()) return -1;
-    const size_t k = c->evused;
-    c->evused += 2;
-    (void)hipEventRecord(c->evpool[k], c->stream);
-    return (long)k;
-}
-void t_end(fgpe_ctx* c, long k, int cls)
-{
-    if (k < 0) return;
-    (void)hipEventRecord(c->evpool[k + 1], c->stream);
-    c->evrec.emplace_back(cls, (size_t)k);
-}
-struct TScope {
-    fgpe_ctx* c;
-    int cls;
-    long k;
-    TScope(fgpe_ctx* c_, int cls_) : c(c_), cls(cls_), k(t_begin(c_)) {}
-    ~TScope() { t_end(c, k, cls); }
-};
-
-// ---- launchers ---------------------------------------------------------
-template <typename T, int N> int launch_fft_x(fgpe_ctx* c, const void* in, void* out, int dir)
-{
-    constexpr int P = px_of<T, N>();
-    const long long rows = c->ntot / N;
-    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
-    using T2 = c2_t<T>;
-    const T2* tw = (const T2*)tw_of<T>(c, 0);
-    TScope ts(c, FGPE_KCLASS_FFTX);
-    if (dir < 0) hipLaunchKernelGGL((k_fft_x<T, N, -1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out, tw);
-    else hipLaunchKernelGGL((k_fft_x<T, N, +1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out, tw);
-    LAUNCHCHK("k_fft_x");
-    return FGPE_OK;
-}
-
-StridedArgs strided_args(fgpe_ctx* c, int axis, const double* scale, int ncomp)
-{
-    StridedArgs a;
-    a.nx = c->n[0];
-    a.chan_stride = c->ntot;
-    a.scale = scale;
-    a.ncomp = ncomp;
-    if (axis == 1) {
-        a.stride = c->n[0];
-        a.outer_stride = (long long)c->n[0] * c->n[1];
-    } else {
-        a.stride = (long long)c->n[0] * c->n[1];
-        a.outer_stride = c->n[0];
+    if (*p) {
+        (void)hipFree(*p);
+        c->bytes -= bytes;
+        *p = nullptr;
     }
-    return a;
-}
-
-long long n_outer(fgpe_ctx* c, int axis) { return axis == 1 ? c->n[2] : c->n[1]; }
-
-template <typename T, int N> int launch_strided(fgpe_ctx* c, void* data, int axis, int dir, int nch,
-                                                const double* scale, int ncomp)
-{
-    constexpr int P = ps_of<T, N>();
-    if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    StridedArgs a = strided_args(c, axis, scale, ncomp);
-    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N>::T);
-    using T2 = c2_t<T>;
-    const T2* tw = (const T2*)tw_of<T>(c, axis);
-    TScope ts(c, FGPE_KCLASS_STRIDED);
-    if (dir < 0) hipLaunchKernelGGL((k_strided<T, N, -1>), grid, block, 0, c->stream, (T2*)data, tw, a);
-    else hipLaunchKernelGGL((k_strided<T, N, +1>), grid, block, 0, c->stream, (T2*)data, tw, a);
-    LAUNCHCHK("k_strided");
-    return FGPE_OK;
-}
-
-template <typename T, int N> int launch_last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
-                                             const double* scale, bool filter)
-{
-    constexpr int P = ps_of<T, N>();
-    if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    LastArgs a;
-    a.s = strided_args(c, axis, scale, ncomp);
-    const int oaxis = axis == 1 ? 2 : 1;
-    a.kA = c->k[axis];
-    a.kX = c->k[0];
-    a.kO = c->k[oaxis];
-    const double* d = c->hp.dip;
-    a.dA = d[axis];
-    a.dX = d[0];
-    a.dO = d[oaxis];
-    a.lap_scale = 19.739208802 / (double)c->ntot;
-    a.inv_ntot = 1.0 / (double)c->ntot;
-    a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
-    a.wtab = filter ? c->wtab : nullptr;
-    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N>::T);
-    using T2 = c2_t<T>;
-    TScope ts(c, FGPE_KCLASS_LAST);
-    hipLaunchKernelGGL((k_last<T, N>), grid, block, 0, c->stream, (T2*)data, (const T2*)tw_of<T>(c, axis), a);
-    LAUNCHCHK("k_last");
-    return FGPE_OK;
-}
-
-template <typename T, int N, bool DENS, int MODE> int launch_xstage_t(fgpe_ctx* c, const XArgs<T>& a)
-{
-    constexpr int P = px_of<T, N>();
-    const long long rows = c->ntot / N;
-    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
-    TScope ts(c, FGPE_KCLASS_XSTAGE);
-    hipLaunchKernelGGL((k_xstage<T, N, DENS, MODE>), grid, block, 0, c->stream, a,
-                       (const c2_t<T>*)tw_of<T>(c, 0));
-    LAUNCHCHK("k_xstage");
-    return FGPE_OK;
-}
-
-template <typename T, int N> int launch_xstage(fgpe_ctx* c, const XArgs<T>& a, bool dens, int mode)
-{
-    if (dens) {
-        if (mode == MODE_PROLOGUE) return launch_xstage_t<T, N, true, MODE_PROLOGUE>(c, a);
-        if (mode == MODE_RK) return launch_xstage_t<T, N, true, MODE_RK>(c, a);
-        return launch_xstage_t<T, N, true, MODE_APPLY>(c, a);
-    }
-    if (mode == MODE_PROLOGUE) return launch_xstage_t<T, N, false, MODE_PROLOGUE>(c, a);
-    if (mode == MODE_RK) return launch_xstage_t<T, N, false, MODE_RK>(c, a);
-    return launch_xstage_t<T, N, false, MODE_APPLY>(c, a);
-}
-
-template <typename T> long long xblocks(fgpe_ctx* c)
-{
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return c->ntot / N_ / px_of<T, N_>();
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
-    return 0;
-}
-
-// ---- runtime dispatch over the axis length ---------------------------------
-template <typename T> int fft_x(fgpe_ctx* c, const void* in, void* out, int dir)
-{
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return launch_fft_x<T, N_>(c, in, out, dir);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
-    return fail(FGPE_EUNSUP, "unsupported nx");
-}
-template <typename T> int strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, const double* scale, int ncomp)
-{
-    switch (c->n[axis]) {
-#define CASE(N_) \
-    case N_: return launch_strided<T, N_>(c, data, axis, dir, nch, scale, ncomp);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
-    return fail(FGPE_EUNSUP, "unsupported strided axis length");
-}
-template <typename T> int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter)
-{
-    switch (c->n[axis]) {
-#define CASE(N_) \
-    case N_: return launch_last<T, N_>(c, data, axis, nch, ncomp, scale, filter);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
-    return fail(FGPE_EUNSUP, "unsupported last axis length");
-}
-template <typename T> int xstage(fgpe_ctx* c, const XArgs<T>& a, bool dens, int mode)
-{
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return launch_xstage<T, N_>(c, a, dens, mode);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
-    return fail(FGPE_EUNSUP, "unsupported nx");
 }
 
 int last_axis(fgpe_ctx* c) { return c->n[2] > 1 ? 2 : 1; }
-
-#define RET(expr)                 \
-    do {                          \
-        int r_ = (expr);          \
-        if (r_ != FGPE_OK) return r_; \
-    } while (0)
-
-// Plain 3D transform on a device array (standalone passes)
-template <typename T> int fft3d(fgpe_ctx* c, const void* in, void* out, int dir)
-{
-    RET(fft_x<T>(c, in, out, dir));
-    if (c->n[1] > 1) RET(strided<T>(c, out, 1, dir, 1, nullptr, 1));
-    if (c->n[2] > 1) RET(strided<T>(c, out, 2, dir, 1, nullptr, 1));
-    return FGPE_OK;
-}
-
-// y-forward / last axis / y-inverse on the spectral channels.  `scale` is
-// applied by the first forward pass after a prologue or stage 4.
-template <typename T> int spectral_passes(fgpe_ctx* c, void* spec, int nch, int ncomp, const double* scale,
-                                          bool filter)
-{
-    const int la = last_axis(c);
-    if (la == 2) {
-        RET(strided<T>(c, spec, 1, -1, nch, scale, ncomp));
-        RET(last<T>(c, spec, 2, nch, ncomp, nullptr, filter));
-        RET(strided<T>(c, spec, 1, +1, nch, nullptr, ncomp));
-    } else {
-        RET(last<T>(c, spec, 1, nch, ncomp, scale, filter));
-    }
-    return FGPE_OK;
-}
-
 bool has_filter(const fgpe_hparams& h) { return h.a_dd != 0.0 || h.sc_U != 0.0; }
 
+// d_quench_as, hamiltonians/BEC_oneComponent.cu:167-178
 double a_at(const fgpe_hparams& h, double t)
 {
     if (!h.quench_on) return h.a[0];
-    // d_quench_as, hamiltonians/BEC_oneComponent.cu:167-178
     const double a0 = h.quench[0], a1 = h.quench[1], a2 = h.quench[2];
     const double t1 = h.quench[3], t2 = h.quench[4], t3 = h.quench[5];
     if (0 < t && t <= t1) return a0 + (a1 - a0) / (t1) * t;
@@ -351,16 +153,56 @@ double a_at(const fgpe_hparams& h, double t)
     return a2;
 }
 
-template <typename T> XArgs<T> base_xargs(fgpe_ctx* c)
+// Plain 3D transform on a device array (standalone passes)
+template <typename T> int fft3d(fgpe_ctx* c, const void* in, void* out, int dir)
+{
+    FGPE_RET(Launch<T>::fft_x(c, in, out, dir));
+    if (c->n[1] > 1) FGPE_RET(Launch<T>::strided(c, out, 1, dir, 1, 1, nullptr, nullptr));
+    if (c->n[2] > 1) FGPE_RET(Launch<T>::strided(c, out, 2, dir, 1, 1, nullptr, nullptr));
+    return FGPE_OK;
+}
+
+// y-forward / last axis / y-inverse on the spectral channels.  `scale` is
+// applied by the first forward pass after a prologue or a final stage.
+template <typename T> int spectral_passes(fgpe_ctx* c, const double* scale, const int* stop)
+{
+    const bool filter = has_filter(c->hp);
+    const int nc = c->hp.ncomp;
+    if (last_axis(c) == 2) {
+        FGPE_RET(Launch<T>::strided(c, c->spec, 1, -1, c->nch, nc, scale, stop));
+        FGPE_RET(Launch<T>::last(c, c->spec, 2, c->nch, nc, nullptr, filter, stop));
+        FGPE_RET(Launch<T>::strided(c, c->spec, 1, +1, c->nch, nc, nullptr, stop));
+    } else {
+        FGPE_RET(Launch<T>::last(c, c->spec, 1, c->nch, nc, scale, filter, stop));
+    }
+    return FGPE_OK;
+}
+
+template <typename T> XArgs<T> base_xargs(fgpe_ctx* c, double t)
 {
     XArgs<T> a{};
     a.spec = (c2_t<T>*)c->spec;
     a.ntot = c->ntot;
-    a.P0 = c->P0[0];
-    a.PS = (c2_t<T>*)c->PS[0];
-    a.K = (c2_t<T>*)c->K[0];
-    a.Vr = (const T*)c->Vr[0];
-    a.Vi = c->vcomplex[0] ? (const T*)c->Vi[0] : nullptr;
+    for (int i = 0; i < 2; ++i) {
+        a.P0[i] = c->P0[i];
+        a.PS[i] = (c2_t<T>*)c->PS[i];
+        a.K[i] = (c2_t<T>*)c->K[i];
+        a.Vr[i] = (const T*)c->Vr[i];
+        a.Vi[i] = c->vcomplex[i] ? (const T*)c->Vi[i] : nullptr;
+    }
+    const fgpe_hparams& h = c->hp;
+    if (h.ncomp == 1) {
+        const double av = a_at(h, t);
+        a.g4[0][0] = 4 * kPi * av;
+        a.g2[0][0] = 2 * kPi * av;
+    } else {
+        const double aa[2][2] = {{h.a[0], h.a[2]}, {h.a[2], h.a[1]}};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                a.g4[i][j] = 4 * kPi * aa[i][j];
+                a.g2[i][j] = 2 * kPi * aa[i][j];
+            }
+    }
     a.partials = c->partials;
     return a;
 }
@@ -369,17 +211,41 @@ int ensure_ready(fgpe_ctx* c)
 {
     if (!c->kset) return fail(FGPE_ESTATE, "fgpe_set_kspace has not been called");
     if (!c->hset) return fail(FGPE_ESTATE, "fgpe_set_hamiltonian has not been called");
-    if (!c->Vr[0]) return fail(FGPE_ESTATE, "no external potential uploaded");
-    if (c->hp.ncomp != 1) return fail(FGPE_EUNSUP, "two-component stepping is not available in this build");
+    for (int i = 0; i < c->hp.ncomp; ++i) {
+        if (!c->Vr[i]) return fail(FGPE_ESTATE, "no external potential uploaded for a component");
+        if (!c->P0[i]) return fail(FGPE_ESTATE, "no state uploaded for a component");
+    }
     if (c->n[1] == 1) return fail(FGPE_EUNSUP, "1D grids are not supported");
     return FGPE_OK;
 }
 
 template <typename T> int prologue(fgpe_ctx* c)
 {
-    XArgs<T> a = base_xargs<T>(c);
-    RET(xstage<T>(c, a, c->nch > c->hp.ncomp, MODE_PROLOGUE));
+    XArgs<T> a = base_xargs<T>(c, 0.0);
+    FGPE_RET(Launch<T>::xstage(c, a, c->hp.ncomp, c->nch > c->hp.ncomp, MODE_PROLOGUE));
     c->spec_valid = true;
+    return FGPE_OK;
+}
+
+int finalize(fgpe_ctx* c, long long nblocks, int kind, double npart, double* out, double eps)
+{
+    FinArgs f{};
+    f.partials = c->partials;
+    f.nblocks = (int)nblocks;
+    f.nc = c->hp.ncomp;
+    f.kind = kind;
+    f.dv = c->dv;
+    f.npart[0] = f.npart[1] = npart;
+    f.out = out;
+    f.scale = c->scal + SC_SCALE;
+    f.mu = c->scal + SC_MU;
+    f.eprev = c->scal + SC_EPREV;
+    f.stop = kind == FIN_HB ? c->flags + FL_STOP : nullptr;
+    f.done = c->flags + FL_DONE;
+    f.eps = eps;
+    TScope ts(c, FGPE_KCLASS_OTHER);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, f);
+    FGPE_LAUNCHCHK("k_finalize");
     return FGPE_OK;
 }
 
@@ -387,73 +253,137 @@ template <typename T> int prologue(fgpe_ctx* c)
 template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps, bool imaginary, double npart,
                                   double* hist_dev)
 {
-    RET(ensure_ready(c));
-    const bool dens = c->nch > c->hp.ncomp;
-    const bool filter = has_filter(c->hp);
-    if (!c->spec_valid) RET(prologue<T>(c));
-    const double* scale = imaginary ? c->scal : nullptr;
-    const long long nb = xblocks<T>(c);
+    FGPE_RET(ensure_ready(c));
+    const int nc = c->hp.ncomp;
+    const bool dens = c->nch > nc;
+    if (imaginary && nc == 2 && dens)
+        return fail(FGPE_EUNSUP, "imaginary time with two components and a k-space kernel is not available");
+    if (!c->spec_valid) FGPE_RET(prologue<T>(c));
+    const double* scale = imaginary ? c->scal + SC_SCALE : nullptr;
+    const long long nb = Launch<T>::xblocks(c);
     for (int it = 0; it < nsteps; ++it) {
         for (int s = 1; s <= 4; ++s) {
-            const double frac = s == 1 ? 0.0 : (s == 4 ? 1.0 : 0.5);
             // PARS[T] at each stage (src/oneComponentGPSolver.cu:175,184)
+            const double frac = s == 1 ? 0.0 : (s == 4 ? 1.0 : 0.5);
             const double t = s == 1 ? t0 + it * dt : t0 + (it + frac) * dt;
-            RET(spectral_passes<T>(c, c->spec, c->nch, c->hp.ncomp, s == 1 ? scale : nullptr, filter));
-            XArgs<T> a = base_xargs<T>(c);
-            const double av = a_at(c->hp, t);
-            a.g4 = 4 * kPi * av;
-            a.g2 = 2 * kPi * av;
+            FGPE_RET(spectral_passes<T>(c, s == 1 ? scale : nullptr, nullptr));
+            XArgs<T> a = base_xargs<T>(c, t);
             const double cf = (s <= 2) ? -0.5 * dt : -1.0 * dt;
             if (imaginary) { a.cr = cf; a.ci = 0.0; }
             else { a.cr = 0.0; a.ci = cf; }
             a.stage = s;
             a.scale = scale;
             a.want_norm = (imaginary && s == 4) ? 1 : 0;
-            RET(xstage<T>(c, a, dens, MODE_RK));
-            if (imaginary && s == 4) {
-                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partials, (int)nb, 1,
-                                   (int)FIN_RENORM, c->dv, npart, hist_dev ? hist_dev + 3 * it : c->scal + 8,
-                                   c->scal);
-                LAUNCHCHK("k_finalize");
-            }
+            FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_RK));
+            if (imaginary && s == 4)
+                FGPE_RET(finalize(c, nb, FIN_RENORM, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP,
+                                  0.0));
         }
+    }
+    return FGPE_OK;
+}
+
+__global__ void k_hb_init(double* scal, int* flags)
+{
+    for (int i = 0; i < 2; ++i) {
+        scal[SC_SCALE + i] = 1.0;
+        scal[SC_MU + i] = 0.0;
+        scal[SC_EPREV + i] = 0.0;
+    }
+    flags[FL_STOP] = 0;
+    flags[FL_DONE] = 0;
+}
+
+// Heavy-ball imaginary time (the reference's scheme), fully on the device: the
+// convergence test runs in k_finalize and stops every later launch.
+template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, double npart, double eps, int nsteps,
+                                 double* hist_dev)
+{
+    FGPE_RET(ensure_ready(c));
+    const int nc = c->hp.ncomp;
+    const bool dens = c->nch > nc;
+    if (nc == 2 && dens)
+        return fail(FGPE_EUNSUP, "imaginary time with two components and a k-space kernel is not available");
+    hipLaunchKernelGGL(k_hb_init, dim3(1), dim3(1), 0, c->stream, c->scal, c->flags);
+    FGPE_LAUNCHCHK("k_hb_init");
+    // old = psi (src/oneComponentGPSolver.cu:33)
+    for (int i = 0; i < nc; ++i) {
+        if (sizeof(T) == 8) {
+            FGPE_HIPCHK(hipMemcpyAsync(c->PS[i], c->P0[i], sizeof(double2) * c->ntot, hipMemcpyDeviceToDevice,
+                                       c->stream));
+        } else {
+            const long long n = 2 * c->ntot;
+            hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                               c->stream, (float*)c->PS[i], (const double*)c->P0[i], n);
+            FGPE_LAUNCHCHK("k_convert");
+        }
+    }
+    FGPE_RET(prologue<T>(c));
+    const long long nb = Launch<T>::xblocks(c);
+    const int* stop = c->flags + FL_STOP;
+    for (int it = 0; it < nsteps; ++it) {
+        FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, stop));
+        XArgs<T> a = base_xargs<T>(c, it * dt);
+        a.scale = c->scal + SC_SCALE;
+        a.dt = dt;
+        a.beta = beta;
+        a.mu = nc == 2 ? c->scal + SC_MU : nullptr;
+        a.stop = stop;
+        FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_HB));
+        FGPE_RET(finalize(c, nb, FIN_HB, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP, eps));
     }
     return FGPE_OK;
 }
 
 template <typename T> int apply_h(fgpe_ctx* c, double t, void* hmu, void* hen, double* obs_dev)
 {
-    RET(ensure_ready(c));
-    RET(prologue<T>(c));
-    RET(spectral_passes<T>(c, c->spec, c->nch, c->hp.ncomp, c->scal, has_filter(c->hp)));
-    XArgs<T> a = base_xargs<T>(c);
-    const double av = a_at(c->hp, t);
-    a.g4 = 4 * kPi * av;
-    a.g2 = 2 * kPi * av;
-    a.scale = c->scal;
+    FGPE_RET(ensure_ready(c));
+    FGPE_RET(prologue<T>(c));
+    FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, nullptr));
+    XArgs<T> a = base_xargs<T>(c, t);
+    a.scale = c->scal + SC_SCALE;
     a.hmu = (double2*)hmu;
     a.hen = (double2*)hen;
-    RET(xstage<T>(c, a, c->nch > c->hp.ncomp, MODE_APPLY));
+    FGPE_RET(Launch<T>::xstage(c, a, c->hp.ncomp, c->nch > c->hp.ncomp, MODE_APPLY));
     c->spec_valid = false;
-    if (obs_dev) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partials, (int)xblocks<T>(c), 3,
-                           (int)FIN_OBS, c->dv, 1.0, obs_dev, c->scal);
-        LAUNCHCHK("k_finalize");
-    }
+    if (obs_dev) FGPE_RET(finalize(c, Launch<T>::xblocks(c), FIN_OBS, 1.0, obs_dev, 0.0));
     return FGPE_OK;
 }
+
+__global__ void k_apply_scales(double2* __restrict__ p0, double2* __restrict__ p1, long long n, double* scale)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double s0 = scale[0];
+        p0[i] = double2{s0 * p0[i].x, s0 * p0[i].y};
+        if (p1) {
+            const double s1 = scale[1];
+            p1[i] = double2{s1 * p1[i].x, s1 * p1[i].y};
+        }
+    }
+}
+__global__ void k_reset_scales(double* scale) { scale[0] = scale[1] = 1.0; }
 
 int apply_pending_scale(fgpe_ctx* c)
 {
     const int th = 256;
-    for (int comp = 0; comp < c->hp.ncomp; ++comp) {
-        hipLaunchKernelGGL(k_apply_scale, dim3((unsigned)((c->ntot + th - 1) / th)), dim3(th), 0, c->stream,
-                           c->P0[comp], c->ntot, c->scal);
-        LAUNCHCHK("k_apply_scale");
-    }
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, c->scal, 1.0);
-    LAUNCHCHK("k_set_scalar");
+    hipLaunchKernelGGL(k_apply_scales, dim3((unsigned)((c->ntot + th - 1) / th)), dim3(th), 0, c->stream,
+                       c->P0[0], c->hp.ncomp == 2 ? c->P0[1] : nullptr, c->ntot, c->scal + SC_SCALE);
+    FGPE_LAUNCHCHK("k_apply_scales");
+    hipLaunchKernelGGL(k_reset_scales, dim3(1), dim3(1), 0, c->stream, c->scal + SC_SCALE);
+    FGPE_LAUNCHCHK("k_reset_scales");
     c->spec_valid = false;
+    return FGPE_OK;
+}
+
+int ensure_hist(fgpe_ctx* c, size_t n)
+{
+    if (n > c->hist_n) {
+        dfree(c, (void**)&c->hist, sizeof(double) * c->hist_n);
+        c->hist_n = 0;
+        FGPE_RET(dalloc(c, (void**)&c->hist, sizeof(double) * n));
+        c->hist_n = n;
+    }
     return FGPE_OK;
 }
 
@@ -473,6 +403,7 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     if (nx <= 1 || ny < 1 || nz < 1) return fail(FGPE_EINVAL, "grid extents must be positive (nx > 1)");
     if (!supported_n(nx) || (ny > 1 && !supported_n(ny)) || (nz > 1 && !supported_n(nz)))
         return fail(FGPE_EUNSUP, "grid extents must be 1 or one of 32, 64, 96, 128, 192, 256, 512");
+    if (ny == 1 && nz > 1) return fail(FGPE_EUNSUP, "a grid with ny = 1 must have nz = 1");
     fgpe_ctx* c = new fgpe_ctx();
     c->n[0] = nx;
     c->n[1] = ny;
@@ -502,14 +433,17 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     if (r == FGPE_OK) r = dalloc(c, (void**)&c->P0[0], sizeof(double2) * N);
     if (r == FGPE_OK) r = dalloc(c, &c->PS[0], 2 * c->tsz() * N);
     if (r == FGPE_OK) r = dalloc(c, &c->K[0], 2 * c->tsz() * N);
-    if (r == FGPE_OK) r = dalloc(c, (void**)&c->scal, sizeof(double) * 16);
+    if (r == FGPE_OK) r = dalloc(c, (void**)&c->scal, sizeof(double) * SC_COUNT);
+    if (r == FGPE_OK) r = dalloc(c, (void**)&c->flags, sizeof(int) * FL_COUNT);
     if (r == FGPE_OK) {
-        c->partials_n = 3 * (size_t)(c->ntot / 2 + 1024);
+        c->partials_n = 6 * (size_t)(c->ntot / 32 + 1024);
         r = dalloc(c, (void**)&c->partials, sizeof(double) * c->partials_n);
     }
     if (r == FGPE_OK) {
-        double one[16] = {1.0};
-        if (hipMemcpy(c->scal, one, sizeof(one), hipMemcpyHostToDevice) != hipSuccess)
+        double init[SC_COUNT] = {1.0, 1.0};
+        int fl[FL_COUNT] = {0};
+        if (hipMemcpy(c->scal, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->flags, fl, sizeof(fl), hipMemcpyHostToDevice) != hipSuccess)
             r = fail(FGPE_EHIP, "scalar init failed");
     }
     if (r != FGPE_OK) {
@@ -527,7 +461,7 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     (void)hipDeviceSynchronize();
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
-                    c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->hist};
+                    c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -546,7 +480,7 @@ int fgpe_ctx_set_stream(fgpe_ctx* c, void* s)
 int fgpe_ctx_sync(fgpe_ctx* c)
 {
     if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
-    HIPCHK(hipStreamSynchronize(c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     return FGPE_OK;
 }
 
@@ -562,9 +496,10 @@ int fgpe_set_kspace(fgpe_ctx* c, const double* kx, const double* ky, const doubl
     if (!c || !kx || !ky || !kz) return fail(FGPE_EINVAL, "NULL argument");
     const double* src[3] = {kx, ky, kz};
     for (int a = 0; a < 3; ++a)
-        HIPCHK(hipMemcpy(c->k[a], src[a], sizeof(double) * c->n[a], hipMemcpyHostToDevice));
+        FGPE_HIPCHK(hipMemcpy(c->k[a], src[a], sizeof(double) * c->n[a], hipMemcpyHostToDevice));
     c->dv = dv;
     c->kset = true;
+    c->spec_valid = false;
     return FGPE_OK;
 }
 
@@ -574,21 +509,22 @@ int fgpe_set_hamiltonian(fgpe_ctx* c, const fgpe_hparams* hp)
     if (hp->ncomp != 1 && hp->ncomp != 2) return fail(FGPE_EINVAL, "ncomp must be 1 or 2");
     if (hp->sc_U != 0.0 && (hp->sc_dim < 1 || hp->sc_dim > 3)) return fail(FGPE_EINVAL, "sc_dim must be 1, 2 or 3");
     if (hp->sc_U != 0.0 && !c->kset) return fail(FGPE_ESTATE, "set the k space before a soft-core term");
+    if (hp->ncomp == 2 && hp->quench_on) return fail(FGPE_EUNSUP, "the a(t) ramp is one-component only");
     c->hp = *hp;
     const int nch = hp->ncomp + (has_filter(*hp) ? 1 : 0);
     const size_t N = (size_t)c->ntot;
     if (nch != c->nch) {
-        if (c->spec) { (void)hipFree(c->spec); c->bytes -= 2 * c->tsz() * N * c->nch; c->spec = nullptr; }
-        RET(dalloc(c, &c->spec, 2 * c->tsz() * N * nch));
+        dfree(c, &c->spec, 2 * c->tsz() * N * c->nch);
+        FGPE_RET(dalloc(c, &c->spec, 2 * c->tsz() * N * nch));
         c->nch = nch;
     }
     if (hp->ncomp == 2) {
-        if (!c->P0[1]) RET(dalloc(c, (void**)&c->P0[1], sizeof(double2) * N));
-        if (!c->PS[1]) RET(dalloc(c, &c->PS[1], 2 * c->tsz() * N));
-        if (!c->K[1]) RET(dalloc(c, &c->K[1], 2 * c->tsz() * N));
+        if (!c->P0[1]) FGPE_RET(dalloc(c, (void**)&c->P0[1], sizeof(double2) * N));
+        if (!c->PS[1]) FGPE_RET(dalloc(c, &c->PS[1], 2 * c->tsz() * N));
+        if (!c->K[1]) FGPE_RET(dalloc(c, &c->K[1], 2 * c->tsz() * N));
     }
     if (hp->sc_U != 0.0) {
-        if (!c->wtab) RET(dalloc(c, &c->wtab, c->tsz() * N));
+        if (!c->wtab) FGPE_RET(dalloc(c, &c->wtab, c->tsz() * N));
         const int th = 256;
         dim3 grid((unsigned)((N + th - 1) / th));
         if (c->mixed())
@@ -597,11 +533,9 @@ int fgpe_set_hamiltonian(fgpe_ctx* c, const fgpe_hparams* hp)
         else
             hipLaunchKernelGGL(k_softcore_table<double>, grid, dim3(th), 0, c->stream, (double*)c->wtab, c->k[0],
                                c->k[1], c->k[2], c->n[0], c->n[1], c->ntot, hp->sc_dim, hp->sc_R, hp->sc_U);
-        LAUNCHCHK("k_softcore_table");
-    } else if (c->wtab) {
-        (void)hipFree(c->wtab);
-        c->bytes -= c->tsz() * N;
-        c->wtab = nullptr;
+        FGPE_LAUNCHCHK("k_softcore_table");
+    } else {
+        dfree(c, &c->wtab, c->tsz() * N);
     }
     c->hset = true;
     c->spec_valid = false;
@@ -612,10 +546,11 @@ int fgpe_upload_psi(fgpe_ctx* c, int comp, const double* host)
 {
     if (!c || !host) return fail(FGPE_EINVAL, "NULL argument");
     if (comp < 0 || comp > 1) return fail(FGPE_EINVAL, "comp must be 0 or 1");
-    if (!c->P0[comp]) RET(dalloc(c, (void**)&c->P0[comp], sizeof(double2) * c->ntot));
-    HIPCHK(hipMemcpyAsync(c->P0[comp], host, sizeof(double2) * c->ntot, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, c->scal, 1.0);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->P0[comp]) FGPE_RET(dalloc(c, (void**)&c->P0[comp], sizeof(double2) * c->ntot));
+    FGPE_HIPCHK(hipMemcpyAsync(c->P0[comp], host, sizeof(double2) * c->ntot, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_reset_scales, dim3(1), dim3(1), 0, c->stream, c->scal + SC_SCALE);
+    FGPE_LAUNCHCHK("k_reset_scales");
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     c->spec_valid = false;
     return FGPE_OK;
 }
@@ -624,8 +559,8 @@ int fgpe_download_psi(fgpe_ctx* c, int comp, double* host)
 {
     if (!c || !host) return fail(FGPE_EINVAL, "NULL argument");
     if (comp < 0 || comp > 1 || !c->P0[comp]) return fail(FGPE_EINVAL, "no such component");
-    HIPCHK(hipMemcpyAsync(host, c->P0[comp], sizeof(double2) * c->ntot, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    FGPE_HIPCHK(hipMemcpyAsync(host, c->P0[comp], sizeof(double2) * c->ntot, hipMemcpyDeviceToHost, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     return FGPE_OK;
 }
 
@@ -636,24 +571,24 @@ int fgpe_upload_vext(fgpe_ctx* c, int comp, const double* host)
     const size_t N = (size_t)c->ntot;
     bool cplx = false;
     for (size_t i = 0; i < N && !cplx; ++i) cplx = host[2 * i + 1] != 0.0;
-    if (!c->Vr[comp]) RET(dalloc(c, &c->Vr[comp], c->tsz() * N));
-    if (cplx && !c->Vi[comp]) RET(dalloc(c, &c->Vi[comp], c->tsz() * N));
+    if (!c->Vr[comp]) FGPE_RET(dalloc(c, &c->Vr[comp], c->tsz() * N));
+    if (cplx && !c->Vi[comp]) FGPE_RET(dalloc(c, &c->Vi[comp], c->tsz() * N));
     if (c->mixed()) {
         std::vector<float> re(N), im(cplx ? N : 0);
         for (size_t i = 0; i < N; ++i) {
             re[i] = (float)host[2 * i];
             if (cplx) im[i] = (float)host[2 * i + 1];
         }
-        HIPCHK(hipMemcpy(c->Vr[comp], re.data(), 4 * N, hipMemcpyHostToDevice));
-        if (cplx) HIPCHK(hipMemcpy(c->Vi[comp], im.data(), 4 * N, hipMemcpyHostToDevice));
+        FGPE_HIPCHK(hipMemcpy(c->Vr[comp], re.data(), 4 * N, hipMemcpyHostToDevice));
+        if (cplx) FGPE_HIPCHK(hipMemcpy(c->Vi[comp], im.data(), 4 * N, hipMemcpyHostToDevice));
     } else {
         std::vector<double> re(N), im(cplx ? N : 0);
         for (size_t i = 0; i < N; ++i) {
             re[i] = host[2 * i];
             if (cplx) im[i] = host[2 * i + 1];
         }
-        HIPCHK(hipMemcpy(c->Vr[comp], re.data(), 8 * N, hipMemcpyHostToDevice));
-        if (cplx) HIPCHK(hipMemcpy(c->Vi[comp], im.data(), 8 * N, hipMemcpyHostToDevice));
+        FGPE_HIPCHK(hipMemcpy(c->Vr[comp], re.data(), 8 * N, hipMemcpyHostToDevice));
+        if (cplx) FGPE_HIPCHK(hipMemcpy(c->Vi[comp], im.data(), 8 * N, hipMemcpyHostToDevice));
     }
     c->vcomplex[comp] = cplx;
     return FGPE_OK;
@@ -686,12 +621,12 @@ int fgpe_laplacian_f64(fgpe_ctx* c, const void* psi, void* out)
     if (!c || !psi || !out) return fail(FGPE_EINVAL, "NULL argument");
     if (!c->kset) return fail(FGPE_ESTATE, "fgpe_set_kspace has not been called");
     if (c->n[1] == 1) return fail(FGPE_EUNSUP, "1D grids are not supported");
-    RET(fft_x<double>(c, psi, out, -1));
+    FGPE_RET(Launch<double>::fft_x(c, psi, out, -1));
     const int la = last_axis(c);
-    if (la == 2) RET(strided<double>(c, out, 1, -1, 1, nullptr, 1));
-    RET(last<double>(c, out, la, 1, 1, nullptr, false));
-    if (la == 2) RET(strided<double>(c, out, 1, +1, 1, nullptr, 1));
-    RET(fft_x<double>(c, out, out, +1));
+    if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, -1, 1, 1, nullptr, nullptr));
+    FGPE_RET(Launch<double>::last(c, out, la, 1, 1, nullptr, false, nullptr));
+    if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, +1, 1, 1, nullptr, nullptr));
+    FGPE_RET(Launch<double>::fft_x(c, out, out, +1));
     return FGPE_OK;
 }
 
@@ -704,10 +639,10 @@ int fgpe_apply_h(fgpe_ctx* c, double t, void* hmu, void* hen)
 int fgpe_observables(fgpe_ctx* c, double t, double* out)
 {
     if (!c || !out) return fail(FGPE_EINVAL, "NULL argument");
-    double* obs = c->scal + 4;
-    RET(c->mixed() ? apply_h<float>(c, t, nullptr, nullptr, obs) : apply_h<double>(c, t, nullptr, nullptr, obs));
-    HIPCHK(hipMemcpyAsync(out, obs, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    double* obs = c->scal + SC_OBS;
+    FGPE_RET(c->mixed() ? apply_h<float>(c, t, nullptr, nullptr, obs) : apply_h<double>(c, t, nullptr, nullptr, obs));
+    FGPE_HIPCHK(hipMemcpyAsync(out, obs, 3 * c->hp.ncomp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     return FGPE_OK;
 }
 
@@ -722,26 +657,32 @@ int fgpe_rk4_steps(fgpe_ctx* c, double dt, double t0, int nsteps)
 int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, double npart, double eps, int nsteps,
                     double* host_hist, int* steps_done)
 {
-    (void)beta;
-    (void)eps;
     if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
     if (nsteps < 0) return fail(FGPE_EINVAL, "nsteps < 0");
-    if (scheme != FGPE_IMAG_RK4_RENORM) return fail(FGPE_EUNSUP, "only FGPE_IMAG_RK4_RENORM is available");
+    if (scheme != FGPE_IMAG_RK4_RENORM && scheme != FGPE_IMAG_HEAVY_BALL)
+        return fail(FGPE_EINVAL, "unknown imaginary-time scheme");
+    if (!c->hset) return fail(FGPE_ESTATE, "fgpe_set_hamiltonian has not been called");
+    const int nc = c->hp.ncomp;
     double* hd = nullptr;
     if (host_hist && nsteps > 0) {
-        if ((size_t)nsteps * 3 > c->hist_n) {
-            if (c->hist) { (void)hipFree(c->hist); c->bytes -= sizeof(double) * c->hist_n; c->hist = nullptr; }
-            RET(dalloc(c, (void**)&c->hist, sizeof(double) * 3 * nsteps));
-            c->hist_n = 3 * (size_t)nsteps;
-        }
+        FGPE_RET(ensure_hist(c, (size_t)3 * nc * nsteps));
         hd = c->hist;
     }
-    RET(c->mixed() ? rk4_run<float>(c, dt, 0.0, nsteps, true, npart, hd)
-                   : rk4_run<double>(c, dt, 0.0, nsteps, true, npart, hd));
-    RET(apply_pending_scale(c));
-    if (hd) HIPCHK(hipMemcpyAsync(host_hist, hd, sizeof(double) * 3 * nsteps, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (steps_done) *steps_done = nsteps;
+    if (scheme == FGPE_IMAG_RK4_RENORM) {
+        FGPE_RET(c->mixed() ? rk4_run<float>(c, dt, 0.0, nsteps, true, npart, hd)
+                            : rk4_run<double>(c, dt, 0.0, nsteps, true, npart, hd));
+    } else {
+        FGPE_RET(c->mixed() ? hb_run<float>(c, dt, beta, npart, eps, nsteps, hd)
+                            : hb_run<double>(c, dt, beta, npart, eps, nsteps, hd));
+    }
+    FGPE_RET(apply_pending_scale(c));
+    int done = nsteps;
+    if (scheme == FGPE_IMAG_HEAVY_BALL)
+        FGPE_HIPCHK(hipMemcpyAsync(&done, c->flags + FL_DONE, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
+    if (hd && done > 0)
+        FGPE_HIPCHK(hipMemcpy(host_hist, hd, sizeof(double) * 3 * nc * done, hipMemcpyDeviceToHost));
+    if (steps_done) *steps_done = done;
     return FGPE_OK;
 }
 
@@ -753,7 +694,7 @@ int fgpe_timing_enable(fgpe_ctx* c, int on)
     c->evrec.clear();
     if (c->timing && c->evpool.empty()) {
         c->evpool.resize(16384);
-        for (auto& e : c->evpool) HIPCHK(hipEventCreate(&e));
+        for (auto& e : c->evpool) FGPE_HIPCHK(hipEventCreate(&e));
     }
     return FGPE_OK;
 }
@@ -761,11 +702,11 @@ int fgpe_timing_enable(fgpe_ctx* c, int on)
 int fgpe_timing_read(fgpe_ctx* c, double* out)
 {
     if (!c || !out) return fail(FGPE_EINVAL, "NULL argument");
-    HIPCHK(hipStreamSynchronize(c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     for (int k = 0; k < 2 * FGPE_NKCLASS; ++k) out[k] = 0.0;
     for (auto& r : c->evrec) {
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->evpool[r.second], c->evpool[r.second + 1]));
+        FGPE_HIPCHK(hipEventElapsedTime(&ms, c->evpool[r.second], c->evpool[r.second + 1]));
         out[2 * r.first] += ms;
         out[2 * r.first + 1] += 1.0;
     }

@@ -107,8 +107,9 @@ struct StridedArgs {
     long long stride;       // element stride along the axis
     long long outer_stride; // stride of the outer index
     long long chan_stride;  // elements per channel
-    const double* scale;
+    const double* scale;    // per-component pending scale (forward only) or null
     int ncomp;
+    const int* stop;        // nonzero: skip (a converged imaginary-time loop)
 };
 
 template <typename T, int N, int DIR>
@@ -118,6 +119,7 @@ k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArg
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = ps_of<T, N>();
     __shared__ T2 lds[P * lds_stride<N>()];
+    if (a.stop && *a.stop) return;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const int ntile = a.nx / P;
     const int xt = blockIdx.x % ntile;
@@ -130,7 +132,7 @@ k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArg
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N>(t, q) * a.stride];
         if (a.scale) {
-            double s = a.scale[0];
+            double s = a.scale[ch < a.ncomp ? ch : 0];
             if (ch >= a.ncomp) s = s * s;
             const T st = (T)s;
 #pragma unroll
@@ -172,6 +174,7 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = ps_of<T, N>();
     __shared__ T2 lds[P * lds_stride<N>()];
+    if (a.s.stop && *a.s.stop) return;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const int ntile = a.s.nx / P;
     const int xt = blockIdx.x % ntile;
@@ -185,7 +188,7 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N>(t, q) * a.s.stride];
     if (a.s.scale) {
-        double s = a.s.scale[0];
+        double s = a.s.scale[ch < a.s.ncomp ? ch : 0];
         if (ch >= a.s.ncomp) s = s * s;
         const T st = (T)s;
 #pragma unroll
@@ -236,198 +239,245 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// x-axis stage kernel (one component).
-//   MODE_PROLOGUE: spec <- Fx(psi) [, Fx(|psi|^2)] of the stored (raw) state;
-//                  a pending scale is applied by the next forward pass
-//   MODE_RK:       inverse x of spec -> KE [, Phi]; H at every point; RK4
-//                  stage `stage` (src/oneComponentGPSolver.cu:165-187:
-//                  UpdateRKStep / FinalRKStep, src/parallel.cu:230-271);
-//                  spec <- Fx(next state) [, Fx(its density)]
-//   MODE_APPLY:    inverse x -> H and Hen (BECHamiltonian,
-//                  hamiltonians/BEC_oneComponent.cu:128-150) to hmu/hen (if
-//                  non-null) and the sums |psi|^2, Re(psi* H), Re(psi* Hen).
-// H = KE + (V.x + g4 n + Phi) psi + i V.y psi, Hen with g2 and Phi/2.
-enum { MODE_PROLOGUE = 0, MODE_RK = 1, MODE_APPLY = 2 };
+// x-axis stage kernel, NC = 1 or 2 components.
+//   MODE_PROLOGUE: spec <- Fx(psi_c) [, Fx(sum_c |psi_c|^2)] of the stored
+//                  (raw) state; a pending scale is applied by the next forward
+//                  pass.
+//   MODE_RK:       inverse x of spec -> KE_c [, Phi]; H_c at every point; RK4
+//                  stage `stage` (src/oneComponentGPSolver.cu:165-187 and
+//                  src/twoComponentGPSolver.cu:150-176: UpdateRKStep /
+//                  FinalRKStep, src/parallel.cu:230-271); spec <- Fx(next
+//                  state) [, Fx(its density)].
+//   MODE_APPLY:    inverse x -> H_c and Hen_c (BECHamiltonian,
+//                  hamiltonians/BEC_oneComponent.cu:128-150; BECHamiltonian_2,
+//                  hamiltonians/BEC_twoComponent.cu:168-212) to hmu/hen (if
+//                  non-null) and per-block sums |psi|^2, Re(psi* H), Re(psi* Hen).
+//   MODE_HB:       heavy-ball imaginary-time update (BECUpdatePsi,
+//                  src/parallel.cu:294-302): new = (1 + mu dt) psi - dt H
+//                  + beta (psi - old); P0 <- new (raw), PS <- psi (the next
+//                  old), sums of new; spec <- Fx(new) [, Fx(|new|^2)].
+// H_c = KE_c + (V_c.x + g4[c][c] n_c + g4[c][c'] n_c' + Phi) psi_c + i V_c.y psi_c,
+// Hen_c with g2 and Phi/2.
+enum { MODE_PROLOGUE = 0, MODE_RK = 1, MODE_APPLY = 2, MODE_HB = 3 };
 
 template <typename T> struct XArgs {
     c2_t<T>* spec;
     long long ntot;
-    double2* P0;            // FP64 state psi_n (psi_old of the step)
-    c2_t<T>* PS;            // stage state
-    c2_t<T>* K;             // RK accumulator
-    const T* Vr;            // Re V
-    const T* Vi;            // Im V or null
-    double g4, g2;          // 4 pi a(t), 2 pi a(t)
-    double cr, ci;          // stage coefficient c
-    int stage;              // 1..4
-    const double* scale;    // pending renormalisation of P0 (null: 1)
-    double* partials;       // 3 doubles per block (MODE_APPLY, or want_norm)
-    int want_norm;          // MODE_RK stage 4: write sum |psi_new|^2 per block
-    double2* hmu;
+    double2* P0[2];         // FP64 state psi_n (RK: psi_old of the step)
+    c2_t<T>* PS[2];         // RK: stage state; HB: previous psi
+    c2_t<T>* K[2];          // RK accumulator
+    const T* Vr[2];         // Re V_c
+    const T* Vi[2];         // Im V_c or null
+    double g4[2][2];        // 4 pi a_cc'
+    double g2[2][2];        // 2 pi a_cc'
+    double cr, ci;          // RK stage coefficient c
+    int stage;              // RK stage 1..4
+    const double* scale;    // pending renormalisation per component (null: 1)
+    double* partials;       // 3*NC doubles per block
+    int want_norm;          // RK stage 4: per-block sums of |psi_new|^2
+    double2* hmu;           // APPLY outputs, component c at c*ntot
     double2* hen;
+    double dt, beta;        // HB
+    const double* mu;       // HB: mu_c of the previous iteration (null: 0)
+    const int* stop;        // nonzero: the loop has converged, do nothing
 };
 
-template <typename T, int N, bool DENS, int MODE>
+template <typename T, int N, int NC, bool DENS, int MODE>
 __global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
 k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    constexpr int NV = 3 * NC;
     __shared__ T2 lds[P * lds_stride<N>()];
+    if (a.stop && *a.stop) return;
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
-    T2* spec0 = a.spec + row0;
-    T2* spec1 = a.spec + a.ntot + row0;
-    const double sig = a.scale ? a.scale[0] : 1.0;
+    T2* specd = a.spec + NC * a.ntot + row0;   // density channel
 
     T2 v[E];
     double nd[E];        // density of the new state (DENS)
+#pragma unroll
+    for (int q = 0; q < E; ++q) nd[q] = 0.0;
 
     if (MODE == MODE_PROLOGUE) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            v[q] = from_d2<T>(a.P0[row0 + elemA<N>(t, q)]);
-            if (DENS) { const T2 w = v[q]; nd[q] = (double)(w.x * w.x + w.y * w.y); }
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                v[q] = from_d2<T>(a.P0[c][row0 + elemA<N>(t, q)]);
+                if (DENS) nd[q] += (double)(v[q].x * v[q].x + v[q].y * v[q].y);
+            }
+            fft_forward<T2, N>(v, t, my, tw);
+            T2* sc = a.spec + c * a.ntot + row0;
+#pragma unroll
+            for (int q = 0; q < E; ++q) sc[elemB<N>(t, q)] = v[q];
         }
     } else {
         double phi[E];
         if (DENS) {
 #pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = spec1[elemB<N>(t, q)];
+            for (int q = 0; q < E; ++q) v[q] = specd[elemB<N>(t, q)];
             fft_inverse<T2, N>(v, t, my, tw);
 #pragma unroll
             for (int q = 0; q < E; ++q) phi[q] = (double)v[q].x;
         }
+        double nsave[E];     // NC == 2: density of component 0's stage input
+        double sums[NV];
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = spec0[elemB<N>(t, q)];
-        fft_inverse<T2, N>(v, t, my, tw);   // v = KE psi_s (layout A)
+        for (int i = 0; i < NV; ++i) sums[i] = 0.0;
+        const bool from_p0 = (MODE == MODE_APPLY || MODE == MODE_HB || a.stage == 1);
 
-        double sums[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const long long g = row0 + elemA<N>(t, q);
-            // stage state psi_s
-            double2 po = double2{0.0, 0.0};
-            double2 ps;
-            if (MODE == MODE_APPLY || a.stage == 1) {
-                const double2 r = a.P0[g];
-                po = double2{sig * r.x, sig * r.y};
-                ps = po;
-            } else {
-                ps = to_d2<T>(a.PS[g]);
-            }
-            const double vr = (double)a.Vr[g];
-            const double vi = a.Vi ? (double)a.Vi[g] : 0.0;
-            const double n = ps.x * ps.x + ps.y * ps.y;
-            const double2 ke = to_d2<T>(v[q]);
-            const double ph = DENS ? phi[q] : 0.0;
-            double rX = (vr + a.g4 * n);
-            if (DENS) rX += ph;
-            const double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
-            if (MODE == MODE_APPLY) {
-                double eX = (vr + a.g2 * n);
-                if (DENS) eX += 0.5 * ph;
-                const double2 he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
-                if (a.hmu) a.hmu[g] = h;
-                if (a.hen) a.hen[g] = he;
-                sums[0] += n;
-                sums[1] += ps.x * h.x + ps.y * h.y;
-                sums[2] += ps.x * he.x + ps.y * he.y;
-            } else {
-                if (a.stage != 1) {
-                    const double2 r = a.P0[g];
+        for (int c = 0; c < NC; ++c) {
+            T2* sc = a.spec + c * a.ntot + row0;
+#pragma unroll
+            for (int q = 0; q < E; ++q) v[q] = sc[elemB<N>(t, q)];
+            fft_inverse<T2, N>(v, t, my, tw);   // v = KE psi_c (layout A)
+            const double sig = a.scale ? a.scale[c] : 1.0;
+            const double sig1 = (NC == 2 && a.scale) ? a.scale[1] : 1.0;
+            const double g4s = a.g4[c][c], g2s = a.g2[c][c];
+            const double g4o = a.g4[c][1 - c], g2o = a.g2[c][1 - c];
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const long long g = row0 + elemA<N>(t, q);
+                double2 po = double2{0.0, 0.0};
+                double2 ps;
+                if (from_p0) {
+                    const double2 r = a.P0[c][g];
                     po = double2{sig * r.x, sig * r.y};
-                }
-                double2 nw;
-                if (a.stage < 4) {
-                    // UpdateRKStep: psi = old + c*h ; k = h (stage 1) or k + 2h
-                    nw = double2{po.x + (a.cr * h.x - a.ci * h.y), po.y + (a.cr * h.y + a.ci * h.x)};
-                    if (a.stage == 1) {
-                        a.K[g] = from_d2<T>(h);
-                    } else {
-                        const double2 k = to_d2<T>(a.K[g]);
-                        a.K[g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
-                    }
-                    const T2 st = from_d2<T>(nw);
-                    a.PS[g] = st;
-                    v[q] = st;
+                    ps = po;
                 } else {
-                    // FinalRKStep: psi = old + c*(k + h)/6
-                    const double2 k = to_d2<T>(a.K[g]);
-                    const double aX = k.x + h.x, aY = k.y + h.y;
-                    nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0, po.y + (a.cr * aY + a.ci * aX) / 6.0};
-                    a.P0[g] = nw;
-                    v[q] = from_d2<T>(nw);
-                    if (a.want_norm) sums[0] += nw.x * nw.x + nw.y * nw.y;
+                    ps = to_d2<T>(a.PS[c][g]);
                 }
-                if (DENS) { const T2 w = v[q]; nd[q] = (double)(w.x * w.x + w.y * w.y); }
+                const double n = ps.x * ps.x + ps.y * ps.y;
+                double no = 0.0;
+                if (NC == 2) {
+                    if (c == 0) {
+                        double2 o;
+                        if (from_p0) {
+                            const double2 r = a.P0[1][g];
+                            o = double2{sig1 * r.x, sig1 * r.y};
+                        } else {
+                            o = to_d2<T>(a.PS[1][g]);
+                        }
+                        no = o.x * o.x + o.y * o.y;
+                        nsave[q] = n;
+                    } else {
+                        no = nsave[q];
+                    }
+                }
+                const double vr = (double)a.Vr[c][g];
+                const double vi = a.Vi[c] ? (double)a.Vi[c][g] : 0.0;
+                const double2 ke = to_d2<T>(v[q]);
+                double rX = (vr + g4s * n);
+                if (NC == 2) rX = rX + g4o * no;
+                if (DENS) rX += phi[q];
+                const double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                double2 he = double2{0.0, 0.0};
+                if (MODE == MODE_APPLY || MODE == MODE_HB) {
+                    double eX = (vr + g2s * n);
+                    if (NC == 2) eX = eX + g2o * no;
+                    if (DENS) eX += 0.5 * phi[q];
+                    he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
+                }
+                if (MODE == MODE_APPLY) {
+                    if (a.hmu) a.hmu[c * a.ntot + g] = h;
+                    if (a.hen) a.hen[c * a.ntot + g] = he;
+                    sums[3 * c + 0] += n;
+                    sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
+                    sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
+                } else if (MODE == MODE_HB) {
+                    const double2 old = to_d2<T>(a.PS[c][g]);
+                    const double m = a.mu ? a.mu[c] : 0.0;
+                    const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
+                                               (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
+                    a.PS[c][g] = from_d2<T>(ps);
+                    a.P0[c][g] = nw;
+                    sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
+                    sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
+                    sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
+                    v[q] = from_d2<T>(nw);
+                    if (DENS) nd[q] += (double)(v[q].x * v[q].x + v[q].y * v[q].y);
+                } else {
+                    if (a.stage != 1) {
+                        const double2 r = a.P0[c][g];
+                        po = double2{sig * r.x, sig * r.y};
+                    }
+                    if (a.stage < 4) {
+                        // UpdateRKStep: psi = old + c*h ; k = h (stage 1) or k + 2h
+                        const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
+                                                   po.y + (a.cr * h.y + a.ci * h.x)};
+                        if (a.stage == 1) {
+                            a.K[c][g] = from_d2<T>(h);
+                        } else {
+                            const double2 k = to_d2<T>(a.K[c][g]);
+                            a.K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
+                        }
+                        const T2 st = from_d2<T>(nw);
+                        a.PS[c][g] = st;
+                        v[q] = st;
+                    } else {
+                        // FinalRKStep: psi = old + c*(k + h)/6
+                        const double2 k = to_d2<T>(a.K[c][g]);
+                        const double aX = k.x + h.x, aY = k.y + h.y;
+                        const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
+                                                   po.y + (a.cr * aY + a.ci * aX) / 6.0};
+                        a.P0[c][g] = nw;
+                        v[q] = from_d2<T>(nw);
+                        if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
+                    }
+                    if (DENS) nd[q] += (double)(v[q].x * v[q].x + v[q].y * v[q].y);
+                }
+            }
+            if (MODE != MODE_APPLY) {
+                fft_forward<T2, N>(v, t, my, tw);
+#pragma unroll
+                for (int q = 0; q < E; ++q) sc[elemB<N>(t, q)] = v[q];
             }
         }
-        if (MODE == MODE_APPLY) {
-            block_sum_store<3>(sums, a.partials + 3 * blockIdx.x);
-            return;
-        }
-        if (a.want_norm) {
-            double s1[1] = {sums[0]};
-            block_sum_store<1>(s1, a.partials + blockIdx.x);
-        }
+        if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm))
+            block_sum_store<NV>(sums, a.partials + (long long)NV * blockIdx.x);
+        if (MODE == MODE_APPLY) return;
     }
-    // forward x-FFT of the next state and of its density
-    fft_forward<T2, N>(v, t, my, tw);
-#pragma unroll
-    for (int q = 0; q < E; ++q) spec0[elemB<N>(t, q)] = v[q];
     if (DENS) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)nd[q], (T)0};
         fft_forward<T2, N>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) spec1[elemB<N>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) specd[elemB<N>(t, q)] = v[q];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Final reductions over the per-block partial sums, in a fixed order.
-//   kind 0 (observables): out[0] = S0 dv, out[1] = S1 dv / out[0], out[2] = S2 dv / out[0]
-//   kind 1 (renormalise): scale[0] = sqrt(npart / (S0 dv)), out[0] = S0 dv
-enum { FIN_OBS = 0, FIN_RENORM = 1 };
-__global__ void __launch_bounds__(256)
-k_finalize(const double* __restrict__ partials, int nblocks, int nv, int kind, double dv,
-           double npart, double* out, double* scale)
-{
-    __shared__ double sh[3][256];
-    const int t = threadIdx.x;
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int b = t; b < nblocks; b += 256)
-        for (int q = 0; q < nv; ++q) s[q] += partials[(long long)b * nv + q];
-    for (int q = 0; q < 3; ++q) sh[q][t] = s[q];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w)
-            for (int q = 0; q < 3; ++q) sh[q][t] += sh[q][t + w];
-        __syncthreads();
-    }
-    if (t == 0) {
-        const double norm = sh[0][0] * dv;
-        if (kind == FIN_OBS) {
-            out[0] = norm;
-            out[1] = sh[1][0] * dv / norm;
-            out[2] = sh[2][0] * dv / norm;
-        } else {
-            out[0] = norm;
-            scale[0] = sqrt(npart / norm);
-        }
-    }
-}
+// Final reductions over the per-block partial sums (3 per component), summed
+// in a fixed order by one block.
+//   FIN_OBS:    out[3c..3c+2] = norm_c = S0 dv, S1 dv / norm_c, S2 dv / norm_c
+//   FIN_RENORM: scale[c] = sqrt(npart_c / (S0 dv)); out[3c] = S0 dv
+//   FIN_HB:     the bookkeeping of one heavy-ball iteration
+//               (src/oneComponentGPSolver.cu:68-88, src/twoComponentGPSolver.cu
+//               :59-94): norm_c, scale_c = sqrt(npart_c/norm_c),
+//               mu_c, E_c = scale_c S1,2 dv / normarg_c (normarg = npart_c for
+//               one component, the unscaled norm_c for two), out <- (norm, mu,
+//               E) per component, convergence |dE|/E_prev < eps (one
+//               component) or |dE_c| < eps for both (two) sets *stop.
+enum { FIN_OBS = 0, FIN_RENORM = 1, FIN_HB = 2 };
+struct FinArgs {
+    const double* partials;
+    int nblocks, nc, kind;
+    double dv;
+    double npart[2];
+    double* out;
+    double* scale;
+    double* mu;
+    double* eprev;
+    int* stop;
+    int* done;
+    double eps;
+};
 
-// psi <- scale * psi (applies a pending renormalisation), then scale <- 1
-__global__ void k_apply_scale(double2* __restrict__ psi, long long n, double* scale)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const double s = scale[0];
-    if (i < n) psi[i] = double2{s * psi[i].x, s * psi[i].y};
-}
-__global__ void k_set_scalar(double* p, double v) { p[0] = v; }
+__global__ void k_finalize(FinArgs f);   // defined in fgpe_capi.hip
 
 template <typename TD, typename TS>
 __global__ void k_convert(TD* __restrict__ dst, const TS* __restrict__ src, long long n)

@@ -1,0 +1,182 @@
+// launch.hpp -- template launchers of the pass kernels, dispatched over the
+// compiled pencil lengths.  Included once per precision (launch_f64.hip,
+// launch_f32.hip) so the two sets of kernels compile in parallel.
+#pragma once
+#include "ctx.hpp"
+
+namespace fgpe {
+namespace detail {
+
+template <typename T> const c2_t<T>* tw_of(fgpe_ctx* c, int axis)
+{
+    return (const c2_t<T>*)(sizeof(T) == 8 ? c->tw64[axis] : c->tw32[axis]);
+}
+
+inline long long n_outer(fgpe_ctx* c, int axis) { return axis == 1 ? c->n[2] : c->n[1]; }
+
+inline StridedArgs strided_args(fgpe_ctx* c, int axis, const double* scale, int ncomp, const int* stop)
+{
+    StridedArgs a;
+    a.nx = c->n[0];
+    a.chan_stride = c->ntot;
+    a.scale = scale;
+    a.ncomp = ncomp;
+    a.stop = stop;
+    if (axis == 1) {
+        a.stride = c->n[0];
+        a.outer_stride = (long long)c->n[0] * c->n[1];
+    } else {
+        a.stride = (long long)c->n[0] * c->n[1];
+        a.outer_stride = c->n[0];
+    }
+    return a;
+}
+
+template <typename T, int N> int fft_x_n(fgpe_ctx* c, const void* in, void* out, int dir)
+{
+    constexpr int P = px_of<T, N>();
+    const long long rows = c->ntot / N;
+    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
+    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
+    using T2 = c2_t<T>;
+    TScope ts(c, FGPE_KCLASS_FFTX);
+    if (dir < 0)
+        hipLaunchKernelGGL((k_fft_x<T, N, -1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out, tw_of<T>(c, 0));
+    else
+        hipLaunchKernelGGL((k_fft_x<T, N, +1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out, tw_of<T>(c, 0));
+    FGPE_LAUNCHCHK("k_fft_x");
+    return FGPE_OK;
+}
+
+template <typename T, int N> int strided_n(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
+                                           const double* scale, const int* stop)
+{
+    constexpr int P = ps_of<T, N>();
+    if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
+    StridedArgs a = strided_args(c, axis, scale, ncomp, stop);
+    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N>::T);
+    using T2 = c2_t<T>;
+    TScope ts(c, FGPE_KCLASS_STRIDED);
+    if (dir < 0)
+        hipLaunchKernelGGL((k_strided<T, N, -1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
+    else
+        hipLaunchKernelGGL((k_strided<T, N, +1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
+    FGPE_LAUNCHCHK("k_strided");
+    return FGPE_OK;
+}
+
+template <typename T, int N> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
+                                        const double* scale, bool filter, const int* stop)
+{
+    constexpr int P = ps_of<T, N>();
+    if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
+    LastArgs a;
+    a.s = strided_args(c, axis, scale, ncomp, stop);
+    const int oaxis = axis == 1 ? 2 : 1;
+    a.kA = c->k[axis];
+    a.kX = c->k[0];
+    a.kO = c->k[oaxis];
+    const double* d = c->hp.dip;
+    a.dA = d[axis];
+    a.dX = d[0];
+    a.dO = d[oaxis];
+    a.lap_scale = 19.739208802 / (double)c->ntot;
+    a.inv_ntot = 1.0 / (double)c->ntot;
+    a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
+    a.wtab = filter ? c->wtab : nullptr;
+    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N>::T);
+    TScope ts(c, FGPE_KCLASS_LAST);
+    hipLaunchKernelGGL((k_last<T, N>), grid, block, 0, c->stream, (c2_t<T>*)data, tw_of<T>(c, axis), a);
+    FGPE_LAUNCHCHK("k_last");
+    return FGPE_OK;
+}
+
+template <typename T, int N, int NC, bool DENS, int MODE> int xstage_t(fgpe_ctx* c, const XArgs<T>& a)
+{
+    constexpr int P = px_of<T, N>();
+    const long long rows = c->ntot / N;
+    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
+    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
+    TScope ts(c, FGPE_KCLASS_XSTAGE);
+    hipLaunchKernelGGL((k_xstage<T, N, NC, DENS, MODE>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    FGPE_LAUNCHCHK("k_xstage");
+    return FGPE_OK;
+}
+
+template <typename T, int N, int NC, bool DENS> int xstage_m(fgpe_ctx* c, const XArgs<T>& a, int mode)
+{
+    switch (mode) {
+    case MODE_PROLOGUE: return xstage_t<T, N, NC, DENS, MODE_PROLOGUE>(c, a);
+    case MODE_RK: return xstage_t<T, N, NC, DENS, MODE_RK>(c, a);
+    case MODE_APPLY: return xstage_t<T, N, NC, DENS, MODE_APPLY>(c, a);
+    case MODE_HB: return xstage_t<T, N, NC, DENS, MODE_HB>(c, a);
+    }
+    return fail(FGPE_EINVAL, "bad x-stage mode");
+}
+
+template <typename T, int N> int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
+{
+    if (nc == 1) return dens ? xstage_m<T, N, 1, true>(c, a, mode) : xstage_m<T, N, 1, false>(c, a, mode);
+    return dens ? xstage_m<T, N, 2, true>(c, a, mode) : xstage_m<T, N, 2, false>(c, a, mode);
+}
+
+}  // namespace detail
+
+template <typename T> long long Launch<T>::xblocks(fgpe_ctx* c)
+{
+    switch (c->n[0]) {
+#define CASE(N_) \
+    case N_: return c->ntot / N_ / px_of<T, N_>();
+        FGPE_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return 0;
+}
+
+template <typename T> int Launch<T>::fft_x(fgpe_ctx* c, const void* in, void* out, int dir)
+{
+    switch (c->n[0]) {
+#define CASE(N_) \
+    case N_: return detail::fft_x_n<T, N_>(c, in, out, dir);
+        FGPE_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return fail(FGPE_EUNSUP, "unsupported nx");
+}
+
+template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
+                                             const double* scale, const int* stop)
+{
+    switch (c->n[axis]) {
+#define CASE(N_) \
+    case N_: return detail::strided_n<T, N_>(c, data, axis, dir, nch, ncomp, scale, stop);
+        FGPE_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return fail(FGPE_EUNSUP, "unsupported strided axis length");
+}
+
+template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
+                                          const double* scale, bool filter, const int* stop)
+{
+    switch (c->n[axis]) {
+#define CASE(N_) \
+    case N_: return detail::last_n<T, N_>(c, data, axis, nch, ncomp, scale, filter, stop);
+        FGPE_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return fail(FGPE_EUNSUP, "unsupported last axis length");
+}
+
+template <typename T> int Launch<T>::xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
+{
+    switch (c->n[0]) {
+#define CASE(N_) \
+    case N_: return detail::xstage_n<T, N_>(c, a, nc, dens, mode);
+        FGPE_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return fail(FGPE_EUNSUP, "unsupported nx");
+}
+
+}  // namespace fgpe

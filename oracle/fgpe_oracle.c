@@ -493,42 +493,62 @@ void orc_rk4(const orc_grid *g, const orc_ham *h, double *psi, const double *V,
     free(old); free(k); free(hp);
 }
 
-/* Imaginary time of the one-component solver
-   (src/oneComponentGPSolver.cu:26-104):
+/* Imaginary time (heavy ball).  One component, src/oneComponentGPSolver.cu:26-104:
      H, Hen = H(psi, it*dt)                         alg_calcHpsi
      new = (1 + 0*mu dt) psi - dt H + beta (psi - old)   BECUpdatePsi
                                                     (src/parallel.cu:294-302)
      norm = |new|^2 DV; old = psi; psi = sqrt(npart/norm) new
      mu, E = <psi|H>, <psi|Hen> * DV / npart        alg_calc2Observables
      stop when |E - E_prev| / E_prev < eps          (:84-88, no abs on E_prev)
-   Returns iterations run (the converging one included). */
+   Two components, src/twoComponentGPSolver.cu:15-112 (PROPER_2BEC_NORMALIZATION
+   undefined): the update uses mu_c of the previous iteration, each component
+   is renormalised to npart on its own, mu_c and E_c divide by the unscaled
+   norm_c (PARS[NORM_c] is not reset, :67-69), and the loop stops when
+   |E_c - E_c,prev| < eps for both (:94).
+   Returns iterations run (the converging one included); hist gets
+   (norm, mu, E) per component per iteration. */
 int orc_imag_heavy_ball(const orc_grid *g, const orc_ham *h, double *psi,
                         const double *V, double dt, double beta, double npart,
                         double eps, int niter, double *hist)
 {
     long long N = (long long)g->nx * g->ny * g->nz;
-    size_t bytes = sizeof(double) * 2 * N;
+    int nc = h->ncomp;
+    size_t bytes = sizeof(double) * 2 * N * nc;
     double *old = (double *)malloc(bytes);
     double *nw = (double *)malloc(bytes);
     double *hp = (double *)malloc(bytes);
     double *he = (double *)malloc(bytes);
     memcpy(old, psi, bytes);
-    double en = 0.0, mu = 0.0;
+    double en[2] = {0.0, 0.0}, mu[2] = {0.0, 0.0};
     int it;
     for (it = 0; it < niter; ++it) {
-        double en_prev = en;
+        double en_prev[2] = {en[0], en[1]};
         double t = it * dt;
         orc_hpsi(g, h, t, psi, V, hp, he);
-        double m = 0 * mu;
-        for (long long i = 0; i < 2 * N; ++i)
-            nw[i] = (1.0 + m * dt) * psi[i] - dt * hp[i] + beta * (psi[i] - old[i]);
-        double nrm = orc_norm(g, nw);
-        double s = pow(npart / nrm, 0.5);
+        for (int c = 0; c < nc; ++c) {
+            double m = nc == 1 ? 0 * mu[0] : mu[c];
+            long long o = 2 * N * c;
+            for (long long i = 0; i < 2 * N; ++i)
+                nw[o + i] = (1.0 + m * dt) * psi[o + i] - dt * hp[o + i] + beta * (psi[o + i] - old[o + i]);
+        }
         memcpy(old, psi, bytes);
-        for (long long i = 0; i < 2 * N; ++i) psi[i] = s * nw[i];
-        orc_obs2(g, npart, psi, hp, he, &mu, &en);
-        if (hist) { hist[3 * it] = nrm; hist[3 * it + 1] = mu; hist[3 * it + 2] = en; }
-        if (fabs(en - en_prev) / en_prev < eps) { ++it; break; }
+        for (int c = 0; c < nc; ++c) {
+            long long o = 2 * N * c;
+            double nrm = orc_norm(g, nw + o);
+            double s = pow(npart / nrm, 0.5);
+            for (long long i = 0; i < 2 * N; ++i) psi[o + i] = s * nw[o + i];
+            double narg = nc == 1 ? npart : nrm;
+            orc_obs2(g, narg, psi + o, hp + o, he + o, &mu[c], &en[c]);
+            if (hist) {
+                hist[3 * (nc * it + c)] = nrm;
+                hist[3 * (nc * it + c) + 1] = mu[c];
+                hist[3 * (nc * it + c) + 2] = en[c];
+            }
+        }
+        int conv;
+        if (nc == 1) conv = fabs(en[0] - en_prev[0]) / en_prev[0] < eps;
+        else conv = (fabs(en[0] - en_prev[0]) < eps) && (fabs(en[1] - en_prev[1]) < eps);
+        if (conv) { ++it; break; }
     }
     free(old); free(nw); free(hp); free(he);
     return it;

@@ -56,9 +56,9 @@ void   orc_obs2(const orc_grid *g, double norm, const double *psi,
 /* Real-time RK4 (coef = -i dt) or imaginary-time RK4 (coef = -dt, renorm) */
 void   orc_rk4(const orc_grid *g, const orc_ham *h, double *psi, const double *V,
                double dt, double t0, int nsteps, int imaginary, double npart);
-/* Heavy-ball imaginary time of the one-component solver.  Returns the number
-   of iterations run; hist (3*niter doubles or NULL) gets norm, mu, E per
-   iteration. */
+/* Heavy-ball imaginary time of the one- and two-component solvers.  Returns
+   the number of iterations run; hist (3*ncomp*niter doubles or NULL) gets
+   norm, mu, E per component per iteration. */
 int    orc_imag_heavy_ball(const orc_grid *g, const orc_ham *h, double *psi,
                            const double *V, double dt, double beta, double npart,
                            double eps, int niter, double *hist);

@@ -210,7 +210,7 @@ def rk4(g: Grid, h: Ham, psi, V, dt, nsteps, t0=0.0, imaginary=False, npart=1.0)
 def imag_heavy_ball(g: Grid, h: Ham, psi, V, dt, beta, npart, eps, niter):
     a = _c128(psi).copy()
     V = _c128(V)
-    hist = np.zeros((max(niter, 1), 3))
+    hist = np.zeros((max(niter, 1), 3 * h.ncomp))
     n = lib().orc_imag_heavy_ball(C.byref(g._s), C.byref(h._s), _p(a.view(np.float64)),
                                   _p(V.view(np.float64)), dt, beta, npart, eps, niter,
                                   _p(hist))

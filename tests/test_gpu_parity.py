@@ -191,3 +191,108 @@ def test_mixed_precision_dipolar_error():
     print(f"mixed vs f64 after {w.nsteps} steps: L2 psi {l2:.3e}, rel energy {de:.3e}")
     assert de < 1e-6
     assert l2 < 1e-4
+
+
+def test_heavy_ball_one_component_f64():
+    """The reference's imaginary-time scheme, convergence test on the device."""
+    from fgpe_amd import FGPE_IMAG_HEAVY_BALL, configs
+    c = Case(32, 32, 32, L=(12.0, 12.0, 12.0))
+    h = dict(a=(0.05,), a_dd=0.03, dip=(0.0, 0.0, 1.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 1.3))
+    psi = configs.gaussian_noisy(c.spec, 6, width=1.2)
+    dt, beta, npart, eps, n = 2e-3, 0.9, 1.0, 1e-7, 3000
+    ref, nref, href = orc.imag_heavy_ball(c.grid, oracle_ham(c, h), psi, V, dt, beta, npart, eps, n)
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        done, hist = ctx.imag_steps(dt, n, scheme=FGPE_IMAG_HEAVY_BALL, beta=beta, npart=npart,
+                                    eps=eps, history=True)
+        gpu = ctx.download_psi()
+    assert nref < n, "reference run should converge within the step budget"
+    assert done == nref, (done, nref)
+    err = rel_max(gpu, ref)
+    assert err < 1e-11, err
+    assert np.abs(hist - href).max() <= 1e-11 * np.abs(href).max()
+
+
+def two_comp_case():
+    from fgpe_amd import configs
+    c = Case(32, 32, 64, L=(12.0, 12.0, 16.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.5))
+    p1 = configs.gaussian_noisy(c.spec, 7, offset=(1.0, 0, 0), noise=0.02)
+    p2 = configs.gaussian_noisy(c.spec, 8, offset=(-1.0, 0, 0), noise=0.02)
+    return c, np.stack([V, V + 0.1]), np.stack([p1, p2 * np.exp(0.3j)])
+
+
+def two_comp_ham(c, h):
+    vt = None
+    if h.get("sc_U"):
+        vt = c.vtilde(0.0, sc=(h["sc_dim"], h["sc_R"], h["sc_U"]))
+    return orc.Ham(2, h["a"], vtilde=vt)
+
+
+@pytest.mark.parametrize("name", ["contact", "softcore"])
+def test_two_component_rk4_f64(name):
+    c, V, psi = two_comp_case()
+    f = 1.0 / (4 * np.pi)
+    h = dict(ncomp=2, a=(100 * f, 100 * f, 80 * f))
+    if name == "softcore":
+        h.update(sc_U=5.0, sc_R=1.5, sc_dim=3)
+    dt, n = 5e-4, 10
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ctx.rk4_steps(dt, n)
+        gpu = ctx.download_psi()
+        obs = ctx.observables(n * dt)
+    ref = orc.rk4(c.grid, two_comp_ham(c, h), psi, V, dt, n)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
+    hmu, hen = orc.hpsi(c.grid, two_comp_ham(c, h), n * dt, ref, V)
+    for k in range(2):
+        nrm = orc.norm(c.grid, ref[k])
+        mu, en = orc.obs2(c.grid, nrm, ref[k], hmu[k], hen[k])
+        assert abs(obs[3 * k] - nrm) < 1e-12 * nrm
+        assert abs(obs[3 * k + 1] - mu) < 1e-12 * abs(mu)
+        assert abs(obs[3 * k + 2] - en) < 1e-12 * abs(en)
+
+
+def test_two_component_heavy_ball_f64():
+    from fgpe_amd import FGPE_IMAG_HEAVY_BALL
+    c, V, psi = two_comp_case()
+    h = dict(ncomp=2, a=(0.02, 0.03, 0.01))
+    dt, beta, npart, eps, n = 2e-3, 0.9, 1.0, 1e-9, 60
+    ref, nref, href = orc.imag_heavy_ball(c.grid, two_comp_ham(c, h), psi, V, dt, beta, npart, eps, n)
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        done, hist = ctx.imag_steps(dt, n, scheme=FGPE_IMAG_HEAVY_BALL, beta=beta, npart=npart,
+                                    eps=eps, history=True)
+        gpu = ctx.download_psi()
+    assert done == nref
+    err = rel_max(gpu, ref)
+    assert err < 1e-11, err
+    assert np.abs(hist - href).max() <= 1e-11 * np.abs(href).max()
+
+
+def test_two_component_mixed_error():
+    """Mixed form of the two-component soft-core step against FP64."""
+    c, V, psi = two_comp_case()
+    f = 1.0 / (4 * np.pi)
+    h = dict(ncomp=2, a=(100 * f, 100 * f, 80 * f), sc_U=5.0, sc_R=1.5, sc_dim=3)
+    res = []
+    for prec in (0, 1):
+        with c.ctx(prec) as ctx:
+            ctx.set_hamiltonian(**h)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            ctx.rk4_steps(5e-4, 100)
+            res.append((ctx.download_psi(), ctx.observables(0.05)))
+    (g64, o64), (g32, o32) = res
+    l2 = float(np.sqrt(np.sum(np.abs(g32 - g64) ** 2) / np.sum(np.abs(g64) ** 2)))
+    de = max(abs(o32[2] - o64[2]) / abs(o64[2]), abs(o32[5] - o64[5]) / abs(o64[5]))
+    print(f"two-component mixed vs f64: L2 psi {l2:.3e}, rel energy {de:.3e}")
+    assert de < 1e-6 and l2 < 1e-4

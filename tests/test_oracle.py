@@ -279,3 +279,53 @@ def test_harmonic_ground_state_energy():
     hmu, hen = orc.hpsi(g, h, 0.0, out, V)
     mu, en = orc.obs2(g, orc.norm(g, out), out, hmu, hen)
     assert abs(en - 1.5) < 1e-7 and abs(mu - 1.5) < 1e-7
+
+
+def test_heavy_ball_two_component_uniform_scalar():
+    """Two uniform components reduce the 3D loop to a recursion on four
+    numbers (amplitudes and previous amplitudes), run here by hand."""
+    g = grid(4, 4, 4)
+    Vol = g.N * g.dv
+    a11, a22, a12 = 0.5, 0.7, 0.3
+    v = (0.2, 0.35)
+    dt, beta, npart, niter = 0.01, 0.6, 1.5, 6
+    A = [0.3, 0.5]
+    psi = np.stack([np.full(g.shape, A[0] + 0j), np.full(g.shape, A[1] + 0j)])
+    V = np.stack([np.full(g.shape, v[0] + 0j), np.full(g.shape, v[1] + 0j)])
+    out, n, hist = orc.imag_heavy_ball(g, orc.Ham(2, (a11, a22, a12)), psi, V, dt, beta, npart,
+                                       0.0, niter)
+    assert n == niter
+    old = list(A)
+    mu = [0.0, 0.0]
+    aa = [[a11, a12], [a12, a22]]
+    for it in range(niter):
+        n_ = [A[0] ** 2, A[1] ** 2]
+        h = [(v[c] + 4 * np.pi * (aa[c][c] * n_[c] + aa[c][1 - c] * n_[1 - c])) * A[c] for c in range(2)]
+        he = [(v[c] + 2 * np.pi * (aa[c][c] * n_[c] + aa[c][1 - c] * n_[1 - c])) * A[c] for c in range(2)]
+        B = [(1 + mu[c] * dt) * A[c] - dt * h[c] + beta * (A[c] - old[c]) for c in range(2)]
+        old = list(A)
+        for c in range(2):
+            nrm = B[c] ** 2 * Vol
+            s = (npart / nrm) ** 0.5
+            A[c] = s * B[c]
+            mu[c] = A[c] * h[c] * Vol / nrm
+            en = A[c] * he[c] * Vol / nrm
+            assert abs(hist[it, 3 * c] - nrm) < 1e-12 * nrm
+            assert abs(hist[it, 3 * c + 1] - mu[c]) < 1e-12 * abs(mu[c])
+            assert abs(hist[it, 3 * c + 2] - en) < 1e-12 * abs(en)
+    assert abs(out[0, 0, 0, 0] - A[0]) < 1e-14 and abs(out[1, 3, 2, 1] - A[1]) < 1e-14
+
+
+def test_two_component_hpsi_uniform():
+    g = grid(4, 4, 4)
+    A = (0.4 + 0.1j, 0.2 - 0.3j)
+    a11, a22, a12 = 0.5, 0.7, 0.3
+    psi = np.stack([np.full(g.shape, A[0]), np.full(g.shape, A[1])])
+    V = np.stack([np.full(g.shape, 0.1 + 0.05j), np.full(g.shape, -0.2 + 0.0j)])
+    hmu, hen = orc.hpsi(g, orc.Ham(2, (a11, a22, a12)), 0.0, psi, V)
+    n1, n2 = abs(A[0]) ** 2, abs(A[1]) ** 2
+    ref1 = (0.1 + 0.05j + 4 * np.pi * (a11 * n1 + a12 * n2)) * A[0]
+    ref2 = (-0.2 + 4 * np.pi * (a22 * n2 + a12 * n1)) * A[1]
+    assert np.abs(hmu[0] - ref1).max() < 1e-14 and np.abs(hmu[1] - ref2).max() < 1e-14
+    e1 = (0.1 + 0.05j + 2 * np.pi * (a11 * n1 + a12 * n2)) * A[0]
+    assert np.abs(hen[0] - e1).max() < 1e-14
