@@ -24,7 +24,8 @@ EXPORTS = (
     "fgpe_ctx_sync", "fgpe_ctx_device_bytes", "fgpe_set_kspace", "fgpe_set_hamiltonian",
     "fgpe_upload_psi", "fgpe_download_psi", "fgpe_upload_vext", "fgpe_psi_device_ptr",
     "fgpe_fft3d_c2c_f64", "fgpe_fft3d_c2c_f32", "fgpe_laplacian_f64", "fgpe_apply_h",
-    "fgpe_observables", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_timing_enable",
+    "fgpe_observables", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
+    "fgpe_timing_enable",
     "fgpe_timing_read",
 )
 KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
@@ -71,8 +72,11 @@ def load() -> C.CDLL:
     L.fgpe_apply_h.argtypes = [_vp, C.c_double, _vp, _vp]
     L.fgpe_observables.argtypes = [_vp, C.c_double, _dp]
     L.fgpe_rk4_steps.argtypes = [_vp, C.c_double, C.c_double, C.c_int]
-    L.fgpe_imag_steps.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+    L.fgpe_imag_steps.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _dp, C.c_double,
                                   C.c_int, _dp, C.POINTER(C.c_int)]
+    L.fgpe_calc_px.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_calc_cos.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp]
+    L.fgpe_imag_converged.argtypes = [_vp, C.POINTER(C.c_int)]
     L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
     L.fgpe_timing_read.argtypes = [_vp, _dp]
     _lib = L
@@ -197,6 +201,22 @@ class Context:
     def rk4_steps(self, dt: float, nsteps: int, t0: float = 0.0):
         check(load().fgpe_rk4_steps(self.h, dt, t0, nsteps))
 
+    def calc_px(self, comp=0) -> float:
+        v = C.c_double()
+        check(load().fgpe_calc_px(self.h, comp, C.byref(v)))
+        return v.value
+
+    def calc_cos(self, x, L, comp=0) -> float:
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        v = C.c_double()
+        check(load().fgpe_calc_cos(self.h, comp, _dptr(xx), L, C.byref(v)))
+        return v.value
+
+    def imag_converged(self) -> bool:
+        f = C.c_int(0)
+        check(load().fgpe_imag_converged(self.h, C.byref(f)))
+        return bool(f.value)
+
     def timing_enable(self, on=True):
         check(load().fgpe_timing_enable(self.h, 1 if on else 0))
 
@@ -209,6 +229,7 @@ class Context:
                    history=False):
         hist = np.zeros((max(nsteps, 1), 3 * self.ncomp)) if history else None
         done = C.c_int(0)
-        check(load().fgpe_imag_steps(self.h, scheme, dt, beta, npart, eps, nsteps,
+        np_ = np.ascontiguousarray(np.broadcast_to(np.asarray(npart, dtype=np.float64), (2,)))
+        check(load().fgpe_imag_steps(self.h, scheme, dt, beta, _dptr(np_), eps, nsteps,
                                      _dptr(hist) if history else _dp(), C.byref(done)))
         return (done.value, hist[:done.value]) if history else done.value

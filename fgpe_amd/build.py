@@ -57,8 +57,14 @@ def build_oracle() -> None:
     subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
+def build_host() -> None:
+    """The reference-shaped C++ host API, `gpe` and `gpe_params` (g++)."""
+    subprocess.run(["make", "-s", "-B", "-j8", "-C", os.path.join(HERE, "host")], check=True)
+
+
 def main() -> None:
     build_lib(verbose=True)
+    build_host()
     build_oracle()
 
 

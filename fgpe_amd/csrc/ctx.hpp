@@ -81,6 +81,9 @@ struct fgpe_ctx {
     double* hist = nullptr;
     size_t hist_n = 0;
     bool spec_valid = false;
+    bool hb_active = false;   // a heavy-ball run can be continued
+    double2* aux = nullptr;   // FP64 scratch of the diagnostics
+    double* xbuf = nullptr;   // x cell centres of fgpe_calc_cos
 
     void* tw64[3] = {nullptr, nullptr, nullptr};
     void* tw32[3] = {nullptr, nullptr, nullptr};

@@ -227,7 +227,7 @@ template <typename T> int prologue(fgpe_ctx* c)
     return FGPE_OK;
 }
 
-int finalize(fgpe_ctx* c, long long nblocks, int kind, double npart, double* out, double eps)
+int finalize(fgpe_ctx* c, long long nblocks, int kind, const double* npart, double* out, double eps)
 {
     FinArgs f{};
     f.partials = c->partials;
@@ -235,7 +235,8 @@ int finalize(fgpe_ctx* c, long long nblocks, int kind, double npart, double* out
     f.nc = c->hp.ncomp;
     f.kind = kind;
     f.dv = c->dv;
-    f.npart[0] = f.npart[1] = npart;
+    f.npart[0] = npart[0];
+    f.npart[1] = c->hp.ncomp == 2 ? npart[1] : npart[0];
     f.out = out;
     f.scale = c->scal + SC_SCALE;
     f.mu = c->scal + SC_MU;
@@ -250,8 +251,8 @@ int finalize(fgpe_ctx* c, long long nblocks, int kind, double npart, double* out
 }
 
 // RK4 steps; imaginary: c = -dt and renormalisation to npart after each step
-template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps, bool imaginary, double npart,
-                                  double* hist_dev)
+template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps, bool imaginary,
+                                  const double* npart, double* hist_dev)
 {
     FGPE_RET(ensure_ready(c));
     const int nc = c->hp.ncomp;
@@ -283,6 +284,8 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
     return FGPE_OK;
 }
 
+__global__ void k_hb_continue(int* flags) { flags[FL_DONE] = 0; }
+
 __global__ void k_hb_init(double* scal, int* flags)
 {
     for (int i = 0; i < 2; ++i) {
@@ -296,14 +299,18 @@ __global__ void k_hb_init(double* scal, int* flags)
 
 // Heavy-ball imaginary time (the reference's scheme), fully on the device: the
 // convergence test runs in k_finalize and stops every later launch.
-template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, double npart, double eps, int nsteps,
-                                 double* hist_dev)
+template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const double* npart, double eps,
+                                 int nsteps, double* hist_dev)
 {
     FGPE_RET(ensure_ready(c));
     const int nc = c->hp.ncomp;
     const bool dens = c->nch > nc;
     if (nc == 2 && dens)
         return fail(FGPE_EUNSUP, "imaginary time with two components and a k-space kernel is not available");
+    if (c->hb_active) {
+        hipLaunchKernelGGL(k_hb_continue, dim3(1), dim3(1), 0, c->stream, c->flags);
+        FGPE_LAUNCHCHK("k_hb_continue");
+    } else {
     hipLaunchKernelGGL(k_hb_init, dim3(1), dim3(1), 0, c->stream, c->scal, c->flags);
     FGPE_LAUNCHCHK("k_hb_init");
     // old = psi (src/oneComponentGPSolver.cu:33)
@@ -317,6 +324,8 @@ template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, double npa
                                c->stream, (float*)c->PS[i], (const double*)c->P0[i], n);
             FGPE_LAUNCHCHK("k_convert");
         }
+    }
+    c->hb_active = true;
     }
     FGPE_RET(prologue<T>(c));
     const long long nb = Launch<T>::xblocks(c);
@@ -346,7 +355,8 @@ template <typename T> int apply_h(fgpe_ctx* c, double t, void* hmu, void* hen, d
     a.hen = (double2*)hen;
     FGPE_RET(Launch<T>::xstage(c, a, c->hp.ncomp, c->nch > c->hp.ncomp, MODE_APPLY));
     c->spec_valid = false;
-    if (obs_dev) FGPE_RET(finalize(c, Launch<T>::xblocks(c), FIN_OBS, 1.0, obs_dev, 0.0));
+    const double one[2] = {1.0, 1.0};
+    if (obs_dev) FGPE_RET(finalize(c, Launch<T>::xblocks(c), FIN_OBS, one, obs_dev, 0.0));
     return FGPE_OK;
 }
 
@@ -373,6 +383,48 @@ int apply_pending_scale(fgpe_ctx* c)
     hipLaunchKernelGGL(k_reset_scales, dim3(1), dim3(1), 0, c->stream, c->scal + SC_SCALE);
     FGPE_LAUNCHCHK("k_reset_scales");
     c->spec_valid = false;
+    return FGPE_OK;
+}
+
+// per-block sums of w(i) |a_i|^2 with w = kx[ix] (kind 0) or
+// cos(2 pi x[ix] / L) (kind 1); deterministic block order
+__global__ void __launch_bounds__(256) k_weighted_sum(const double2* __restrict__ a, const double* __restrict__ w,
+                                                      int nx, long long n, int kind, double L, double* partials)
+{
+    double s[1] = {0.0};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ix = (int)(i % nx);
+        const double wt = kind == 0 ? w[ix] : cos(2 * kPi * w[ix] / L);
+        s[0] += wt * (a[i].x * a[i].x + a[i].y * a[i].y);
+    }
+    block_sum_store<1>(s, partials + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partials, int nb, double scale,
+                                                      double* out)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += partials[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] * scale;
+}
+
+int weighted_sum(fgpe_ctx* c, const double2* a, const double* w, int kind, double L, double* host_out)
+{
+    const int nb = 1024;
+    hipLaunchKernelGGL(k_weighted_sum, dim3(nb), dim3(256), 0, c->stream, a, w, c->n[0], c->ntot, kind, L,
+                       c->partials);
+    FGPE_LAUNCHCHK("k_weighted_sum");
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->dv, c->scal + SC_TMP);
+    FGPE_LAUNCHCHK("k_sum_partials");
+    FGPE_HIPCHK(hipMemcpyAsync(host_out, c->scal + SC_TMP, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     return FGPE_OK;
 }
 
@@ -461,7 +513,8 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     (void)hipDeviceSynchronize();
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
-                    c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist};
+                    c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
+                    c->aux, c->xbuf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -539,6 +592,7 @@ int fgpe_set_hamiltonian(fgpe_ctx* c, const fgpe_hparams* hp)
     }
     c->hset = true;
     c->spec_valid = false;
+    c->hb_active = false;
     return FGPE_OK;
 }
 
@@ -552,6 +606,7 @@ int fgpe_upload_psi(fgpe_ctx* c, int comp, const double* host)
     FGPE_LAUNCHCHK("k_reset_scales");
     FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     c->spec_valid = false;
+    c->hb_active = false;
     return FGPE_OK;
 }
 
@@ -649,15 +704,17 @@ int fgpe_observables(fgpe_ctx* c, double t, double* out)
 int fgpe_rk4_steps(fgpe_ctx* c, double dt, double t0, int nsteps)
 {
     if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    c->hb_active = false;
     if (nsteps < 0) return fail(FGPE_EINVAL, "nsteps < 0");
-    return c->mixed() ? rk4_run<float>(c, dt, t0, nsteps, false, 1.0, nullptr)
-                      : rk4_run<double>(c, dt, t0, nsteps, false, 1.0, nullptr);
+    const double one[2] = {1.0, 1.0};
+    return c->mixed() ? rk4_run<float>(c, dt, t0, nsteps, false, one, nullptr)
+                      : rk4_run<double>(c, dt, t0, nsteps, false, one, nullptr);
 }
 
-int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, double npart, double eps, int nsteps,
-                    double* host_hist, int* steps_done)
+int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, const double* npart, double eps,
+                    int nsteps, double* host_hist, int* steps_done)
 {
-    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (!c || !npart) return fail(FGPE_EINVAL, "NULL argument");
     if (nsteps < 0) return fail(FGPE_EINVAL, "nsteps < 0");
     if (scheme != FGPE_IMAG_RK4_RENORM && scheme != FGPE_IMAG_HEAVY_BALL)
         return fail(FGPE_EINVAL, "unknown imaginary-time scheme");
@@ -669,6 +726,7 @@ int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, double npar
         hd = c->hist;
     }
     if (scheme == FGPE_IMAG_RK4_RENORM) {
+        c->hb_active = false;
         FGPE_RET(c->mixed() ? rk4_run<float>(c, dt, 0.0, nsteps, true, npart, hd)
                             : rk4_run<double>(c, dt, 0.0, nsteps, true, npart, hd));
     } else {
@@ -683,6 +741,35 @@ int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, double npar
     if (hd && done > 0)
         FGPE_HIPCHK(hipMemcpy(host_hist, hd, sizeof(double) * 3 * nc * done, hipMemcpyDeviceToHost));
     if (steps_done) *steps_done = done;
+    return FGPE_OK;
+}
+
+int fgpe_calc_px(fgpe_ctx* c, int comp, double* out)
+{
+    if (!c || !out) return fail(FGPE_EINVAL, "NULL argument");
+    if (comp < 0 || comp > 1 || !c->P0[comp]) return fail(FGPE_EINVAL, "no such component");
+    if (!c->kset) return fail(FGPE_ESTATE, "fgpe_set_kspace has not been called");
+    if (!c->aux) FGPE_RET(dalloc(c, (void**)&c->aux, sizeof(double2) * c->ntot));
+    FGPE_RET(fft3d<double>(c, c->P0[comp], c->aux, -1));
+    return weighted_sum(c, c->aux, c->k[0], 0, 1.0, out);
+}
+
+int fgpe_calc_cos(fgpe_ctx* c, int comp, const double* host_x, double L, double* out)
+{
+    if (!c || !out || !host_x) return fail(FGPE_EINVAL, "NULL argument");
+    if (comp < 0 || comp > 1 || !c->P0[comp]) return fail(FGPE_EINVAL, "no such component");
+    if (!c->xbuf) FGPE_RET(dalloc(c, (void**)&c->xbuf, sizeof(double) * c->n[0]));
+    FGPE_HIPCHK(hipMemcpy(c->xbuf, host_x, sizeof(double) * c->n[0], hipMemcpyHostToDevice));
+    return weighted_sum(c, c->P0[comp], c->xbuf, 1, L, out);
+}
+
+int fgpe_imag_converged(fgpe_ctx* c, int* converged)
+{
+    if (!c || !converged) return fail(FGPE_EINVAL, "NULL argument");
+    int f = 0;
+    FGPE_HIPCHK(hipMemcpyAsync(&f, c->flags + FL_STOP, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
+    *converged = (c->hb_active && f) ? 1 : 0;
     return FGPE_OK;
 }
 

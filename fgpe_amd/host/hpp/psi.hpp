@@ -1,0 +1,40 @@
+// psi.hpp -- initial wave function (hpp/psi.hpp): Gaussian or tube with
+// seeded noise, vortex phase imprint, normalisation, read from psi_final.wdat.
+#ifndef FGPE_HOST_PSI_HPP
+#define FGPE_HOST_PSI_HPP
+
+#include <random>
+
+#include "params.hpp"
+
+class Psi {
+    complex* psi;
+    std::mt19937 generator;
+    std::uniform_real_distribution<double> distribution;
+
+public:
+    Psi(const Params& p, const std::string& type);
+    Psi(const ParamsBase& p, const int& type);
+    Psi(const Psi&) = delete;
+    Psi& operator=(const Psi&) = delete;
+
+    void initTube(const ParamsBase& p, const int& type);
+    void initHarmonic(const ParamsBase& p, const int& type);
+    void initVoid(const ParamsBase& p, const int& type);
+    void normalize(const ParamsBase& p, const int& type = -1);
+    void addNoise(complex& noise_amp, const int& length);
+    void imprintVortexTube(const ParamsBase& p, const double& Ncirc);
+    void readPsi(const std::string& prefix, const int& length);
+
+    inline int iX(int index, int NX, int) { return index % NX; }
+    inline int iY(int index, int NX, int NY) { return (index / NX) % NY; }
+    inline int iZ(int index, int NX, int NY) { return index / (NX * NY); }
+    double getX(const ParamsBase& p, int i);
+    double getY(const ParamsBase& p, int i);
+    double getZ(const ParamsBase& p, int i);
+
+    complex* getPsi() const { return psi; }
+    ~Psi() { free(psi); }
+};
+
+#endif

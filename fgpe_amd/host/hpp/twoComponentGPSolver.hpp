@@ -1,0 +1,55 @@
+// twoComponentGPSolver.hpp -- the two-component time loops and the plugin
+// virtuals (hpp/twoComponentGPSolver.hpp:11-64).
+#ifndef FGPE_HOST_TWO_COMPONENT_GP_SOLVER_HPP
+#define FGPE_HOST_TWO_COMPONENT_GP_SOLVER_HPP
+
+#include "baseHamiltonian.hpp"
+#include "psi.hpp"
+#include "vext.hpp"
+
+class TwoComponentGPSolver : public BaseHamiltonian {
+protected:
+    complex* psi1;
+    complex* psi2;
+    complex* vext1;
+    complex* vext2;
+
+public:
+    Params2 p;
+    double PARS[64] = {0.0};
+    enum Variables { EN1, EN2, MU1, MU2, NORM1, NORM2, PX1, PX2, EN1_PREV, EN2_PREV, AVG_TIME, ITER, ITER_TIME,
+                     EN1_0, EN2_0, MU1_0, MU2_0, NCYCLES, AUX1_1, AUX2_1, AUX3_1, AUX4_1, AUX5_1, AUX1_2,
+                     AUX2_2, AUX3_2, AUX4_2, AUX5_2 };
+
+    TwoComponentGPSolver(Params2& par, int precision = FGPE_PREC_F64);
+    complex* getPsi(int i) { return (i == 1) ? psi1 : psi2; }
+    complex* getVext(int i) { return (i == 1) ? vext1 : vext2; }
+
+    void runImagTimeEvol();
+    void runRealTimeEvol();
+
+    virtual void alg_calcHpsi() {}
+    virtual void alg_calcHpsiMU() {}
+
+    virtual void call_IM_init() = 0;
+    virtual void call_IM_loop_before_step() = 0;
+    virtual void call_IM_loop_after_step() = 0;
+    virtual void call_IM_loop_convergence() = 0;
+    virtual void call_IM_end() = 0;
+    virtual void call_RE_init() = 0;
+    virtual void call_RE_loop_before_step() = 0;
+    virtual void call_RE_loop_after_step() = 0;
+    virtual void call_RE_end() = 0;
+
+    void syncState() override;
+    void downloadPsi();
+    ~TwoComponentGPSolver() override;
+
+private:
+    int pending = 0;
+    double pendingT0 = 0.0;
+    double dtCode = 0.0;
+    void upload();
+};
+
+#endif

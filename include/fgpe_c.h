@@ -113,17 +113,31 @@ int fgpe_apply_h(fgpe_ctx* ctx, double t, void* dev_hmu, void* dev_hen);
    src/baseHamiltonian.cu:173-179,230-236). */
 int fgpe_observables(fgpe_ctx* ctx, double t, double* host_out);
 
+/* Diagnostics on component comp of the current state:
+   px  = DV sum_k kx |F psi|^2, which equals the reference's
+         DV sum Re(psi* F^-1[kx F psi]) (alg_calcPx, src/baseHamiltonian.cu
+         :184-195, with BECVPx and CalculateObservable)
+   cos = DV sum cos(2 pi x / L) |psi|^2 with host_x the NX cell centres
+         (CalcAverageCos, src/parallel.cu:535-561, through alg_calcCos). */
+int fgpe_calc_px(fgpe_ctx* ctx, int comp, double* host_px);
+int fgpe_calc_cos(fgpe_ctx* ctx, int comp, const double* host_x, double L, double* host_out);
+
 /* ---- time stepping ----------------------------------------------------- */
 /* nsteps real-time RK4 steps from time t0 (runRealTimeEvol's loop body,
    src/oneComponentGPSolver.cu:150-208; two components:
    src/twoComponentGPSolver.cu:142-203). */
 int fgpe_rk4_steps(fgpe_ctx* ctx, double dt, double t0, int nsteps);
-/* nsteps imaginary-time steps (scheme FGPE_IMAG_*).  npart: target norm per
-   component.  eps > 0 stops heavy-ball at |E - E_prev| / E_prev < eps
-   (src/oneComponentGPSolver.cu:84-88).  host_hist (3*nsteps doubles per
-   component or NULL) gets norm, mu, E per step; *steps_done the number run. */
-int fgpe_imag_steps(fgpe_ctx* ctx, int scheme, double dt, double beta, double npart,
+/* nsteps imaginary-time steps (scheme FGPE_IMAG_*).  host_npart: target
+   norm of each component (ncomp doubles).  eps > 0 stops heavy-ball at |E - E_prev| / E_prev < eps
+   (src/oneComponentGPSolver.cu:84-88; two components: |dE_c| < eps for
+   both, src/twoComponentGPSolver.cu:94).  host_hist (3*ncomp*nsteps doubles
+   or NULL) gets norm, mu, E per component per step; *steps_done the number
+   run.  A heavy-ball call continues the previous one (momentum, mu, E_prev,
+   convergence) unless the state or the Hamiltonian was set in between. */
+int fgpe_imag_steps(fgpe_ctx* ctx, int scheme, double dt, double beta, const double* host_npart,
                     double eps, int nsteps, double* host_hist, int* steps_done);
+/* *converged = 1 once a heavy-ball run has met its convergence test. */
+int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
 
 /* ---- instrumentation (no counterpart in the reference, which prints only
    an "Average iteration time", hamiltonians/BEC_oneComponent.cu:279) ---- */
