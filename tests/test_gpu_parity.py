@@ -296,3 +296,20 @@ def test_two_component_mixed_error():
     de = max(abs(o32[2] - o64[2]) / abs(o64[2]), abs(o32[5] - o64[5]) / abs(o64[5]))
     print(f"two-component mixed vs f64: L2 psi {l2:.3e}, rel energy {de:.3e}")
     assert de < 1e-6 and l2 < 1e-4
+
+
+def test_px_and_cos_diagnostics():
+    """<Px> and <cos> of the reference's status rows, against numpy."""
+    c = Case(32, 64, 32, L=(12.0, 16.0, 10.0))
+    psi = c.rand_psi(31, 0.3) * np.exp(1j * 2 * np.pi * 2 * np.arange(32) / 32)[None, None, :]
+    x = -6.0 + (np.arange(32) + 0.5) * 12.0 / 32
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(a=(0.0,))
+        ctx.upload_psi(psi)
+        px = ctx.calc_px()
+        cs = ctx.calc_cos(x, 12.0)
+    F = np.fft.fftn(psi)
+    ref_px = c.dv * float(np.sum(c.kx[None, None, :] * np.abs(F) ** 2))
+    ref_cos = c.dv * float(np.sum(np.cos(2 * np.pi * x / 12.0)[None, None, :] * np.abs(psi) ** 2))
+    assert abs(px - ref_px) < 1e-12 * np.sum(np.abs(F) ** 2) * c.dv * np.abs(c.kx).max()
+    assert abs(cs - ref_cos) < 1e-12 * c.dv * np.sum(np.abs(psi) ** 2)
