@@ -24,7 +24,7 @@ EXPORTS = (
     "fgpe_ctx_sync", "fgpe_ctx_device_bytes", "fgpe_set_kspace", "fgpe_set_hamiltonian",
     "fgpe_upload_psi", "fgpe_download_psi", "fgpe_upload_vext", "fgpe_psi_device_ptr",
     "fgpe_fft3d_c2c_f64", "fgpe_fft3d_c2c_f32", "fgpe_laplacian_f64", "fgpe_apply_h",
-    "fgpe_observables", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
+    "fgpe_observables", "fgpe_adjoint_forward", "fgpe_adjoint_backward", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
     "fgpe_timing_enable",
     "fgpe_timing_read",
 )
@@ -74,6 +74,8 @@ def load() -> C.CDLL:
     L.fgpe_rk4_steps.argtypes = [_vp, C.c_double, C.c_double, C.c_int]
     L.fgpe_imag_steps.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _dp, C.c_double,
                                   C.c_int, _dp, C.POINTER(C.c_int)]
+    L.fgpe_adjoint_forward.argtypes = [_vp, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.fgpe_adjoint_backward.argtypes = [_vp, _vp, _vp, _vp, _dp]
     L.fgpe_calc_px.argtypes = [_vp, C.c_int, _dp]
     L.fgpe_calc_cos.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp]
     L.fgpe_imag_converged.argtypes = [_vp, C.POINTER(C.c_int)]
@@ -200,6 +202,14 @@ class Context:
 
     def rk4_steps(self, dt: float, nsteps: int, t0: float = 0.0):
         check(load().fgpe_rk4_steps(self.h, dt, t0, nsteps))
+
+    def adjoint_forward(self, dt: float, nsteps: int, ckpt_every: int, t0: float = 0.0):
+        check(load().fgpe_adjoint_forward(self.h, dt, t0, nsteps, ckpt_every))
+
+    def adjoint_backward(self, dev_lamT: int, dev_lam0: int | None, dev_gradV: int):
+        g = np.zeros(2)
+        check(load().fgpe_adjoint_backward(self.h, dev_lamT, dev_lam0, dev_gradV, _dptr(g)))
+        return g
 
     def calc_px(self, comp=0) -> float:
         v = C.c_double()

@@ -83,6 +83,16 @@ struct fgpe_ctx {
     bool spec_valid = false;
     bool hb_active = false;   // a heavy-ball run can be continued
     double2* aux = nullptr;   // FP64 scratch of the diagnostics
+    // adjoint: checkpoints, segment states, stage states, VJP buffers
+    std::vector<double2*> ckpt;
+    std::vector<double2*> segbuf;
+    void* ystage[3] = {nullptr, nullptr, nullptr};
+    void* spec3 = nullptr;
+    double2* lam[3] = {nullptr, nullptr, nullptr};   // LIN, LAM, GB
+    double* pacc = nullptr;
+    int adj_every = 0, adj_nsteps = 0;
+    double adj_dt = 0.0, adj_t0 = 0.0;
+    bool adj_ready = false;
     double* xbuf = nullptr;   // x cell centres of fgpe_calc_cos
 
     void* tw64[3] = {nullptr, nullptr, nullptr};
@@ -121,6 +131,8 @@ template <typename T> struct Launch {
     static int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter,
                     const int* stop);
     static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode);
+    // adjoint VJP prologue / epilogue; y64: the state is FP64 (else T)
+    static int adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epilogue);
 };
 
 }  // namespace fgpe

@@ -164,18 +164,24 @@ template <typename T> int fft3d(fgpe_ctx* c, const void* in, void* out, int dir)
 
 // y-forward / last axis / y-inverse on the spectral channels.  `scale` is
 // applied by the first forward pass after a prologue or a final stage.
-template <typename T> int spectral_passes(fgpe_ctx* c, const double* scale, const int* stop)
+template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, const double* scale,
+                                             const int* stop)
 {
     const bool filter = has_filter(c->hp);
     const int nc = c->hp.ncomp;
     if (last_axis(c) == 2) {
-        FGPE_RET(Launch<T>::strided(c, c->spec, 1, -1, c->nch, nc, scale, stop));
-        FGPE_RET(Launch<T>::last(c, c->spec, 2, c->nch, nc, nullptr, filter, stop));
-        FGPE_RET(Launch<T>::strided(c, c->spec, 1, +1, c->nch, nc, nullptr, stop));
+        FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop));
+        FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop));
+        FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop));
     } else {
-        FGPE_RET(Launch<T>::last(c, c->spec, 1, c->nch, nc, scale, filter, stop));
+        FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop));
     }
     return FGPE_OK;
+}
+
+template <typename T> int spectral_passes(fgpe_ctx* c, const double* scale, const int* stop)
+{
+    return spectral_passes_on<T>(c, c->spec, c->nch, scale, stop);
 }
 
 template <typename T> XArgs<T> base_xargs(fgpe_ctx* c, double t)
@@ -428,6 +434,177 @@ int weighted_sum(fgpe_ctx* c, const double2* a, const double* w, int kind, doubl
     return FGPE_OK;
 }
 
+__global__ void k_accumulate_impl(double* __restrict__ acc, const double* __restrict__ add, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] += add[i];
+}
+
+int adj_check(fgpe_ctx* c)
+{
+    FGPE_RET(ensure_ready(c));
+    if (c->hp.ncomp != 1) return fail(FGPE_EUNSUP, "the adjoint is one-component");
+    if (c->hp.quench_on) return fail(FGPE_EUNSUP, "the adjoint needs a constant a (no a(t) ramp)");
+    if (c->hp.sc_U != 0.0) return fail(FGPE_EUNSUP, "the adjoint supports the dipolar k-space kernel only");
+    if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
+    return FGPE_OK;
+}
+
+template <typename T> int adj_forward(fgpe_ctx* c, double dt, double t0, int nsteps, int every)
+{
+    FGPE_RET(adj_check(c));
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t bytes = sizeof(double2) * c->ntot;
+    while ((int)c->ckpt.size() < nseg) {
+        double2* p = nullptr;
+        FGPE_RET(dalloc(c, (void**)&p, bytes));
+        c->ckpt.push_back(p);
+    }
+    const double one[2] = {1.0, 1.0};
+    for (int seg = 0; seg < nseg; ++seg) {
+        const int len = std::min(every, nsteps - seg * every);
+        FGPE_HIPCHK(hipMemcpyAsync(c->ckpt[seg], c->P0[0], bytes, hipMemcpyDeviceToDevice, c->stream));
+        FGPE_RET(rk4_run<T>(c, dt, t0 + (double)seg * every * dt, len, false, one, nullptr));
+    }
+    c->adj_every = every;
+    c->adj_nsteps = nsteps;
+    c->adj_dt = dt;
+    c->adj_t0 = t0;
+    c->adj_ready = true;
+    return FGPE_OK;
+}
+
+// One VJP: prologue (abar, n, r channels), spectral passes, epilogue.
+template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, double* ptmp)
+{
+    FGPE_RET(Launch<T>::adj(c, a, y64, dens, false));
+    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? 3 : 1, nullptr, nullptr));
+    a.partials = ptmp;
+    FGPE_RET(Launch<T>::adj(c, a, y64, dens, true));
+    const long long n = 2 * Launch<T>::xblocks(c);
+    hipLaunchKernelGGL(k_accumulate_impl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->pacc,
+                       ptmp, n);
+    FGPE_LAUNCHCHK("k_accumulate");
+    return FGPE_OK;
+}
+
+template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0, double* vbar, double* grads)
+{
+    FGPE_RET(adj_check(c));
+    if (!c->adj_ready) return fail(FGPE_ESTATE, "fgpe_adjoint_forward has not been run");
+    const size_t N = (size_t)c->ntot;
+    const size_t b64 = sizeof(double2) * N;
+    const bool dens = c->nch > 1;
+    const int every = c->adj_every, nsteps = c->adj_nsteps;
+    const int nseg = (nsteps + every - 1) / every;
+    const double dt = c->adj_dt;
+    const long long nb = Launch<T>::xblocks(c);
+    while ((int)c->segbuf.size() < every) {
+        double2* p = nullptr;
+        FGPE_RET(dalloc(c, (void**)&p, b64));
+        c->segbuf.push_back(p);
+    }
+    for (int i = 0; i < 3; ++i) {
+        if (!c->ystage[i]) FGPE_RET(dalloc(c, &c->ystage[i], 2 * c->tsz() * N));
+        if (!c->lam[i]) FGPE_RET(dalloc(c, (void**)&c->lam[i], b64));
+    }
+    if (!c->spec3) FGPE_RET(dalloc(c, &c->spec3, 3 * 2 * c->tsz() * N));
+    if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)nb));
+    double* ptmp = c->pacc + 2 * nb;
+    FGPE_HIPCHK(hipMemsetAsync(c->pacc, 0, sizeof(double) * 2 * nb, c->stream));
+    FGPE_HIPCHK(hipMemsetAsync(vbar, 0, sizeof(double) * N, c->stream));
+    double2 *LIN = c->lam[0], *LAM = c->lam[1], *GB = c->lam[2];
+    FGPE_HIPCHK(hipMemcpyAsync(LIN, lamT, b64, hipMemcpyDefault, c->stream));
+    const double one[2] = {1.0, 1.0};
+    const double2 cc = double2{0.0, dt};   // conj(c), c = -i dt
+    auto scl = [](double s, double2 z) { return double2{s * z.x, s * z.y}; };
+    const double g4 = 4 * kPi * c->hp.a[0];
+
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int len = std::min(every, nsteps - seg * every);
+        const int step0 = seg * every;
+        // recompute the states of this segment
+        FGPE_HIPCHK(hipMemcpyAsync(c->P0[0], c->ckpt[seg], b64, hipMemcpyDeviceToDevice, c->stream));
+        c->spec_valid = false;
+        for (int j = 0; j < len; ++j) {
+            FGPE_HIPCHK(hipMemcpyAsync(c->segbuf[j], c->P0[0], b64, hipMemcpyDeviceToDevice, c->stream));
+            if (j + 1 < len) FGPE_RET(rk4_run<T>(c, dt, c->adj_t0 + (step0 + j) * dt, 1, false, one, nullptr));
+        }
+        for (int j = len - 1; j >= 0; --j) {
+            double2* Y0 = c->segbuf[j];
+            // stage states y1, y2, y3 of this step (src/oneComponentGPSolver.cu:165-182)
+            {
+                XArgs<T> a = base_xargs<T>(c, 0.0);
+                a.P0[0] = Y0;
+                FGPE_RET(Launch<T>::xstage(c, a, 1, dens, MODE_PROLOGUE));
+                for (int s = 1; s <= 3; ++s) {
+                    FGPE_RET(spectral_passes<T>(c, nullptr, nullptr));
+                    XArgs<T> b = base_xargs<T>(c, 0.0);
+                    b.P0[0] = Y0;
+                    b.PS[0] = (c2_t<T>*)(s == 1 ? c->ystage[0] : c->ystage[s - 2]);
+                    b.PSo[0] = (c2_t<T>*)c->ystage[s - 1];
+                    b.cr = 0.0;
+                    b.ci = (s <= 2) ? -0.5 * dt : -1.0 * dt;
+                    b.stage = s;
+                    FGPE_RET(Launch<T>::xstage(c, b, 1, dens, MODE_RK));
+                }
+                c->spec_valid = false;
+            }
+            AdjArgs a{};
+            a.spec = c->spec3;
+            a.ntot = c->ntot;
+            a.Vr = c->Vr[0];
+            a.g4 = g4;
+            a.lam_in = LIN;
+            a.lam = LAM;
+            a.abar = GB;
+            a.vbar = vbar;
+            // s = 3: abar = conj(c)/6 lam_in at y3; lam = lam_in + G; next abar = conj(c)/3 lam_in + conj(c) G
+            a.y = c->ystage[2];
+            a.src = LIN;
+            a.alpha = scl(1.0 / 6.0, cc);
+            a.init = 1;
+            a.gb_out = GB;
+            a.ga = scl(1.0 / 3.0, cc);
+            a.gg = cc;
+            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
+            // s = 2 at y2: next abar = conj(c)/3 lam_in + conj(c)/2 G
+            a.y = c->ystage[1];
+            a.src = nullptr;
+            a.init = 0;
+            a.ga = scl(1.0 / 3.0, cc);
+            a.gg = scl(0.5, cc);
+            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
+            // s = 1 at y1: next abar = conj(c)/6 lam_in + conj(c)/2 G
+            a.y = c->ystage[0];
+            a.ga = scl(1.0 / 6.0, cc);
+            a.gg = scl(0.5, cc);
+            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
+            // s = 0 at y
+            a.y = Y0;
+            a.gb_out = nullptr;
+            FGPE_RET(vjp<T>(c, a, true, dens, ptmp));
+            std::swap(LIN, LAM);
+        }
+    }
+    c->lam[0] = LIN;
+    c->lam[1] = LAM;
+    if (lam0) FGPE_HIPCHK(hipMemcpyAsync(lam0, LIN, b64, hipMemcpyDefault, c->stream));
+    // fixed-order sum of the per-block accumulators
+    std::vector<double> h(2 * (size_t)nb);
+    FGPE_HIPCHK(hipMemcpyAsync(h.data(), c->pacc, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
+    double s0 = 0.0, s1 = 0.0;
+    for (long long b = 0; b < nb; ++b) {
+        s0 += h[2 * b];
+        s1 += h[2 * b + 1];
+    }
+    grads[0] = 4 * kPi * s0;
+    grads[1] = c->hp.a_dd != 0.0 ? s1 / c->hp.a_dd : 0.0;
+    c->spec_valid = false;
+    return FGPE_OK;
+}
+
 int ensure_hist(fgpe_ctx* c, size_t n)
 {
     if (n > c->hist_n) {
@@ -514,9 +691,12 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
-                    c->aux, c->xbuf};
+                    c->aux, c->xbuf, c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
+                    c->lam[1], c->lam[2], c->pacc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (double2* p : c->ckpt) (void)hipFree(p);
+    for (double2* p : c->segbuf) (void)hipFree(p);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -761,6 +941,24 @@ int fgpe_calc_cos(fgpe_ctx* c, int comp, const double* host_x, double L, double*
     if (!c->xbuf) FGPE_RET(dalloc(c, (void**)&c->xbuf, sizeof(double) * c->n[0]));
     FGPE_HIPCHK(hipMemcpy(c->xbuf, host_x, sizeof(double) * c->n[0], hipMemcpyHostToDevice));
     return weighted_sum(c, c->P0[comp], c->xbuf, 1, L, out);
+}
+
+int fgpe_adjoint_forward(fgpe_ctx* c, double dt, double t0, int nsteps, int ckpt_every)
+{
+    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (nsteps < 1 || ckpt_every < 1) return fail(FGPE_EINVAL, "nsteps and ckpt_every must be positive");
+    c->hb_active = false;
+    c->adj_ready = false;
+    return c->mixed() ? adj_forward<float>(c, dt, t0, nsteps, ckpt_every)
+                      : adj_forward<double>(c, dt, t0, nsteps, ckpt_every);
+}
+
+int fgpe_adjoint_backward(fgpe_ctx* c, const void* dev_lamT, void* dev_lam0, double* dev_gradV, double* host_grad)
+{
+    if (!c || !dev_lamT || !dev_gradV || !host_grad) return fail(FGPE_EINVAL, "NULL argument");
+    c->hb_active = false;
+    return c->mixed() ? adj_backward<float>(c, dev_lamT, dev_lam0, dev_gradV, host_grad)
+                      : adj_backward<double>(c, dev_lamT, dev_lam0, dev_gradV, host_grad);
 }
 
 int fgpe_imag_converged(fgpe_ctx* c, int* converged)

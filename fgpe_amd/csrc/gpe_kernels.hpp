@@ -265,6 +265,7 @@ template <typename T> struct XArgs {
     long long ntot;
     double2* P0[2];         // FP64 state psi_n (RK: psi_old of the step)
     c2_t<T>* PS[2];         // RK: stage state; HB: previous psi
+    c2_t<T>* PSo[2];        // RK stages 1-3: where the next stage state goes (null: PS)
     c2_t<T>* K[2];          // RK accumulator
     const T* Vr[2];         // Re V_c
     const T* Vi[2];         // Im V_c or null
@@ -416,7 +417,7 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
                             a.K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
                         }
                         const T2 st = from_d2<T>(nw);
-                        a.PS[c][g] = st;
+                        (a.PSo[c] ? a.PSo[c] : a.PS[c])[g] = st;
                         v[q] = st;
                     } else {
                         // FinalRKStep: psi = old + c*(k + h)/6
@@ -448,6 +449,145 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
 #pragma unroll
         for (int q = 0; q < E; ++q) specd[elemB<N>(t, q)] = v[q];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the one-component RK4 step.  With g(psi) = H(psi) psi,
+// H = T + V + g4 |psi|^2 + Phi[|psi|^2] (real V), the transposed Jacobian
+// (real inner product Re<u, v>) acting on abar is
+//     G^T abar = H(psi) abar + 2 (g4 r + Phi[r]) psi,   r = Re(conj(psi) abar)
+// and the parameter derivatives contribute  dV += r,  d(4 pi a) += n r,
+// d(a_dd) += Phi1[n] r  (Phi1 = Phi / a_dd).  One VJP is the prologue below,
+// the spectral passes on three channels (abar: kinetic; n and r: the k-space
+// kernel), and the epilogue.
+
+struct AdjArgs {
+    void* spec;             // 3 channels of T2 (abar, n, r)
+    long long ntot;
+    const void* y;          // state the Jacobian is taken at (YT2)
+    const double2* src;     // prologue: abar = alpha * src (null: abar read from `abar`)
+    double2 alpha;
+    double2* abar;          // abar of this VJP (written by the prologue when src is set)
+    const void* Vr;         // Re V (T)
+    double g4;              // 4 pi a
+    // epilogue
+    const double2* lam_in;  // adjoint of the step output
+    double2* lam;           // accumulator: lam = lam_in + G (init) or lam += G
+    int init;
+    double2* gb_out;        // next abar = ga * lam_in + gg * G (null: none)
+    double2 ga, gg;
+    double* vbar;           // += r
+    double* partials;       // per block: += (sum n r, sum Phi[n] r)
+};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+template <typename T, typename YT, int N, bool DENS>
+__global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
+k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
+{
+    using T2 = c2_t<T>;
+    using Y2 = c2_t<YT>;
+    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    __shared__ T2 lds[P * lds_stride<N>()];
+    const int t = threadIdx.x % TT, p = threadIdx.x / TT;
+    const long long row0 = ((long long)blockIdx.x * P + p) * N;
+    T2* my = lds + p * lds_stride<N>();
+    T2* sp = (T2*)a.spec;
+    const Y2* y = (const Y2*)a.y;
+    T2 v[E];
+    double nn[E], rr[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const long long g = row0 + elemA<N>(t, q);
+        double2 ab;
+        if (a.src) {
+            ab = zmul(a.alpha, a.src[g]);
+            a.abar[g] = ab;
+        } else {
+            ab = a.abar[g];
+        }
+        const Y2 yr = y[g];
+        const double yx = (double)yr.x, yy = (double)yr.y;
+        nn[q] = yx * yx + yy * yy;
+        rr[q] = yx * ab.x + yy * ab.y;
+        v[q] = T2{(T)ab.x, (T)ab.y};
+    }
+    fft_forward<T2, N>(v, t, my, tw);
+#pragma unroll
+    for (int q = 0; q < E; ++q) sp[row0 + elemB<N>(t, q)] = v[q];
+    if (DENS) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = T2{(T)nn[q], (T)0};
+        fft_forward<T2, N>(v, t, my, tw);
+#pragma unroll
+        for (int q = 0; q < E; ++q) sp[a.ntot + row0 + elemB<N>(t, q)] = v[q];
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = T2{(T)rr[q], (T)0};
+        fft_forward<T2, N>(v, t, my, tw);
+#pragma unroll
+        for (int q = 0; q < E; ++q) sp[2 * a.ntot + row0 + elemB<N>(t, q)] = v[q];
+    }
+}
+
+template <typename T, typename YT, int N, bool DENS>
+__global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
+k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
+{
+    using T2 = c2_t<T>;
+    using Y2 = c2_t<YT>;
+    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    __shared__ T2 lds[P * lds_stride<N>()];
+    const int t = threadIdx.x % TT, p = threadIdx.x / TT;
+    const long long row0 = ((long long)blockIdx.x * P + p) * N;
+    T2* my = lds + p * lds_stride<N>();
+    T2* sp = (T2*)a.spec;
+    const Y2* y = (const Y2*)a.y;
+    const T* Vr = (const T*)a.Vr;
+    T2 v[E];
+    double phin[E], phir[E];
+    if (DENS) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = sp[a.ntot + row0 + elemB<N>(t, q)];
+        fft_inverse<T2, N>(v, t, my, tw);
+#pragma unroll
+        for (int q = 0; q < E; ++q) phin[q] = (double)v[q].x;
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = sp[2 * a.ntot + row0 + elemB<N>(t, q)];
+        fft_inverse<T2, N>(v, t, my, tw);
+#pragma unroll
+        for (int q = 0; q < E; ++q) phir[q] = (double)v[q].x;
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = sp[row0 + elemB<N>(t, q)];
+    fft_inverse<T2, N>(v, t, my, tw);   // T abar
+    double s[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const long long g = row0 + elemA<N>(t, q);
+        const Y2 yr = y[g];
+        const double2 yv = double2{(double)yr.x, (double)yr.y};
+        const double2 ab = a.abar[g];
+        const double n = yv.x * yv.x + yv.y * yv.y;
+        const double r = yv.x * ab.x + yv.y * ab.y;
+        const double pn = DENS ? phin[q] : 0.0;
+        const double pr = DENS ? phir[q] : 0.0;
+        const double w = (double)Vr[g] + a.g4 * n + pn;
+        const double c2 = 2.0 * (a.g4 * r + pr);
+        const double2 G = double2{(double)v[q].x + w * ab.x + c2 * yv.x, (double)v[q].y + w * ab.y + c2 * yv.y};
+        const double2 li = a.lam_in[g];
+        const double2 base = a.init ? li : a.lam[g];
+        a.lam[g] = double2{base.x + G.x, base.y + G.y};
+        if (a.gb_out) {
+            const double2 u = zmul(a.ga, li), wv = zmul(a.gg, G);
+            a.gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
+        }
+        a.vbar[g] += r;
+        s[0] += n * r;
+        s[1] += pn * r;
+    }
+    double tot[2] = {s[0], s[1]};
+    block_sum_store<2>(tot, a.partials + 2 * (long long)blockIdx.x + 0);
 }
 
 // ---------------------------------------------------------------------------

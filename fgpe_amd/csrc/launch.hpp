@@ -120,7 +120,37 @@ template <typename T, int N> int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc
     return dens ? xstage_m<T, N, 2, true>(c, a, mode) : xstage_m<T, N, 2, false>(c, a, mode);
 }
 
+template <typename T, typename YT, int N, bool DENS> int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
+{
+    constexpr int P = px_of<T, N>();
+    const long long rows = c->ntot / N;
+    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
+    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
+    TScope ts(c, FGPE_KCLASS_XSTAGE);
+    if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    else hipLaunchKernelGGL((k_adj_pro<T, YT, N, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    FGPE_LAUNCHCHK("k_adj");
+    return FGPE_OK;
+}
+
+template <typename T, int N> int adj_n(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
+{
+    if (y64) return dens ? adj_t<T, double, N, true>(c, a, epi) : adj_t<T, double, N, false>(c, a, epi);
+    return dens ? adj_t<T, T, N, true>(c, a, epi) : adj_t<T, T, N, false>(c, a, epi);
+}
+
 }  // namespace detail
+
+template <typename T> int Launch<T>::adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
+{
+    switch (c->n[0]) {
+#define CASE(N_) \
+    case N_: return detail::adj_n<T, N_>(c, a, y64, dens, epi);
+        FGPE_FOR_EACH_N(CASE)
+#undef CASE
+    }
+    return fail(FGPE_EUNSUP, "unsupported nx");
+}
 
 template <typename T> long long Launch<T>::xblocks(fgpe_ctx* c)
 {

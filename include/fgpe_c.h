@@ -139,6 +139,22 @@ int fgpe_imag_steps(fgpe_ctx* ctx, int scheme, double dt, double beta, const dou
 /* *converged = 1 once a heavy-ball run has met its convergence test. */
 int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
 
+/* ---- adjoint (no counterpart in the reference) --------------------------
+   Gradients of a real objective J(psi(T)) of nsteps real-time RK4 steps with
+   respect to the trap V (N doubles), a and a_dd, by the discrete adjoint of
+   the RK4 step (one component, constant a, dipolar kernel, real V).
+   fgpe_adjoint_forward runs the steps from the current state and keeps a
+   checkpoint every ckpt_every steps (about sqrt(nsteps) keeps the memory at
+   O(sqrt(nsteps)) states); psi(T) is then the current state.
+   fgpe_adjoint_backward takes dev_lamT = dJ/dRe(psi) + i dJ/dIm(psi) at T
+   (N complex doubles, device), recomputes each segment from its checkpoint,
+   and writes dJ/dpsi(0) in the same convention to dev_lam0 (or NULL),
+   dJ/dV to dev_gradV and host_grad[0] = dJ/da, host_grad[1] = dJ/da_dd.
+   The state is left at the start of the last segment. */
+int fgpe_adjoint_forward(fgpe_ctx* ctx, double dt, double t0, int nsteps, int ckpt_every);
+int fgpe_adjoint_backward(fgpe_ctx* ctx, const void* dev_lamT, void* dev_lam0, double* dev_gradV,
+                          double* host_grad);
+
 /* ---- instrumentation (no counterpart in the reference, which prints only
    an "Average iteration time", hamiltonians/BEC_oneComponent.cu:279) ---- */
 #define FGPE_KCLASS_XSTAGE 0   /* fused x pass (pointwise H + RK stage)      */

@@ -1,0 +1,141 @@
+"""Adjoint gradients of real-time RK4 (fgpe_adjoint_*): against central
+finite differences of the HIP forward, against finite differences of the CPU
+FP64 oracle, through torch.autograd, and mixed against FP64."""
+import numpy as np
+import pytest
+
+from fgpe_amd import configs
+from oracle import oracle as orc
+from tests.cases import Case
+
+pytestmark = pytest.mark.gpu
+
+
+def setup_case():
+    c = Case(32, 32, 64, L=(10.0, 10.0, 20.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25)).real
+    psi0 = configs.normalise(np.exp(-0.5 * configs.harmonic_trap(c.spec, (1, 1, 0.5)).real) +
+                             0.01 * c.rand_psi(41), c.dv)
+    target = configs.normalise(np.exp(-0.6 * configs.harmonic_trap(c.spec, (1, 1, 0.4)).real) *
+                               np.exp(0.2j * c.rand_psi(42).real), c.dv)
+    a, a_dd = 0.6, 0.5
+    return c, V, psi0, target, a, a_dd
+
+
+def smooth_field(c, seed):
+    u = c.rand_psi(seed).real
+    F = np.fft.fftn(u)
+    KZ, KY, KX = np.meshgrid(c.kz, c.ky, c.kx, indexing="ij")
+    F *= np.exp(-4.0 * (KX ** 2 + KY ** 2 + KZ ** 2))
+    f = np.fft.ifftn(F).real
+    return f / np.abs(f).max()
+
+
+def J_of(psiT, target, dv):
+    ov = np.sum(np.conj(target) * psiT) * dv
+    return float(abs(ov) ** 2), ov
+
+
+def gpu_forward(c, V, a, a_dd, psi0, dt, n, prec=0):
+    with c.ctx(prec) as ctx:
+        ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0))
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        ctx.rk4_steps(dt, n)
+        return ctx.download_psi()
+
+
+def gpu_gradient(torch, c, V, a, a_dd, psi0, target, dt, n, every, prec=0):
+    with c.ctx(prec) as ctx:
+        ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0))
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        ctx.adjoint_forward(dt, n, every)
+        psiT = ctx.download_psi()
+        J, ov = J_of(psiT, target, c.dv)
+        lamT = torch.from_numpy(2 * ov * c.dv * target).cuda()
+        gV = torch.empty(c.shape, dtype=torch.float64, device="cuda")
+        lam0 = torch.empty_like(lamT)
+        g = ctx.adjoint_backward(lamT.data_ptr(), lam0.data_ptr(), gV.data_ptr())
+        ctx.sync()
+        return J, gV.cpu().numpy(), g, lam0.cpu().numpy()
+
+
+def test_adjoint_against_finite_differences(torch_cuda):
+    c, V, psi0, target, a, a_dd = setup_case()
+    dt, n, every = 5e-4, 40, 7       # uneven last segment on purpose
+    J, gV, g, lam0 = gpu_gradient(torch_cuda, c, V, a, a_dd, psi0, target, dt, n, every)
+    h = 1e-4
+    for j in range(4):
+        dV = smooth_field(c, 100 + j)
+        Jp = J_of(gpu_forward(c, V + h * dV, a, a_dd, psi0, dt, n), target, c.dv)[0]
+        Jm = J_of(gpu_forward(c, V - h * dV, a, a_dd, psi0, dt, n), target, c.dv)[0]
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(np.sum(gV * dV))
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (j, ad, fd)
+    for k, (pa, pd) in enumerate(((h, 0.0), (0.0, h))):
+        Jp = J_of(gpu_forward(c, V, a + pa, a_dd + pd, psi0, dt, n), target, c.dv)[0]
+        Jm = J_of(gpu_forward(c, V, a - pa, a_dd - pd, psi0, dt, n), target, c.dv)[0]
+        fd = (Jp - Jm) / (2 * h)
+        assert abs(g[k] - fd) <= 1e-6 * abs(fd), (k, g[k], fd)
+    # initial-state gradient: directional derivative along a complex perturbation
+    dpsi = c.rand_psi(77, 1e-2)
+    Jp = J_of(gpu_forward(c, V, a, a_dd, psi0 + h * dpsi, dt, n), target, c.dv)[0]
+    Jm = J_of(gpu_forward(c, V, a, a_dd, psi0 - h * dpsi, dt, n), target, c.dv)[0]
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(np.sum(np.real(np.conj(lam0) * dpsi)))
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+
+
+def test_adjoint_against_oracle_finite_differences(torch_cuda):
+    """The FP64 reference gradient: central differences of the CPU oracle."""
+    c, V, psi0, target, a, a_dd = setup_case()
+    dt, n, every = 5e-4, 20, 5
+    J, gV, g, _ = gpu_gradient(torch_cuda, c, V, a, a_dd, psi0, target, dt, n, every)
+    h = 1e-4
+    for j in range(2):
+        dV = smooth_field(c, 200 + j)
+        Js = []
+        for sgn in (1, -1):
+            hm = orc.Ham(1, (a,), vtilde=orc.vdd(c.grid, a_dd, (0, 0, 1)))
+            out = orc.rk4(c.grid, hm, psi0, (V + sgn * h * dV).astype(np.complex128), dt, n)
+            Js.append(J_of(out, target, c.dv)[0])
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float(np.sum(gV * dV))
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (j, ad, fd)
+
+
+def test_adjoint_through_autograd(torch_cuda):
+    torch = torch_cuda
+    from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
+    c, V, psi0, target, a, a_dd = setup_case()
+    dt, n = 5e-4, 30
+    solver = AdjointSolver(c.spec, dt, n)
+    Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
+    at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    addt = torch.tensor(a_dd, dtype=torch.float64, requires_grad=True)
+    p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda", requires_grad=True)
+    tg = torch.tensor(target, dtype=torch.complex128, device="cuda")
+    psiT = GPEEvolve().apply(Vt, at, addt, p0, solver)
+    loss = fidelity(psiT, tg, c.dv)
+    loss.backward()
+    J, gV, g, lam0 = gpu_gradient(torch, c, V, a, a_dd, psi0, target, dt, n, solver.every)
+    assert abs(loss.item() - J) < 1e-14
+    assert np.abs(Vt.grad.cpu().numpy() - gV).max() <= 1e-13 * np.abs(gV).max()
+    assert abs(at.grad.item() - g[0]) <= 1e-13 * abs(g[0])
+    assert abs(addt.grad.item() - g[1]) <= 1e-13 * abs(g[1])
+    assert np.abs(p0.grad.cpu().numpy() - lam0).max() <= 1e-13 * np.abs(lam0).max()
+    solver.ctx.close()
+
+
+def test_adjoint_mixed_precision(torch_cuda):
+    c, V, psi0, target, a, a_dd = setup_case()
+    dt, n, every = 5e-4, 40, 7
+    _, g64, s64, _ = gpu_gradient(torch_cuda, c, V, a, a_dd, psi0, target, dt, n, every, prec=0)
+    _, g32, s32, _ = gpu_gradient(torch_cuda, c, V, a, a_dd, psi0, target, dt, n, every, prec=1)
+    rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    print(f"mixed vs f64 adjoint: rel L2 dJ/dV {rel:.3e}, da {abs(s32[0]-s64[0])/abs(s64[0]):.3e}, "
+          f"da_dd {abs(s32[1]-s64[1])/abs(s64[1]):.3e}")
+    assert rel < 1e-4
+    assert abs(s32[0] - s64[0]) <= 1e-4 * abs(s64[0])
+    assert abs(s32[1] - s64[1]) <= 1e-4 * abs(s64[1])
