@@ -36,6 +36,7 @@ def run_form(torch, w, psi, V, precision, steps, warmup):
     ctx.set_hamiltonian(a=w.a, a_dd=w.a_dd, dip=w.dip)
     ctx.upload_psi(psi)
     ctx.upload_vext(V)
+    tune = ctx.autotune(5)
     t = 0.0
     ctx.rk4_steps(w.dt, warmup, t)
     t += warmup * w.dt
@@ -56,8 +57,10 @@ def run_form(torch, w, psi, V, precision, steps, warmup):
     t += steps * w.dt
     obs = ctx.observables(t)
     out = ctx.download_psi()
+    chosen = {k: v for k, v in ctx.variants().items() if k[1] == {"xstage": 0, "strided": 1, "last": 2}[k[0]]}
     ctx.close()
-    return ms / steps, kt, obs, out
+    return ms / steps, kt, obs, out, {"autotune_ms": {k: [round(x, 4) for x in v] for k, v in tune.items()},
+                                      "variant": {k[0]: v for k, v in chosen.items()}}
 
 
 def cpu_baseline(w, psi, V, nsteps, threads):
@@ -100,9 +103,9 @@ def main():
 
     res = {}
     for name, prec in (("f64", FGPE_PREC_F64), ("mixed", FGPE_PREC_MIXED)):
-        ms, kt, obs, out = run_form(torch, w, psi, V, prec, args.steps, args.warmup)
-        res[name] = dict(ms=ms, kt=kt, obs=obs, psi=out)
-        print(f"# {name}: {ms:.3f} ms/step, {1000.0 / ms:.2f} steps/s, kernels {kt}",
+        ms, kt, obs, out, tune = run_form(torch, w, psi, V, prec, args.steps, args.warmup)
+        res[name] = dict(ms=ms, kt=kt, obs=obs, psi=out, tune=tune)
+        print(f"# {name}: {ms:.3f} ms/step, {1000.0 / ms:.2f} steps/s, kernels {kt}, tune {tune}",
               file=sys.stderr, flush=True)
 
     N = nx * ny * nz
@@ -147,6 +150,7 @@ def main():
                          "bytes_per_point_mixed": roofline.step_bytes_per_point(True, nch),
                          "f64_GBs": round(step_gbs, 1),
                          "f64_frac_of_peak": round(step_gbs / roofline.HBM_PEAK_GBS, 4)},
+        "autotune": {"f64": f64["tune"], "mixed": mixed["tune"]},
         "kernels_f64_ms_avg": {k: round(v[0] / v[1], 4) for k, v in kt.items()},
         "kernels_mixed_ms_avg": {k: round(v[0] / v[1], 4) for k, v in mixed["kt"].items() if v[1] > 0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),

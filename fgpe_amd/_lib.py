@@ -26,7 +26,7 @@ EXPORTS = (
     "fgpe_fft3d_c2c_f64", "fgpe_fft3d_c2c_f32", "fgpe_laplacian_f64", "fgpe_apply_h",
     "fgpe_observables", "fgpe_adjoint_forward", "fgpe_adjoint_backward", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
     "fgpe_timing_enable",
-    "fgpe_timing_read",
+    "fgpe_timing_read", "fgpe_autotune", "fgpe_set_variant", "fgpe_get_variant",
 )
 KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
 
@@ -79,6 +79,9 @@ def load() -> C.CDLL:
     L.fgpe_calc_px.argtypes = [_vp, C.c_int, _dp]
     L.fgpe_calc_cos.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp]
     L.fgpe_imag_converged.argtypes = [_vp, C.POINTER(C.c_int)]
+    L.fgpe_autotune.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_set_variant.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.fgpe_get_variant.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
     L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
     L.fgpe_timing_read.argtypes = [_vp, _dp]
     _lib = L
@@ -226,6 +229,24 @@ class Context:
         f = C.c_int(0)
         check(load().fgpe_imag_converged(self.h, C.byref(f)))
         return bool(f.value)
+
+    def autotune(self, reps=5):
+        """Time both plan variants per kernel class; returns {class: (ms v0, ms v1)}."""
+        out = np.zeros(6)
+        check(load().fgpe_autotune(self.h, reps, _dptr(out)))
+        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6])}
+
+    def set_variant(self, kclass: int, axis: int, variant: int):
+        check(load().fgpe_set_variant(self.h, kclass, axis, variant))
+
+    def variants(self) -> dict:
+        out = {}
+        for name, k in (("xstage", 0), ("strided", 1), ("last", 2)):
+            for axis in range(3):
+                v = C.c_int()
+                check(load().fgpe_get_variant(self.h, k, axis, C.byref(v)))
+                out[(name, axis)] = v.value
+        return out
 
     def timing_enable(self, on=True):
         check(load().fgpe_timing_enable(self.h, 1 if on else 0))

@@ -97,6 +97,7 @@ struct fgpe_ctx {
 
     void* tw64[3] = {nullptr, nullptr, nullptr};
     void* tw32[3] = {nullptr, nullptr, nullptr};
+    int var[FGPE_NKCLASS][3] = {};   // plan variant per kernel class and axis
 
     // launch timing (fgpe_timing_enable)
     bool timing = false;

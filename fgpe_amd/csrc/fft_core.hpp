@@ -143,34 +143,51 @@ template <int DIR> struct Dft<12, DIR> : DftCT<4, 3, DIR> {};
 template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
 
 // ---- plans: radices in forward order, threads per pencil T -------------------
-template <int N> struct Plan;
-#define FGPE_PLAN(N_, L_, T_, R0, R1, R2)                          \
-    template <> struct Plan<N_> {                                  \
-        static constexpr int L = L_, T = T_, E = N_ / T_;          \
-        static constexpr int r[3] = {R0, R1, R2};                  \
+// Variant 0 favours few threads per pencil (E = 16 elements per thread for
+// the long pencils); variant 1 uses more threads and fewer registers (E = 8)
+// where a second decomposition exists.  `distinct` says whether variant 1
+// differs from variant 0.
+template <int N, int V = 0> struct Plan;
+#define FGPE_PLAN(N_, V_, L_, T_, R0, R1, R2, D_)                 \
+    template <> struct Plan<N_, V_> {                            \
+        static constexpr int L = L_, T = T_, E = N_ / T_;        \
+        static constexpr int r[3] = {R0, R1, R2};                \
+        static constexpr bool distinct = D_;                     \
     };
-FGPE_PLAN(2, 1, 1, 2, 1, 1)
-FGPE_PLAN(4, 1, 1, 4, 1, 1)
-FGPE_PLAN(8, 1, 1, 8, 1, 1)
-FGPE_PLAN(16, 1, 1, 16, 1, 1)
-FGPE_PLAN(32, 2, 4, 8, 4, 1)
-FGPE_PLAN(64, 2, 8, 8, 8, 1)
-FGPE_PLAN(96, 3, 8, 6, 4, 4)
-FGPE_PLAN(128, 2, 8, 16, 8, 1)
-FGPE_PLAN(192, 3, 16, 12, 4, 4)
-FGPE_PLAN(256, 2, 16, 16, 16, 1)
-FGPE_PLAN(384, 3, 16, 12, 8, 4)
-FGPE_PLAN(512, 3, 64, 8, 8, 8)
+FGPE_PLAN(2, 0, 1, 1, 2, 1, 1, false)
+FGPE_PLAN(4, 0, 1, 1, 4, 1, 1, false)
+FGPE_PLAN(8, 0, 1, 1, 8, 1, 1, false)
+FGPE_PLAN(16, 0, 1, 1, 16, 1, 1, false)
+FGPE_PLAN(32, 0, 2, 4, 8, 4, 1, false)
+FGPE_PLAN(64, 0, 2, 8, 8, 8, 1, false)
+FGPE_PLAN(96, 0, 3, 8, 6, 4, 4, false)
+FGPE_PLAN(128, 0, 2, 8, 16, 8, 1, false)
+FGPE_PLAN(192, 0, 3, 16, 12, 4, 4, false)
+FGPE_PLAN(256, 0, 2, 16, 16, 16, 1, false)
+FGPE_PLAN(384, 0, 3, 16, 12, 8, 4, false)
+FGPE_PLAN(512, 0, 3, 64, 8, 8, 8, false)
+FGPE_PLAN(2, 1, 1, 1, 2, 1, 1, false)
+FGPE_PLAN(4, 1, 1, 1, 4, 1, 1, false)
+FGPE_PLAN(8, 1, 1, 1, 8, 1, 1, false)
+FGPE_PLAN(16, 1, 1, 1, 16, 1, 1, false)
+FGPE_PLAN(32, 1, 2, 4, 8, 4, 1, false)
+FGPE_PLAN(64, 1, 3, 16, 4, 4, 4, true)
+FGPE_PLAN(96, 1, 3, 8, 6, 4, 4, false)
+FGPE_PLAN(128, 1, 3, 16, 8, 4, 4, true)
+FGPE_PLAN(192, 1, 3, 16, 12, 4, 4, false)
+FGPE_PLAN(256, 1, 3, 32, 8, 4, 8, true)
+FGPE_PLAN(384, 1, 3, 16, 12, 8, 4, false)
+FGPE_PLAN(512, 1, 3, 64, 8, 8, 8, false)
 #undef FGPE_PLAN
 
-template <int N, int DIR, int S> __host__ __device__ constexpr int radix_of()
+template <int N, int V, int DIR, int S> __host__ __device__ constexpr int radix_of()
 {
-    return DIR < 0 ? Plan<N>::r[S] : Plan<N>::r[Plan<N>::L - 1 - S];
+    return DIR < 0 ? Plan<N, V>::r[S] : Plan<N, V>::r[Plan<N, V>::L - 1 - S];
 }
-template <int N, int DIR, int S> __host__ __device__ constexpr int ns_of()
+template <int N, int V, int DIR, int S> __host__ __device__ constexpr int ns_of()
 {
     if constexpr (S == 0) return 1;
-    else return ns_of<N, DIR, S - 1>() * radix_of<N, DIR, S - 1>();
+    else return ns_of<N, V, DIR, S - 1>() * radix_of<N, V, DIR, S - 1>();
 }
 
 // LDS placement: one pad slot after every 16 elements; pencil stride odd.
@@ -178,15 +195,15 @@ template <int N> __host__ __device__ constexpr int lds_stride() { return (N + N 
 __device__ __forceinline__ int lds_phys(int a) { return a + (a >> 4); }
 
 // register slot q of layout A / B -> element index within the pencil
-template <int N> __device__ __forceinline__ int elemA(int t, int q)
+template <int N, int V> __device__ __forceinline__ int elemA(int t, int q)
 {
-    constexpr int r = Plan<N>::r[0];
-    return (t + Plan<N>::T * (q / r)) + (q % r) * (N / r);
+    constexpr int r = Plan<N, V>::r[0];
+    return (t + Plan<N, V>::T * (q / r)) + (q % r) * (N / r);
 }
-template <int N> __device__ __forceinline__ int elemB(int t, int q)
+template <int N, int V> __device__ __forceinline__ int elemB(int t, int q)
 {
-    constexpr int r = Plan<N>::r[Plan<N>::L - 1];
-    return (t + Plan<N>::T * (q / r)) + (q % r) * (N / r);
+    constexpr int r = Plan<N, V>::r[Plan<N, V>::L - 1];
+    return (t + Plan<N, V>::T * (q / r)) + (q % r) * (N / r);
 }
 
 // Twiddle table entry W_N^m = exp(-2 pi i m / N) (forward), conjugated for the
@@ -198,12 +215,12 @@ template <int DIR, typename T2> __device__ __forceinline__ T2 twid(const T2* __r
     return w;
 }
 
-template <typename T2, int N, int DIR, int S>
+template <typename T2, int N, int V, int DIR, int S>
 __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
-    constexpr int L = Plan<N>::L, TT = Plan<N>::T, E = N / TT;
-    constexpr int r = radix_of<N, DIR, S>();
-    constexpr int Ns = ns_of<N, DIR, S>();
+    constexpr int L = Plan<N, V>::L, TT = Plan<N, V>::T, E = N / TT;
+    constexpr int r = radix_of<N, V, DIR, S>();
+    constexpr int Ns = ns_of<N, V, DIR, S>();
     constexpr int M = E / r;
     static_assert(E % r == 0, "plan: E must be a multiple of every radix");
 #pragma unroll
@@ -227,7 +244,7 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __re
             for (int i = 0; i < r; ++i) lds[lds_phys(base + i * Ns)] = v[m * r + i];
         }
         __syncthreads();
-        constexpr int r2 = radix_of<N, DIR, S + 1>();
+        constexpr int r2 = radix_of<N, V, DIR, S + 1>();
         constexpr int M2 = E / r2;
 #pragma unroll
         for (int m = 0; m < M2; ++m) {
@@ -236,21 +253,21 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __re
             for (int i = 0; i < r2; ++i) v[m * r2 + i] = lds[lds_phys(j + i * (N / r2))];
         }
         __syncthreads();
-        fft_stages<T2, N, DIR, S + 1>(v, t, lds, tw);
+        fft_stages<T2, N, V, DIR, S + 1>(v, t, lds, tw);
     }
 }
 
 // Forward: v in layout A -> layout B.  Inverse: layout B -> layout A.
 // Every thread of the block must call these (they contain barriers).
-template <typename T2, int N>
+template <typename T2, int N, int V>
 __device__ __forceinline__ void fft_forward(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
-    fft_stages<T2, N, -1, 0>(v, t, lds, tw);
+    fft_stages<T2, N, V, -1, 0>(v, t, lds, tw);
 }
-template <typename T2, int N>
+template <typename T2, int N, int V>
 __device__ __forceinline__ void fft_inverse(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
-    fft_stages<T2, N, +1, 0>(v, t, lds, tw);
+    fft_stages<T2, N, V, +1, 0>(v, t, lds, tw);
 }
 
 }  // namespace fgpe

@@ -509,7 +509,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
         if (!c->lam[i]) FGPE_RET(dalloc(c, (void**)&c->lam[i], b64));
     }
     if (!c->spec3) FGPE_RET(dalloc(c, &c->spec3, 3 * 2 * c->tsz() * N));
-    if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)nb));
+    if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)(c->ntot / 32 + 1024)));
     double* ptmp = c->pacc + 2 * nb;
     FGPE_HIPCHK(hipMemsetAsync(c->pacc, 0, sizeof(double) * 2 * nb, c->stream));
     FGPE_HIPCHK(hipMemsetAsync(vbar, 0, sizeof(double) * N, c->stream));
@@ -603,6 +603,63 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
     grads[1] = c->hp.a_dd != 0.0 ? s1 / c->hp.a_dd : 0.0;
     c->spec_valid = false;
     return FGPE_OK;
+}
+
+// Times both plan variants of each kernel class on the context's own buffers
+// (stage-2 x passes read P0 and write only PS and K, so the state survives)
+// and keeps the faster one.
+template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
+{
+    FGPE_RET(ensure_ready(c));
+    const int nc = c->hp.ncomp;
+    const bool dens = c->nch > nc;
+    const int la = last_axis(c);
+    hipEvent_t e0, e1;
+    FGPE_HIPCHK(hipEventCreate(&e0));
+    FGPE_HIPCHK(hipEventCreate(&e1));
+    const bool timing = c->timing;
+    c->timing = false;
+    int rc = FGPE_OK;
+    auto run = [&](int cls) -> int {
+        if (cls == FGPE_KCLASS_STRIDED) {
+            FGPE_RET(Launch<T>::strided(c, c->spec, 1, -1, c->nch, nc, nullptr, nullptr));
+            return Launch<T>::strided(c, c->spec, 1, +1, c->nch, nc, nullptr, nullptr);
+        }
+        if (cls == FGPE_KCLASS_LAST) return Launch<T>::last(c, c->spec, la, c->nch, nc, nullptr, has_filter(c->hp), nullptr);
+        XArgs<T> a = base_xargs<T>(c, 0.0);
+        a.cr = 0.0;
+        a.ci = -1e-30;
+        a.stage = 2;
+        return Launch<T>::xstage(c, a, nc, dens, MODE_RK);
+    };
+    const int classes[3] = {FGPE_KCLASS_STRIDED, FGPE_KCLASS_LAST, FGPE_KCLASS_XSTAGE};
+    for (int ci = 0; ci < 3 && rc == FGPE_OK; ++ci) {
+        const int cls = classes[ci];
+        if (cls == FGPE_KCLASS_STRIDED && la != 2) continue;
+        const int axis = cls == FGPE_KCLASS_STRIDED ? 1 : (cls == FGPE_KCLASS_LAST ? la : 0);
+        double best = 1e300;
+        int bestv = 0;
+        for (int v = 0; v < 2 && rc == FGPE_OK; ++v) {
+            c->var[cls][axis] = v;
+            rc = run(cls);                                  // warm-up
+            if (rc != FGPE_OK) break;
+            (void)hipEventRecord(e0, c->stream);
+            for (int r = 0; r < reps && rc == FGPE_OK; ++r) rc = run(cls);
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / reps;
+            if (out) out[2 * ci + v] = per;
+            if (per < best) { best = per; bestv = v; }
+        }
+        c->var[cls][axis] = bestv;
+    }
+    c->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->spec_valid = false;
+    return rc;
 }
 
 int ensure_hist(fgpe_ctx* c, size_t n)
@@ -959,6 +1016,33 @@ int fgpe_adjoint_backward(fgpe_ctx* c, const void* dev_lamT, void* dev_lam0, dou
     c->hb_active = false;
     return c->mixed() ? adj_backward<float>(c, dev_lamT, dev_lam0, dev_gradV, host_grad)
                       : adj_backward<double>(c, dev_lamT, dev_lam0, dev_gradV, host_grad);
+}
+
+int fgpe_autotune(fgpe_ctx* c, int reps, double* host_ms)
+{
+    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (reps < 1) return fail(FGPE_EINVAL, "reps must be positive");
+    c->hb_active = false;
+    return c->mixed() ? autotune<float>(c, reps, host_ms) : autotune<double>(c, reps, host_ms);
+}
+
+int fgpe_set_variant(fgpe_ctx* c, int kclass, int axis, int variant)
+{
+    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (kclass < 0 || kclass >= FGPE_NKCLASS || axis < 0 || axis > 2 || variant < 0 || variant > 1)
+        return fail(FGPE_EINVAL, "kclass, axis or variant out of range");
+    c->var[kclass][axis] = variant;
+    c->spec_valid = false;
+    return FGPE_OK;
+}
+
+int fgpe_get_variant(fgpe_ctx* c, int kclass, int axis, int* variant)
+{
+    if (!c || !variant) return fail(FGPE_EINVAL, "NULL argument");
+    if (kclass < 0 || kclass >= FGPE_NKCLASS || axis < 0 || axis > 2)
+        return fail(FGPE_EINVAL, "kclass or axis out of range");
+    *variant = c->var[kclass][axis];
+    return FGPE_OK;
 }
 
 int fgpe_imag_converged(fgpe_ctx* c, int* converged)

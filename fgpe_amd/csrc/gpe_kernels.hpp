@@ -25,15 +25,18 @@ namespace fgpe {
 constexpr double kPi = 3.141592653589793;  // hpp/header.hpp:28
 
 // pencils per block of the x kernels and tile width of the strided kernels
-template <typename T, int N> __host__ __device__ constexpr int px_of()
+// Variant 0: 128-thread x blocks, 128-byte row segments in the strided
+// passes; variant 1 (with its plan of more threads per pencil): 256-thread x
+// blocks and 256-byte row segments.
+template <typename T, int N, int V = 0> __host__ __device__ constexpr int px_of()
 {
-    return Plan<N>::T >= 128 ? 1 : 128 / Plan<N>::T;
+    constexpr int th = V == 0 ? 128 : 256;
+    return Plan<N, V>::T >= th ? 1 : th / Plan<N, V>::T;
 }
-template <typename T, int N> __host__ __device__ constexpr int ps_of()
+template <typename T, int N, int V = 0> __host__ __device__ constexpr int ps_of()
 {
-    // 8 consecutive x columns (128 B of FP64 complex) per row segment; 16 for
-    // FP32; fewer for the long pencils so that the LDS tile stays <= 64 KB
-    return (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
+    constexpr int base = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
+    return V == 0 ? base : 2 * base;
 }
 
 template <typename T> __device__ __forceinline__ double2 to_d2(c2_t<T> a) { return double2{(double)a.x, (double)a.y}; }
@@ -67,12 +70,12 @@ template <int NV> __device__ __forceinline__ void block_sum_store(double (&s)[NV
 
 // ---------------------------------------------------------------------------
 // Standalone x-axis transform of rows of length N (the building-block API).
-template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
+template <typename T, int N, int V, int DIR>
+__global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
 {
     using T2 = c2_t<T>;
-    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row = (long long)blockIdx.x * P + p;
@@ -82,16 +85,16 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
     T2 v[E];
     if (DIR < 0) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = src[elemA<N>(t, q)];
-        fft_forward<T2, N>(v, t, my, tw);
+        for (int q = 0; q < E; ++q) v[q] = src[elemA<N, V>(t, q)];
+        fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) dst[elemB<N>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) dst[elemB<N, V>(t, q)] = v[q];
     } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = src[elemB<N>(t, q)];
-        fft_inverse<T2, N>(v, t, my, tw);
+        for (int q = 0; q < E; ++q) v[q] = src[elemB<N, V>(t, q)];
+        fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) dst[elemA<N>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) dst[elemA<N, V>(t, q)] = v[q];
     }
 }
 
@@ -112,12 +115,12 @@ struct StridedArgs {
     const int* stop;        // nonzero: skip (a converged imaginary-time loop)
 };
 
-template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((ps_of<T, N>() * Plan<N>::T))
+template <typename T, int N, int V, int DIR>
+__global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
 k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArgs a)
 {
     using T2 = c2_t<T>;
-    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = ps_of<T, N>();
+    constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     if (a.stop && *a.stop) return;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
@@ -130,7 +133,7 @@ k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArg
     T2 v[E];
     if (DIR < 0) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N>(t, q) * a.stride];
+        for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N, V>(t, q) * a.stride];
         if (a.scale) {
             double s = a.scale[ch < a.ncomp ? ch : 0];
             if (ch >= a.ncomp) s = s * s;
@@ -138,15 +141,15 @@ k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArg
 #pragma unroll
             for (int q = 0; q < E; ++q) v[q] = T2{v[q].x * st, v[q].y * st};
         }
-        fft_forward<T2, N>(v, t, my, tw);
+        fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) base[(long long)elemB<N>(t, q) * a.stride] = v[q];
+        for (int q = 0; q < E; ++q) base[(long long)elemB<N, V>(t, q) * a.stride] = v[q];
     } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = base[(long long)elemB<N>(t, q) * a.stride];
-        fft_inverse<T2, N>(v, t, my, tw);
+        for (int q = 0; q < E; ++q) v[q] = base[(long long)elemB<N, V>(t, q) * a.stride];
+        fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) base[(long long)elemA<N>(t, q) * a.stride] = v[q];
+        for (int q = 0; q < E; ++q) base[(long long)elemA<N, V>(t, q) * a.stride] = v[q];
     }
 }
 
@@ -167,12 +170,12 @@ struct LastArgs {
     const void* wtab;   // T table of Vt (spatial-index order) or null
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__((ps_of<T, N>() * Plan<N>::T))
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
 k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
 {
     using T2 = c2_t<T>;
-    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = ps_of<T, N>();
+    constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     if (a.s.stop && *a.s.stop) return;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
@@ -186,7 +189,7 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
     T2* my = lds + p * lds_stride<N>();
     T2 v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N>(t, q) * a.s.stride];
+    for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N, V>(t, q) * a.s.stride];
     if (a.s.scale) {
         double s = a.s.scale[ch < a.s.ncomp ? ch : 0];
         if (ch >= a.s.ncomp) s = s * s;
@@ -194,14 +197,14 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{v[q].x * st, v[q].y * st};
     }
-    fft_forward<T2, N>(v, t, my, tw);
+    fft_forward<T2, N, V>(v, t, my, tw);
 
     const double kx = a.kX[ix], ko = a.kO[outer];
     const double kxo2 = kx * kx + ko * ko;
     if (ch < a.s.ncomp) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const double ka = a.kA[elemB<N>(t, q)];
+            const double ka = a.kA[elemB<N, V>(t, q)];
             const double k2 = kxo2 + ka * ka;
             if (sizeof(T) == 8) {
                 v[q] = T2{(T)((a.lap_scale * (double)v[q].x) * k2), (T)((a.lap_scale * (double)v[q].y) * k2)};
@@ -215,7 +218,7 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
         const double num0 = kx * a.dX + ko * a.dO;
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const int ea = elemB<N>(t, q);
+            const int ea = elemB<N, V>(t, q);
             const double ka = a.kA[ea];
             double vt = 0.0;
             if (a.c_dd != 0.0) {
@@ -233,9 +236,9 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
             }
         }
     }
-    fft_inverse<T2, N>(v, t, my, tw);
+    fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
-    for (int q = 0; q < E; ++q) base[(long long)elemA<N>(t, q) * a.s.stride] = v[q];
+    for (int q = 0; q < E; ++q) base[(long long)elemA<N, V>(t, q) * a.s.stride] = v[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -283,12 +286,12 @@ template <typename T> struct XArgs {
     const int* stop;        // nonzero: the loop has converged, do nothing
 };
 
-template <typename T, int N, int NC, bool DENS, int MODE>
-__global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
+template <typename T, int N, int V, int NC, bool DENS, int MODE>
+__global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
 {
     using T2 = c2_t<T>;
-    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     constexpr int NV = 3 * NC;
     __shared__ T2 lds[P * lds_stride<N>()];
     if (a.stop && *a.stop) return;
@@ -298,31 +301,31 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
     T2* specd = a.spec + NC * a.ntot + row0;   // density channel
 
     T2 v[E];
-    double nd[E];        // density of the new state (DENS)
+    T nd[E];             // density of the new state (DENS)
 #pragma unroll
-    for (int q = 0; q < E; ++q) nd[q] = 0.0;
+    for (int q = 0; q < E; ++q) nd[q] = (T)0;
 
     if (MODE == MODE_PROLOGUE) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                v[q] = from_d2<T>(a.P0[c][row0 + elemA<N>(t, q)]);
-                if (DENS) nd[q] += (double)(v[q].x * v[q].x + v[q].y * v[q].y);
+                v[q] = from_d2<T>(a.P0[c][row0 + elemA<N, V>(t, q)]);
+                if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
             }
-            fft_forward<T2, N>(v, t, my, tw);
+            fft_forward<T2, N, V>(v, t, my, tw);
             T2* sc = a.spec + c * a.ntot + row0;
 #pragma unroll
-            for (int q = 0; q < E; ++q) sc[elemB<N>(t, q)] = v[q];
+            for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
         }
     } else {
-        double phi[E];
+        T phi[E];
         if (DENS) {
 #pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = specd[elemB<N>(t, q)];
-            fft_inverse<T2, N>(v, t, my, tw);
+            for (int q = 0; q < E; ++q) v[q] = specd[elemB<N, V>(t, q)];
+            fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
-            for (int q = 0; q < E; ++q) phi[q] = (double)v[q].x;
+            for (int q = 0; q < E; ++q) phi[q] = v[q].x;
         }
         double nsave[E];     // NC == 2: density of component 0's stage input
         double sums[NV];
@@ -334,15 +337,15 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
         for (int c = 0; c < NC; ++c) {
             T2* sc = a.spec + c * a.ntot + row0;
 #pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = sc[elemB<N>(t, q)];
-            fft_inverse<T2, N>(v, t, my, tw);   // v = KE psi_c (layout A)
+            for (int q = 0; q < E; ++q) v[q] = sc[elemB<N, V>(t, q)];
+            fft_inverse<T2, N, V>(v, t, my, tw);   // v = KE psi_c (layout A)
             const double sig = a.scale ? a.scale[c] : 1.0;
             const double sig1 = (NC == 2 && a.scale) ? a.scale[1] : 1.0;
             const double g4s = a.g4[c][c], g2s = a.g2[c][c];
             const double g4o = a.g4[c][1 - c], g2o = a.g2[c][1 - c];
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                const long long g = row0 + elemA<N>(t, q);
+                const long long g = row0 + elemA<N, V>(t, q);
                 double2 po = double2{0.0, 0.0};
                 double2 ps;
                 if (from_p0) {
@@ -374,13 +377,13 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
                 const double2 ke = to_d2<T>(v[q]);
                 double rX = (vr + g4s * n);
                 if (NC == 2) rX = rX + g4o * no;
-                if (DENS) rX += phi[q];
+                if (DENS) rX += (double)phi[q];
                 const double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
                 double2 he = double2{0.0, 0.0};
                 if (MODE == MODE_APPLY || MODE == MODE_HB) {
                     double eX = (vr + g2s * n);
                     if (NC == 2) eX = eX + g2o * no;
-                    if (DENS) eX += 0.5 * phi[q];
+                    if (DENS) eX += 0.5 * (double)phi[q];
                     he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
                 }
                 if (MODE == MODE_APPLY) {
@@ -400,7 +403,7 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
                     sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
                     sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
                     v[q] = from_d2<T>(nw);
-                    if (DENS) nd[q] += (double)(v[q].x * v[q].x + v[q].y * v[q].y);
+                    if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
                 } else {
                     if (a.stage != 1) {
                         const double2 r = a.P0[c][g];
@@ -429,13 +432,13 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
                         v[q] = from_d2<T>(nw);
                         if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                     }
-                    if (DENS) nd[q] += (double)(v[q].x * v[q].x + v[q].y * v[q].y);
+                    if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
                 }
             }
             if (MODE != MODE_APPLY) {
-                fft_forward<T2, N>(v, t, my, tw);
+                fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-                for (int q = 0; q < E; ++q) sc[elemB<N>(t, q)] = v[q];
+                for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
             }
         }
         if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm))
@@ -444,10 +447,10 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
     }
     if (DENS) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = T2{(T)nd[q], (T)0};
-        fft_forward<T2, N>(v, t, my, tw);
+        for (int q = 0; q < E; ++q) v[q] = T2{nd[q], (T)0};
+        fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) specd[elemB<N>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) specd[elemB<N, V>(t, q)] = v[q];
     }
 }
 
@@ -482,13 +485,13 @@ struct AdjArgs {
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-template <typename T, typename YT, int N, bool DENS>
-__global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
+template <typename T, typename YT, int N, int V, bool DENS>
+__global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
 {
     using T2 = c2_t<T>;
     using Y2 = c2_t<YT>;
-    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
@@ -499,7 +502,7 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
     double nn[E], rr[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-        const long long g = row0 + elemA<N>(t, q);
+        const long long g = row0 + elemA<N, V>(t, q);
         double2 ab;
         if (a.src) {
             ab = zmul(a.alpha, a.src[g]);
@@ -513,30 +516,30 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
         rr[q] = yx * ab.x + yy * ab.y;
         v[q] = T2{(T)ab.x, (T)ab.y};
     }
-    fft_forward<T2, N>(v, t, my, tw);
+    fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-    for (int q = 0; q < E; ++q) sp[row0 + elemB<N>(t, q)] = v[q];
+    for (int q = 0; q < E; ++q) sp[row0 + elemB<N, V>(t, q)] = v[q];
     if (DENS) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)nn[q], (T)0};
-        fft_forward<T2, N>(v, t, my, tw);
+        fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) sp[a.ntot + row0 + elemB<N>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) sp[a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)rr[q], (T)0};
-        fft_forward<T2, N>(v, t, my, tw);
+        fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) sp[2 * a.ntot + row0 + elemB<N>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) sp[2 * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
     }
 }
 
-template <typename T, typename YT, int N, bool DENS>
-__global__ void __launch_bounds__((px_of<T, N>() * Plan<N>::T))
+template <typename T, typename YT, int N, int V, bool DENS>
+__global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
 {
     using T2 = c2_t<T>;
     using Y2 = c2_t<YT>;
-    constexpr int TT = Plan<N>::T, E = Plan<N>::E, P = px_of<T, N>();
+    constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
@@ -548,23 +551,23 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
     double phin[E], phir[E];
     if (DENS) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[a.ntot + row0 + elemB<N>(t, q)];
-        fft_inverse<T2, N>(v, t, my, tw);
+        for (int q = 0; q < E; ++q) v[q] = sp[a.ntot + row0 + elemB<N, V>(t, q)];
+        fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phin[q] = (double)v[q].x;
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[2 * a.ntot + row0 + elemB<N>(t, q)];
-        fft_inverse<T2, N>(v, t, my, tw);
+        for (int q = 0; q < E; ++q) v[q] = sp[2 * a.ntot + row0 + elemB<N, V>(t, q)];
+        fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phir[q] = (double)v[q].x;
     }
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = sp[row0 + elemB<N>(t, q)];
-    fft_inverse<T2, N>(v, t, my, tw);   // T abar
+    for (int q = 0; q < E; ++q) v[q] = sp[row0 + elemB<N, V>(t, q)];
+    fft_inverse<T2, N, V>(v, t, my, tw);   // T abar
     double s[2] = {0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-        const long long g = row0 + elemA<N>(t, q);
+        const long long g = row0 + elemA<N, V>(t, q);
         const Y2 yr = y[g];
         const double2 yv = double2{(double)yr.x, (double)yr.y};
         const double2 ab = a.abar[g];

@@ -1,5 +1,7 @@
 // launch.hpp -- template launchers of the pass kernels, dispatched over the
-// compiled pencil lengths.  Included once per precision (launch_f64.hip,
+// compiled pencil lengths and the two plan variants (the variant of each
+// kernel class and axis is a per-context choice, set by fgpe_autotune or
+// fgpe_set_variant).  Included once per precision (launch_f64.hip,
 // launch_f32.hip) so the two sets of kernels compile in parallel.
 #pragma once
 #include "ctx.hpp"
@@ -32,43 +34,45 @@ inline StridedArgs strided_args(fgpe_ctx* c, int axis, const double* scale, int 
     return a;
 }
 
-template <typename T, int N> int fft_x_n(fgpe_ctx* c, const void* in, void* out, int dir)
+template <typename T, int N, int V> int fft_x_n(fgpe_ctx* c, const void* in, void* out, int dir)
 {
-    constexpr int P = px_of<T, N>();
+    constexpr int P = px_of<T, N, V>();
     const long long rows = c->ntot / N;
     if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
+    dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
     using T2 = c2_t<T>;
     TScope ts(c, FGPE_KCLASS_FFTX);
     if (dir < 0)
-        hipLaunchKernelGGL((k_fft_x<T, N, -1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out, tw_of<T>(c, 0));
+        hipLaunchKernelGGL((k_fft_x<T, N, V, -1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out,
+                           tw_of<T>(c, 0));
     else
-        hipLaunchKernelGGL((k_fft_x<T, N, +1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out, tw_of<T>(c, 0));
+        hipLaunchKernelGGL((k_fft_x<T, N, V, +1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out,
+                           tw_of<T>(c, 0));
     FGPE_LAUNCHCHK("k_fft_x");
     return FGPE_OK;
 }
 
-template <typename T, int N> int strided_n(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
-                                           const double* scale, const int* stop)
+template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
+                                                  const double* scale, const int* stop)
 {
-    constexpr int P = ps_of<T, N>();
+    constexpr int P = ps_of<T, N, V>();
     if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
     StridedArgs a = strided_args(c, axis, scale, ncomp, stop);
-    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N>::T);
+    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N, V>::T);
     using T2 = c2_t<T>;
     TScope ts(c, FGPE_KCLASS_STRIDED);
     if (dir < 0)
-        hipLaunchKernelGGL((k_strided<T, N, -1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
+        hipLaunchKernelGGL((k_strided<T, N, V, -1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
     else
-        hipLaunchKernelGGL((k_strided<T, N, +1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
+        hipLaunchKernelGGL((k_strided<T, N, V, +1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
     FGPE_LAUNCHCHK("k_strided");
     return FGPE_OK;
 }
 
-template <typename T, int N> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
-                                        const double* scale, bool filter, const int* stop)
+template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
+                                               const double* scale, bool filter, const int* stop)
 {
-    constexpr int P = ps_of<T, N>();
+    constexpr int P = ps_of<T, N, V>();
     if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
     LastArgs a;
     a.s = strided_args(c, axis, scale, ncomp, stop);
@@ -84,129 +88,124 @@ template <typename T, int N> int last_n(fgpe_ctx* c, void* data, int axis, int n
     a.inv_ntot = 1.0 / (double)c->ntot;
     a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
     a.wtab = filter ? c->wtab : nullptr;
-    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N>::T);
+    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_LAST);
-    hipLaunchKernelGGL((k_last<T, N>), grid, block, 0, c->stream, (c2_t<T>*)data, tw_of<T>(c, axis), a);
+    hipLaunchKernelGGL((k_last<T, N, V>), grid, block, 0, c->stream, (c2_t<T>*)data, tw_of<T>(c, axis), a);
     FGPE_LAUNCHCHK("k_last");
     return FGPE_OK;
 }
 
-template <typename T, int N, int NC, bool DENS, int MODE> int xstage_t(fgpe_ctx* c, const XArgs<T>& a)
+template <typename T, int N, int V, int NC, bool DENS, int MODE> int xstage_t(fgpe_ctx* c, const XArgs<T>& a)
 {
-    constexpr int P = px_of<T, N>();
+    constexpr int P = px_of<T, N, V>();
     const long long rows = c->ntot / N;
     if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
+    dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_XSTAGE);
-    hipLaunchKernelGGL((k_xstage<T, N, NC, DENS, MODE>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
     FGPE_LAUNCHCHK("k_xstage");
     return FGPE_OK;
 }
 
-template <typename T, int N, int NC, bool DENS> int xstage_m(fgpe_ctx* c, const XArgs<T>& a, int mode)
+template <typename T, int N, int V, int NC, bool DENS> int xstage_m(fgpe_ctx* c, const XArgs<T>& a, int mode)
 {
     switch (mode) {
-    case MODE_PROLOGUE: return xstage_t<T, N, NC, DENS, MODE_PROLOGUE>(c, a);
-    case MODE_RK: return xstage_t<T, N, NC, DENS, MODE_RK>(c, a);
-    case MODE_APPLY: return xstage_t<T, N, NC, DENS, MODE_APPLY>(c, a);
-    case MODE_HB: return xstage_t<T, N, NC, DENS, MODE_HB>(c, a);
+    case MODE_PROLOGUE: return xstage_t<T, N, V, NC, DENS, MODE_PROLOGUE>(c, a);
+    case MODE_RK: return xstage_t<T, N, V, NC, DENS, MODE_RK>(c, a);
+    case MODE_APPLY: return xstage_t<T, N, V, NC, DENS, MODE_APPLY>(c, a);
+    case MODE_HB: return xstage_t<T, N, V, NC, DENS, MODE_HB>(c, a);
     }
     return fail(FGPE_EINVAL, "bad x-stage mode");
 }
 
-template <typename T, int N> int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
+template <typename T, int N, int V> int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
 {
-    if (nc == 1) return dens ? xstage_m<T, N, 1, true>(c, a, mode) : xstage_m<T, N, 1, false>(c, a, mode);
-    return dens ? xstage_m<T, N, 2, true>(c, a, mode) : xstage_m<T, N, 2, false>(c, a, mode);
+    if (nc == 1) return dens ? xstage_m<T, N, V, 1, true>(c, a, mode) : xstage_m<T, N, V, 1, false>(c, a, mode);
+    return dens ? xstage_m<T, N, V, 2, true>(c, a, mode) : xstage_m<T, N, V, 2, false>(c, a, mode);
 }
 
-template <typename T, typename YT, int N, bool DENS> int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
+template <typename T, typename YT, int N, int V, bool DENS> int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
 {
-    constexpr int P = px_of<T, N>();
+    constexpr int P = px_of<T, N, V>();
     const long long rows = c->ntot / N;
     if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N>::T);
+    dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_XSTAGE);
-    if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
-    else hipLaunchKernelGGL((k_adj_pro<T, YT, N, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, V, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    else hipLaunchKernelGGL((k_adj_pro<T, YT, N, V, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
     FGPE_LAUNCHCHK("k_adj");
     return FGPE_OK;
 }
 
-template <typename T, int N> int adj_n(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
+template <typename T, int N, int V> int adj_n(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
 {
-    if (y64) return dens ? adj_t<T, double, N, true>(c, a, epi) : adj_t<T, double, N, false>(c, a, epi);
-    return dens ? adj_t<T, T, N, true>(c, a, epi) : adj_t<T, T, N, false>(c, a, epi);
+    if (y64) return dens ? adj_t<T, double, N, V, true>(c, a, epi) : adj_t<T, double, N, V, false>(c, a, epi);
+    return dens ? adj_t<T, T, N, V, true>(c, a, epi) : adj_t<T, T, N, V, false>(c, a, epi);
 }
 
 }  // namespace detail
 
+#define FGPE_DISPATCH(len, var, CALL)                                        \
+    switch (len) {                                                           \
+    case 32: return (var) ? CALL(32, 1) : CALL(32, 0);                       \
+    case 64: return (var) ? CALL(64, 1) : CALL(64, 0);                       \
+    case 96: return (var) ? CALL(96, 1) : CALL(96, 0);                       \
+    case 128: return (var) ? CALL(128, 1) : CALL(128, 0);                    \
+    case 192: return (var) ? CALL(192, 1) : CALL(192, 0);                    \
+    case 256: return (var) ? CALL(256, 1) : CALL(256, 0);                    \
+    case 512: return (var) ? CALL(512, 1) : CALL(512, 0);                    \
+    }
+
 template <typename T> int Launch<T>::adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
 {
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return detail::adj_n<T, N_>(c, a, y64, dens, epi);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
+#define CALL(N_, V_) detail::adj_n<T, N_, V_>(c, a, y64, dens, epi)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
+#undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
 
 template <typename T> long long Launch<T>::xblocks(fgpe_ctx* c)
 {
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return c->ntot / N_ / px_of<T, N_>();
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
+#define CALL(N_, V_) (c->ntot / N_ / px_of<T, N_, V_>())
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
+#undef CALL
     return 0;
 }
 
 template <typename T> int Launch<T>::fft_x(fgpe_ctx* c, const void* in, void* out, int dir)
 {
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return detail::fft_x_n<T, N_>(c, in, out, dir);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
+#define CALL(N_, V_) detail::fft_x_n<T, N_, V_>(c, in, out, dir)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_FFTX][0], CALL)
+#undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
 
 template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
                                              const double* scale, const int* stop)
 {
-    switch (c->n[axis]) {
-#define CASE(N_) \
-    case N_: return detail::strided_n<T, N_>(c, data, axis, dir, nch, ncomp, scale, stop);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
+#define CALL(N_, V_) detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop)
+    FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_STRIDED][axis], CALL)
+#undef CALL
     return fail(FGPE_EUNSUP, "unsupported strided axis length");
 }
 
 template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
                                           const double* scale, bool filter, const int* stop)
 {
-    switch (c->n[axis]) {
-#define CASE(N_) \
-    case N_: return detail::last_n<T, N_>(c, data, axis, nch, ncomp, scale, filter, stop);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
+#define CALL(N_, V_) detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop)
+    FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis], CALL)
+#undef CALL
     return fail(FGPE_EUNSUP, "unsupported last axis length");
 }
 
 template <typename T> int Launch<T>::xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
 {
-    switch (c->n[0]) {
-#define CASE(N_) \
-    case N_: return detail::xstage_n<T, N_>(c, a, nc, dens, mode);
-        FGPE_FOR_EACH_N(CASE)
-#undef CASE
-    }
+#define CALL(N_, V_) detail::xstage_n<T, N_, V_>(c, a, nc, dens, mode)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
+#undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
+
+#undef FGPE_DISPATCH
 
 }  // namespace fgpe

@@ -170,6 +170,19 @@ int fgpe_timing_enable(fgpe_ctx* ctx, int on);
    each class k < FGPE_NKCLASS, and clears the record. */
 int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
 
+/* ---- plan variants and the autotuner (no counterpart in the reference) ----
+   Every kernel class has two compiled plans per pencil length: variant 0
+   (16 elements per thread on long pencils, 128-thread x blocks, 128-byte row
+   segments) and variant 1 (8 elements per thread where a second radix split
+   exists, 256-thread x blocks, 256-byte row segments).  fgpe_autotune times
+   both for the strided (y), last-axis and x-stage classes on the context's
+   own buffers (reps launches each; the state is preserved) and keeps the
+   faster; host_ms (6 doubles or NULL) gets the times per class and variant.
+   Needs the Hamiltonian, trap and state set. */
+int fgpe_autotune(fgpe_ctx* ctx, int reps, double* host_ms);
+int fgpe_set_variant(fgpe_ctx* ctx, int kclass, int axis, int variant);
+int fgpe_get_variant(fgpe_ctx* ctx, int kclass, int axis, int* variant);
+
 #ifdef __cplusplus
 }
 #endif

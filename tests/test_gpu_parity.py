@@ -313,3 +313,59 @@ def test_px_and_cos_diagnostics():
     ref_cos = c.dv * float(np.sum(np.cos(2 * np.pi * x / 12.0)[None, None, :] * np.abs(psi) ** 2))
     assert abs(px - ref_px) < 1e-12 * np.sum(np.abs(F) ** 2) * c.dv * np.abs(c.kx).max()
     assert abs(cs - ref_cos) < 1e-12 * c.dv * np.sum(np.abs(psi) ** 2)
+
+
+@pytest.mark.parametrize("shape", [(64, 128, 256), (128, 64, 64), (256, 32, 32)])
+def test_variant1_plans_parity(torch_cuda, shape):
+    """The second plan variant of every kernel class against numpy / the oracle."""
+    torch = torch_cuda
+    from fgpe_amd import configs
+    c = Case(*shape, L=(10.0, 12.0, 20.0))
+    x = c.rand_psi(51)
+    with c.ctx() as ctx:
+        for k in range(5):
+            for axis in range(3):
+                ctx.set_variant(k, axis, 1)
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        xi = dev(torch, x)
+        out = torch.empty_like(xi)
+        ctx.fft3d_f64(xi.data_ptr(), out.data_ptr(), -1)
+        torch.cuda.synchronize()
+        err = rel_max(out.cpu().numpy(), np.fft.fftn(x))
+        assert err < 1e-14, err
+    h = dict(a=(0.9,), a_dd=0.8, dip=(0.0, 0.0, 1.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(configs.harmonic_trap(c.spec, (1, 1, 0.5)))) +
+                            0.01 * c.rand_psi(52), c.dv)
+    with c.ctx() as ctx:
+        for k in range(5):
+            for axis in range(3):
+                ctx.set_variant(k, axis, 1)
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ctx.rk4_steps(5e-4, 3)
+        gpu = ctx.download_psi()
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 3)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
+
+
+def test_autotune_keeps_state():
+    from fgpe_amd import configs
+    c = Case(64, 64, 128, L=(16.0, 16.0, 32.0))
+    h = dict(a=(1.0,), a_dd=1.3, dip=(0.0, 0.0, 1.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(V)) + 0.01 * c.rand_psi(53), c.dv)
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ms = ctx.autotune(3)
+        assert all(t > 0 for v in ms.values() for t in v if t != 0)
+        assert np.array_equal(ctx.download_psi(), psi)
+        ctx.rk4_steps(5e-4, 4)
+        gpu = ctx.download_psi()
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 4)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
