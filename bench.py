@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 launches), the known-size "
+                         "reference that scales FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is "
@@ -92,6 +95,15 @@ def main():
         return 2
 
     import torch
+    if args.copy_calib > 0:
+        n = args.copy_calib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_each_way": n * 8, "launches": 3}), flush=True)
+        return 0
     from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, configs, roofline
 
     nx, ny, nz = (int(v) for v in args.grid.split(","))
@@ -109,7 +121,6 @@ def main():
               file=sys.stderr, flush=True)
 
     N = nx * ny * nz
-    nch = 2
     f64 = res["f64"]
     mixed = res["mixed"]
     steps_s = 1000.0 / f64["ms"]
@@ -118,11 +129,11 @@ def main():
     dom = max(kt, key=lambda k: kt[k][0])
     avg_ms = kt[dom][0] / kt[dom][1]
     if dom == "xstage":
-        per_pt = sum(roofline.xstage_bytes(s, False, nch) for s in (1, 2, 3, 4)) / 4.0
+        per_pt = sum(roofline.xstage_bytes(s, False, 1, True, nx) for s in (1, 2, 3, 4)) / 4.0
     else:
-        per_pt = roofline.pass_bytes(False, nch)
+        per_pt = roofline.pass_bytes(False, 1, True, nx)
     achieved = per_pt * N / (avg_ms * 1e-3) / 1e9
-    step_gbs = roofline.step_bytes_per_point(False, nch) * N * steps_s / 1e9
+    step_gbs = roofline.step_bytes_per_point(False, 1, True, nx) * N * steps_s / 1e9
     mixed_steps = 1000.0 / mixed["ms"]
     l2 = float(np.sqrt(np.sum(np.abs(mixed["psi"] - f64["psi"]) ** 2) / np.sum(np.abs(f64["psi"]) ** 2)))
     de = float(abs(mixed["obs"][2] - f64["obs"][2]) / abs(f64["obs"][2]))
@@ -146,8 +157,8 @@ def main():
         "mixed": {"steps_per_s": round(mixed_steps, 3), "ms_per_step": round(mixed["ms"], 4),
                   "speedup_vs_f64": round(mixed_steps / steps_s, 3), "l2_psi_err_vs_f64": l2,
                   "rel_energy_err_vs_f64": de, "steps_compared": args.warmup + 2 * args.steps},
-        "step_traffic": {"bytes_per_point_f64": roofline.step_bytes_per_point(False, nch),
-                         "bytes_per_point_mixed": roofline.step_bytes_per_point(True, nch),
+        "step_traffic": {"bytes_per_point_f64": round(roofline.step_bytes_per_point(False, 1, True, nx), 2),
+                         "bytes_per_point_mixed": round(roofline.step_bytes_per_point(True, 1, True, nx), 2),
                          "f64_GBs": round(step_gbs, 1),
                          "f64_frac_of_peak": round(step_gbs / roofline.HBM_PEAK_GBS, 4)},
         "autotune": {"f64": f64["tune"], "mixed": mixed["tune"]},

@@ -122,15 +122,23 @@ struct TScope {
     ~TScope() { t_end(c, k, cls); }
 };
 
+// Row pitch (elements) of a half-spectrum real channel: kx = 0..nx/2 stored,
+// padded to a multiple of 16 elements (128-byte row starts for every
+// supported nx and both precisions).
+inline long long dens_pitch(const fgpe_ctx* c) { return c->n[0] / 2 + 16; }
+
 // ---- per-precision launchers (launch_f64.hip / launch_f32.hip) --------------
-// axis: 1 = y, 2 = z.  nch channels of ntot points at ctx->spec (or `data`).
+// axis: 1 = y, 2 = z.  nch channels at ctx->spec (or `data`): ncomp complex
+// channels of ntot points, then real-field channels, in the half-spectrum
+// layout when `half` is set (rows of dens_pitch elements), else like the
+// complex ones.
 template <typename T> struct Launch {
     static long long xblocks(fgpe_ctx* c);
     static int fft_x(fgpe_ctx* c, const void* in, void* out, int dir);
     static int strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp, const double* scale,
-                       const int* stop);
+                       const int* stop, bool half);
     static int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter,
-                    const int* stop);
+                    const int* stop, bool half);
     static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode);
     // adjoint VJP prologue / epilogue; y64: the state is FP64 (else T)
     static int adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epilogue);

@@ -179,7 +179,6 @@ FGPE_PLAN(256, 1, 3, 32, 8, 4, 8, true)
 FGPE_PLAN(384, 1, 3, 16, 12, 8, 4, false)
 FGPE_PLAN(512, 1, 3, 64, 8, 8, 8, false)
 #undef FGPE_PLAN
-
 template <int N, int V, int DIR, int S> __host__ __device__ constexpr int radix_of()
 {
     return DIR < 0 ? Plan<N, V>::r[S] : Plan<N, V>::r[Plan<N, V>::L - 1 - S];
@@ -215,6 +214,34 @@ template <int DIR, typename T2> __device__ __forceinline__ T2 twid(const T2* __r
     return w;
 }
 
+// Stage-to-stage exchange through LDS: write / barrier / read / barrier.
+template <typename T2, int N, int V, int DIR, int S>
+__device__ __forceinline__ void exchange(T2* v, int t, T2* lds)
+{
+    constexpr int TT = Plan<N, V>::T, E = N / TT;
+    constexpr int r = radix_of<N, V, DIR, S>();
+    constexpr int Ns = ns_of<N, V, DIR, S>();
+    constexpr int M = E / r;
+    constexpr int r2 = radix_of<N, V, DIR, S + 1>();
+    constexpr int M2 = E / r2;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int j = t + TT * m;
+        const int k = j % Ns;
+        const int base = (j / Ns) * Ns * r + k;
+#pragma unroll
+        for (int i = 0; i < r; ++i) lds[lds_phys(base + i * Ns)] = v[m * r + i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M2; ++m) {
+        const int j = t + TT * m;
+#pragma unroll
+        for (int i = 0; i < r2; ++i) v[m * r2 + i] = lds[lds_phys(j + i * (N / r2))];
+    }
+    __syncthreads();
+}
+
 template <typename T2, int N, int V, int DIR, int S>
 __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
@@ -235,30 +262,14 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __re
         Dft<r, DIR>::run(v + m * r);
     }
     if constexpr (S + 1 < L) {
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int j = t + TT * m;
-            const int k = j % Ns;
-            const int base = (j / Ns) * Ns * r + k;
-#pragma unroll
-            for (int i = 0; i < r; ++i) lds[lds_phys(base + i * Ns)] = v[m * r + i];
-        }
-        __syncthreads();
-        constexpr int r2 = radix_of<N, V, DIR, S + 1>();
-        constexpr int M2 = E / r2;
-#pragma unroll
-        for (int m = 0; m < M2; ++m) {
-            const int j = t + TT * m;
-#pragma unroll
-            for (int i = 0; i < r2; ++i) v[m * r2 + i] = lds[lds_phys(j + i * (N / r2))];
-        }
-        __syncthreads();
+        exchange<T2, N, V, DIR, S>(v, t, lds);
         fft_stages<T2, N, V, DIR, S + 1>(v, t, lds, tw);
     }
 }
 
 // Forward: v in layout A -> layout B.  Inverse: layout B -> layout A.
-// Every thread of the block must call these (they contain barriers).
+// lds: this pencil's lds_stride<N>() elements.  Every thread of the block
+// must call these (they contain barriers).
 template <typename T2, int N, int V>
 __device__ __forceinline__ void fft_forward(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {

@@ -157,31 +157,33 @@ double a_at(const fgpe_hparams& h, double t)
 template <typename T> int fft3d(fgpe_ctx* c, const void* in, void* out, int dir)
 {
     FGPE_RET(Launch<T>::fft_x(c, in, out, dir));
-    if (c->n[1] > 1) FGPE_RET(Launch<T>::strided(c, out, 1, dir, 1, 1, nullptr, nullptr));
-    if (c->n[2] > 1) FGPE_RET(Launch<T>::strided(c, out, 2, dir, 1, 1, nullptr, nullptr));
+    if (c->n[1] > 1) FGPE_RET(Launch<T>::strided(c, out, 1, dir, 1, 1, nullptr, nullptr, false));
+    if (c->n[2] > 1) FGPE_RET(Launch<T>::strided(c, out, 2, dir, 1, 1, nullptr, nullptr, false));
     return FGPE_OK;
 }
 
 // y-forward / last axis / y-inverse on the spectral channels.  `scale` is
 // applied by the first forward pass after a prologue or a final stage.
-template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, const double* scale,
-                                             const int* stop)
+// half: the real-field channels are in the half-spectrum layout.
+template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, int nc, const double* scale,
+                                             const int* stop, bool half)
 {
     const bool filter = has_filter(c->hp);
-    const int nc = c->hp.ncomp;
     if (last_axis(c) == 2) {
-        FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop));
-        FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop));
-        FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop));
+        FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop, half));
+        FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop, half));
+        FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop, half));
     } else {
-        FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop));
+        FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop, half));
     }
     return FGPE_OK;
 }
 
+// The solver's channels: the components, then the total density as a half
+// spectrum when a k-space kernel is on.
 template <typename T> int spectral_passes(fgpe_ctx* c, const double* scale, const int* stop)
 {
-    return spectral_passes_on<T>(c, c->spec, c->nch, scale, stop);
+    return spectral_passes_on<T>(c, c->spec, c->nch, c->hp.ncomp, scale, stop, true);
 }
 
 template <typename T> XArgs<T> base_xargs(fgpe_ctx* c, double t)
@@ -189,6 +191,8 @@ template <typename T> XArgs<T> base_xargs(fgpe_ctx* c, double t)
     XArgs<T> a{};
     a.spec = (c2_t<T>*)c->spec;
     a.ntot = c->ntot;
+    a.specd = (c2_t<T>*)c->spec + (long long)c->hp.ncomp * c->ntot;
+    a.dpitch = (int)dens_pitch(c);
     for (int i = 0; i < 2; ++i) {
         a.P0[i] = c->P0[i];
         a.PS[i] = (c2_t<T>*)c->PS[i];
@@ -478,7 +482,7 @@ template <typename T> int adj_forward(fgpe_ctx* c, double dt, double t0, int nst
 template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, double* ptmp)
 {
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, false));
-    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? 3 : 1, nullptr, nullptr));
+    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? 3 : 1, 1, nullptr, nullptr, false));
     a.partials = ptmp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, true));
     const long long n = 2 * Launch<T>::xblocks(c);
@@ -622,10 +626,11 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
     int rc = FGPE_OK;
     auto run = [&](int cls) -> int {
         if (cls == FGPE_KCLASS_STRIDED) {
-            FGPE_RET(Launch<T>::strided(c, c->spec, 1, -1, c->nch, nc, nullptr, nullptr));
-            return Launch<T>::strided(c, c->spec, 1, +1, c->nch, nc, nullptr, nullptr);
+            FGPE_RET(Launch<T>::strided(c, c->spec, 1, -1, c->nch, nc, nullptr, nullptr, true));
+            return Launch<T>::strided(c, c->spec, 1, +1, c->nch, nc, nullptr, nullptr, true);
         }
-        if (cls == FGPE_KCLASS_LAST) return Launch<T>::last(c, c->spec, la, c->nch, nc, nullptr, has_filter(c->hp), nullptr);
+        if (cls == FGPE_KCLASS_LAST)
+            return Launch<T>::last(c, c->spec, la, c->nch, nc, nullptr, has_filter(c->hp), nullptr, true);
         XArgs<T> a = base_xargs<T>(c, 0.0);
         a.cr = 0.0;
         a.ci = -1e-30;
@@ -915,9 +920,9 @@ int fgpe_laplacian_f64(fgpe_ctx* c, const void* psi, void* out)
     if (c->n[1] == 1) return fail(FGPE_EUNSUP, "1D grids are not supported");
     FGPE_RET(Launch<double>::fft_x(c, psi, out, -1));
     const int la = last_axis(c);
-    if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, -1, 1, 1, nullptr, nullptr));
-    FGPE_RET(Launch<double>::last(c, out, la, 1, 1, nullptr, false, nullptr));
-    if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, +1, 1, 1, nullptr, nullptr));
+    if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, -1, 1, 1, nullptr, nullptr, false));
+    FGPE_RET(Launch<double>::last(c, out, la, 1, 1, nullptr, false, nullptr, false));
+    if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, +1, 1, 1, nullptr, nullptr, false));
     FGPE_RET(Launch<double>::fft_x(c, out, out, +1));
     return FGPE_OK;
 }

@@ -100,20 +100,91 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
 
 // ---------------------------------------------------------------------------
 // Strided axis (y or z) transform, in place.  Block = tile of P consecutive x
-// columns at one `outer` index; channel = blockIdx.y.  Thread (p, t) with p
-// fastest, so a row segment of P columns is one coalesced access.
+// columns at one `outer` index of one channel.  Thread (p, t) with p fastest,
+// so a row segment of P columns is one coalesced access.
+//
+// Channels [0, ncomp) hold complex fields (psi).  The channels after them hold
+// real fields (the density), either in the same layout or, with hcols > 0, as
+// a half spectrum: after the x transform of a real row only kx = 0..nx/2 are
+// stored (the rest are conjugates), hcols = nx/2 + 1 columns in rows of
+// pitch hpitch.  Those channels take ceil(hcols/P) tiles per outer index; the
+// lanes of a last partial tile compute on zeros and store nothing.
+//
 // scale (forward only, may be null): channel c < ncomp is multiplied by
-// scale[0], the density channel by scale[0]^2 (the pending renormalisation
+// scale[c], a real-field channel by scale[0]^2 (the pending renormalisation
 // of the imaginary-time schemes).
 struct StridedArgs {
-    int nx;                 // columns (x extent)
-    long long stride;       // element stride along the axis
-    long long outer_stride; // stride of the outer index
-    long long chan_stride;  // elements per channel
+    int nx;                 // columns of the complex channels (x extent)
+    long long stride;       // element stride along the axis (complex layout)
+    long long outer_stride; // stride of the outer index (complex layout)
+    long long chan_stride;  // elements per complex-layout channel
+    int nouter;             // extent of the outer index
+    int ncomp;              // complex channels
+    int nchan;              // channels in all
+    int hcols;              // > 0: real channels in half-spectrum layout
+    long long hstride;      // half layout: element stride along the axis
+    long long houter;       //              stride of the outer index
+    long long hbase;        //              offset of the first real channel
+    long long hchan;        //              elements per real channel
     const double* scale;    // per-component pending scale (forward only) or null
-    int ncomp;
     const int* stop;        // nonzero: skip (a converged imaginary-time loop)
 };
+
+// Blocks of one strided launch: complex-layout channels first, then the
+// half-layout ones.
+template <int P> __host__ __device__ inline long long strided_blocks(const StridedArgs& a)
+{
+    const long long ntf = a.nx / P;
+    if (a.hcols == 0) return (long long)a.nchan * ntf * a.nouter;
+    const long long nth = (a.hcols + P - 1) / P;
+    return ((long long)a.ncomp * ntf + (long long)(a.nchan - a.ncomp) * nth) * a.nouter;
+}
+
+struct TileAt {
+    long long off;      // element offset of this thread's column at axis index 0
+    long long st;       // element stride along the axis
+    long long sp_off;   // complex-layout offset of the same (x, outer) position, channel-relative
+    long long outer;
+    int ch, ix;
+    bool valid;         // this lane's column is stored
+};
+
+template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArgs& a, int p)
+{
+    TileAt r;
+    const int ntf = a.nx / P;
+    const long long nbf = (long long)(a.hcols ? a.ncomp : a.nchan) * ntf * a.nouter;
+    long long b = blockIdx.x;
+    int xt;
+    if (b < nbf) {
+        const long long per = (long long)ntf * a.nouter;
+        r.ch = (int)(b / per);
+        b -= (long long)r.ch * per;
+        xt = (int)(b % ntf);
+        r.outer = b / ntf;
+        r.ix = xt * P + p;
+        r.sp_off = r.outer * a.outer_stride + r.ix;
+        r.off = r.ch * a.chan_stride + r.sp_off;
+        r.st = a.stride;
+        r.valid = true;
+    } else {
+        b -= nbf;
+        const int nth = (a.hcols + P - 1) / P;
+        const long long per = (long long)nth * a.nouter;
+        const int hc = (int)(b / per);
+        b -= (long long)hc * per;
+        xt = (int)(b % nth);
+        r.outer = b / nth;
+        r.ch = a.ncomp + hc;
+        r.ix = xt * P + p;
+        r.valid = r.ix < a.hcols;
+        if (!r.valid) r.ix = 0;   // keeps table lookups in range; nothing is stored
+        r.sp_off = r.outer * a.outer_stride + r.ix;
+        r.off = a.hbase + hc * a.hchan + r.outer * a.houter + r.ix;
+        r.st = a.hstride;
+    }
+    return r;
+}
 
 template <typename T, int N, int V, int DIR>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
@@ -124,40 +195,44 @@ k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArg
     __shared__ T2 lds[P * lds_stride<N>()];
     if (a.stop && *a.stop) return;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
-    const int ntile = a.nx / P;
-    const int xt = blockIdx.x % ntile;
-    const long long outer = blockIdx.x / ntile;
-    const int ch = blockIdx.y;
-    T2* base = data + ch * a.chan_stride + outer * a.outer_stride + (long long)xt * P + p;
+    const TileAt at = strided_tile<P>(a, p);
+    T2* base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
     T2 v[E];
     if (DIR < 0) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N, V>(t, q) * a.stride];
+        for (int q = 0; q < E; ++q) v[q] = at.valid ? base[(long long)elemA<N, V>(t, q) * at.st] : T2{0, 0};
         if (a.scale) {
-            double s = a.scale[ch < a.ncomp ? ch : 0];
-            if (ch >= a.ncomp) s = s * s;
+            double s = a.scale[at.ch < a.ncomp ? at.ch : 0];
+            if (at.ch >= a.ncomp) s = s * s;
             const T st = (T)s;
 #pragma unroll
             for (int q = 0; q < E; ++q) v[q] = T2{v[q].x * st, v[q].y * st};
         }
         fft_forward<T2, N, V>(v, t, my, tw);
+        if (at.valid)
 #pragma unroll
-        for (int q = 0; q < E; ++q) base[(long long)elemB<N, V>(t, q) * a.stride] = v[q];
+            for (int q = 0; q < E; ++q) base[(long long)elemB<N, V>(t, q) * at.st] = v[q];
     } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = base[(long long)elemB<N, V>(t, q) * a.stride];
+        for (int q = 0; q < E; ++q) v[q] = at.valid ? base[(long long)elemB<N, V>(t, q) * at.st] : T2{0, 0};
         fft_inverse<T2, N, V>(v, t, my, tw);
+        if (at.valid)
 #pragma unroll
-        for (int q = 0; q < E; ++q) base[(long long)elemA<N, V>(t, q) * a.stride] = v[q];
+            for (int q = 0; q < E; ++q) base[(long long)elemA<N, V>(t, q) * at.st] = v[q];
     }
 }
 
 // ---------------------------------------------------------------------------
 // Last axis: forward FFT, k-space multiply, inverse FFT, in place.
 // Channel c < ncomp (psi):  v *= (19.739208802/Ntot) * |k|^2   (src/parallel.cu:194-205)
-// Channel c = ncomp (density): v *= Vt(k) / Ntot, Vt = dipolar (inline,
-//   src/parallel.cu:326-352) + table (soft-core or user kernel, may be null).
+// Real-field channel (density): v *= Vt(k) / Ntot, Vt = dipolar (inline,
+//   src/parallel.cu:326-352) + table (soft-core, may be null).
+// The density result is used through its real part, Re F^-1[Vt F(n)], which
+// for a real n equals F^-1[Vs F(n)] with Vs(k) = (Vt(k) + Vt(k'))/2, k' the
+// index-negated wave vector.  Vs is what the half-spectrum layout needs (its
+// product with F(n) stays Hermitian).  Off the Nyquist planes k' = -k exactly
+// and Vt(k') = Vt(k) bit for bit, so only Nyquist-plane entries change.
 struct LastArgs {
     StridedArgs s;
     const double* kA;   // k along this axis
@@ -167,8 +242,16 @@ struct LastArgs {
     double lap_scale;   // 19.739208802 / Ntot
     double inv_ntot;    // 1 / Ntot
     double c_dd;        // 12 pi a_dd (0: no dipolar term)
-    const void* wtab;   // T table of Vt (spatial-index order) or null
+    const void* wtab;   // T table of Vt (complex-layout spatial index) or null
 };
+
+// 12 pi a_dd ((k.d)^2/|k|^2 - 1/3), 0 for |k| < 1e-6 (src/parallel.cu:336-349)
+__device__ __forceinline__ double vdd_of(double c_dd, double num, double k2)
+{
+    const double den = sqrt(k2);
+    if (den < 1E-6) return 0.0;
+    return c_dd * (((num / den) * (num / den)) - 1.0 / 3.0);
+}
 
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
@@ -179,29 +262,24 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
     __shared__ T2 lds[P * lds_stride<N>()];
     if (a.s.stop && *a.s.stop) return;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
-    const int ntile = a.s.nx / P;
-    const int xt = blockIdx.x % ntile;
-    const long long outer = blockIdx.x / ntile;
-    const int ch = blockIdx.y;
-    const int ix = xt * P + p;
-    const long long sp_off = outer * a.s.outer_stride + ix;   // spatial offset
-    T2* base = data + ch * a.s.chan_stride + sp_off;
+    const TileAt at = strided_tile<P>(a.s, p);
+    T2* base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
     T2 v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = base[(long long)elemA<N, V>(t, q) * a.s.stride];
+    for (int q = 0; q < E; ++q) v[q] = at.valid ? base[(long long)elemA<N, V>(t, q) * at.st] : T2{0, 0};
     if (a.s.scale) {
-        double s = a.s.scale[ch < a.s.ncomp ? ch : 0];
-        if (ch >= a.s.ncomp) s = s * s;
+        double s = a.s.scale[at.ch < a.s.ncomp ? at.ch : 0];
+        if (at.ch >= a.s.ncomp) s = s * s;
         const T st = (T)s;
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{v[q].x * st, v[q].y * st};
     }
     fft_forward<T2, N, V>(v, t, my, tw);
 
-    const double kx = a.kX[ix], ko = a.kO[outer];
+    const double kx = a.kX[at.ix], ko = a.kO[at.outer];
     const double kxo2 = kx * kx + ko * ko;
-    if (ch < a.s.ncomp) {
+    if (at.ch < a.s.ncomp) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const double ka = a.kA[elemB<N, V>(t, q)];
@@ -215,18 +293,24 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
         }
     } else {
         const T* wt = (const T*)a.wtab;
+        const int nxf = a.s.nx, nO = a.s.nouter;
+        const double kxp = a.kX[(nxf - at.ix) % nxf], kop = a.kO[(nO - (int)at.outer) % nO];
         const double num0 = kx * a.dX + ko * a.dO;
+        const double num0p = kxp * a.dX + kop * a.dO;
+        const double kxo2p = kxp * kxp + kop * kop;
+        const bool xo_neg = (kxp == -kx) && (kop == -ko);
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const int ea = elemB<N, V>(t, q);
             const double ka = a.kA[ea];
             double vt = 0.0;
             if (a.c_dd != 0.0) {
-                const double num = num0 + ka * a.dA;
-                const double den = sqrt(kxo2 + ka * ka);
-                if (!(den < 1E-6)) vt = a.c_dd * (((num / den) * (num / den)) - 1.0 / 3.0);
+                vt = vdd_of(a.c_dd, num0 + ka * a.dA, kxo2 + ka * ka);
+                const double kap = a.kA[(N - ea) % N];
+                if (!(xo_neg && kap == -ka))
+                    vt = 0.5 * (vt + vdd_of(a.c_dd, num0p + kap * a.dA, kxo2p + kap * kap));
             }
-            if (wt) vt = vt + (double)wt[sp_off + (long long)ea * a.s.stride];
+            if (wt && at.valid) vt = vt + (double)wt[at.sp_off + (long long)ea * a.s.stride];
             const double f = vt * a.inv_ntot;
             if (sizeof(T) == 8) {
                 v[q] = T2{(T)((double)v[q].x * f), (T)((double)v[q].y * f)};
@@ -237,8 +321,9 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
         }
     }
     fft_inverse<T2, N, V>(v, t, my, tw);
+    if (at.valid)
 #pragma unroll
-    for (int q = 0; q < E; ++q) base[(long long)elemA<N, V>(t, q) * a.s.stride] = v[q];
+        for (int q = 0; q < E; ++q) base[(long long)elemA<N, V>(t, q) * at.st] = v[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -264,8 +349,10 @@ k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
 enum { MODE_PROLOGUE = 0, MODE_RK = 1, MODE_APPLY = 2, MODE_HB = 3 };
 
 template <typename T> struct XArgs {
-    c2_t<T>* spec;
+    c2_t<T>* spec;          // complex channels, component c at c*ntot
     long long ntot;
+    c2_t<T>* specd;         // density channel, half spectrum: row r at r*dpitch, kx = 0..N/2
+    int dpitch;
     double2* P0[2];         // FP64 state psi_n (RK: psi_old of the step)
     c2_t<T>* PS[2];         // RK: stage state; HB: previous psi
     c2_t<T>* PSo[2];        // RK stages 1-3: where the next stage state goes (null: PS)
@@ -298,7 +385,7 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
-    T2* specd = a.spec + NC * a.ntot + row0;   // density channel
+    T2* specd = a.specd + ((long long)blockIdx.x * P + p) * a.dpitch;   // this row's half spectrum
 
     T2 v[E];
     T nd[E];             // density of the new state (DENS)
@@ -321,8 +408,17 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
     } else {
         T phi[E];
         if (DENS) {
+            // kx > N/2 from the conjugate symmetry of a real row's transform
 #pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = specd[elemB<N, V>(t, q)];
+            for (int q = 0; q < E; ++q) {
+                const int e = elemB<N, V>(t, q);
+                if (e <= N / 2) {
+                    v[q] = specd[e];
+                } else {
+                    const T2 w = specd[N - e];
+                    v[q] = T2{w.x, -w.y};
+                }
+            }
             fft_inverse<T2, N, V>(v, t, my, tw);
 #pragma unroll
             for (int q = 0; q < E; ++q) phi[q] = v[q].x;
@@ -450,7 +546,10 @@ k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
         for (int q = 0; q < E; ++q) v[q] = T2{nd[q], (T)0};
         fft_forward<T2, N, V>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) specd[elemB<N, V>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) {
+            const int e = elemB<N, V>(t, q);
+            if (e <= N / 2) specd[e] = v[q];
+        }
     }
 }
 

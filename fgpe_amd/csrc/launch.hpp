@@ -16,20 +16,37 @@ template <typename T> const c2_t<T>* tw_of(fgpe_ctx* c, int axis)
 
 inline long long n_outer(fgpe_ctx* c, int axis) { return axis == 1 ? c->n[2] : c->n[1]; }
 
-inline StridedArgs strided_args(fgpe_ctx* c, int axis, const double* scale, int ncomp, const int* stop)
+inline StridedArgs strided_args(fgpe_ctx* c, int axis, const double* scale, int nch, int ncomp, bool half,
+                                const int* stop)
 {
-    StridedArgs a;
+    StridedArgs a{};
     a.nx = c->n[0];
     a.chan_stride = c->ntot;
     a.scale = scale;
     a.ncomp = ncomp;
+    a.nchan = nch;
     a.stop = stop;
+    a.nouter = (int)n_outer(c, axis);
+    const long long n0 = c->n[0], n1 = c->n[1];
     if (axis == 1) {
-        a.stride = c->n[0];
-        a.outer_stride = (long long)c->n[0] * c->n[1];
+        a.stride = n0;
+        a.outer_stride = n0 * n1;
     } else {
-        a.stride = (long long)c->n[0] * c->n[1];
-        a.outer_stride = c->n[0];
+        a.stride = n0 * n1;
+        a.outer_stride = n0;
+    }
+    if (half && nch > ncomp) {
+        const long long hp = dens_pitch(c);
+        a.hcols = c->n[0] / 2 + 1;
+        a.hbase = (long long)ncomp * c->ntot;
+        a.hchan = hp * n1 * c->n[2];
+        if (axis == 1) {
+            a.hstride = hp;
+            a.houter = hp * n1;
+        } else {
+            a.hstride = hp * n1;
+            a.houter = hp;
+        }
     }
     return a;
 }
@@ -53,12 +70,12 @@ template <typename T, int N, int V> int fft_x_n(fgpe_ctx* c, const void* in, voi
 }
 
 template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
-                                                  const double* scale, const int* stop)
+                                                  const double* scale, const int* stop, bool half)
 {
     constexpr int P = ps_of<T, N, V>();
     if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    StridedArgs a = strided_args(c, axis, scale, ncomp, stop);
-    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N, V>::T);
+    StridedArgs a = strided_args(c, axis, scale, nch, ncomp, half, stop);
+    dim3 grid((unsigned)strided_blocks<P>(a)), block(P * Plan<N, V>::T);
     using T2 = c2_t<T>;
     TScope ts(c, FGPE_KCLASS_STRIDED);
     if (dir < 0)
@@ -70,12 +87,12 @@ template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int a
 }
 
 template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
-                                               const double* scale, bool filter, const int* stop)
+                                               const double* scale, bool filter, const int* stop, bool half)
 {
     constexpr int P = ps_of<T, N, V>();
     if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
     LastArgs a;
-    a.s = strided_args(c, axis, scale, ncomp, stop);
+    a.s = strided_args(c, axis, scale, nch, ncomp, half, stop);
     const int oaxis = axis == 1 ? 2 : 1;
     a.kA = c->k[axis];
     a.kX = c->k[0];
@@ -88,7 +105,7 @@ template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis
     a.inv_ntot = 1.0 / (double)c->ntot;
     a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
     a.wtab = filter ? c->wtab : nullptr;
-    dim3 grid((unsigned)((c->n[0] / P) * n_outer(c, axis)), (unsigned)nch), block(P * Plan<N, V>::T);
+    dim3 grid((unsigned)strided_blocks<P>(a.s)), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_LAST);
     hipLaunchKernelGGL((k_last<T, N, V>), grid, block, 0, c->stream, (c2_t<T>*)data, tw_of<T>(c, axis), a);
     FGPE_LAUNCHCHK("k_last");
@@ -181,18 +198,18 @@ template <typename T> int Launch<T>::fft_x(fgpe_ctx* c, const void* in, void* ou
 }
 
 template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
-                                             const double* scale, const int* stop)
+                                             const double* scale, const int* stop, bool half)
 {
-#define CALL(N_, V_) detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop)
+#define CALL(N_, V_) detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop, half)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_STRIDED][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported strided axis length");
 }
 
 template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
-                                          const double* scale, bool filter, const int* stop)
+                                          const double* scale, bool filter, const int* stop, bool half)
 {
-#define CALL(N_, V_) detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop)
+#define CALL(N_, V_) detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported last axis length");

@@ -2,11 +2,15 @@
 fractions are quoted against).
 
 Every kernel reads and writes its operands exactly once; an FFT pass moves a
-spectral channel in and out once (2 * S_c bytes per point).  Per point:
+spectral channel in and out once (2 * S_c bytes per point).  The spectral
+channels are the ncomp complex components plus, when a k-space kernel is on,
+the total density stored as a half spectrum: kx = 0..nx/2 of each x row, so
+it moves a fraction h = (nx/2 + 1)/nx of a complex channel.  Per point, with
+C = ncomp + h (density on) or ncomp (off):
 
-  strided (y fwd or inv) / last (fwd + multiply + inv):  2 * nch * S_c
-      (+ S_r for the Vt table when a soft-core term is on, last only)
-  xstage, stage s (1 component):  2 * nch * S_c  plus the pointwise operands
+  strided (y fwd or inv) / last (fwd + multiply + inv):  2 * C * S_c
+      (+ h * S_r for the Vt table when a soft-core term is on, last only)
+  xstage, stage s:  2 * C * S_c  plus, per component, the pointwise operands
       s=1: read P0, V;            write K, PS
       s=2,3: read PS, P0, V, K;   write K, PS
       s=4: read PS, P0, K, V;     write P0
@@ -23,24 +27,35 @@ def sizes(mixed: bool):
     return sc, sc // 2, 16   # complex bulk, real V, FP64 state
 
 
-def xstage_bytes(stage: int, mixed: bool, nch: int) -> int:
+def half_fraction(nx: int) -> float:
+    """Share of a complex channel that a half-spectrum real channel moves."""
+    return (nx // 2 + 1) / nx
+
+
+def channels(ncomp: int, dens: bool, nx: int) -> float:
+    return ncomp + (half_fraction(nx) if dens else 0.0)
+
+
+def xstage_bytes(stage: int, mixed: bool, ncomp: int = 1, dens: bool = True, nx: int = 256) -> float:
     sc, sv, s0 = sizes(mixed)
-    spec = 2 * nch * sc
+    spec = 2 * channels(ncomp, dens, nx) * sc
     if stage == 1:
         pw = s0 + sv + sc + sc
     elif stage in (2, 3):
         pw = sc + s0 + sv + 2 * sc + sc
     else:
         pw = sc + 2 * s0 + sc + sv
-    return spec + pw
+    return spec + ncomp * pw
 
 
-def pass_bytes(mixed: bool, nch: int, table: bool = False) -> int:
+def pass_bytes(mixed: bool, ncomp: int = 1, dens: bool = True, nx: int = 256, table: bool = False) -> float:
     sc, sv, _ = sizes(mixed)
-    return 2 * nch * sc + (sv if table else 0)
+    return 2 * channels(ncomp, dens, nx) * sc + (half_fraction(nx) * sv if table else 0.0)
 
 
-def step_bytes_per_point(mixed: bool, nch: int, three_d: bool = True, table: bool = False) -> int:
+def step_bytes_per_point(mixed: bool, ncomp: int = 1, dens: bool = True, nx: int = 256, three_d: bool = True,
+                         table: bool = False) -> float:
     """Bytes per grid point of one RK4 step (4 stages)."""
-    per_stage_passes = (2 * pass_bytes(mixed, nch) if three_d else 0) + pass_bytes(mixed, nch, table)
-    return 4 * per_stage_passes + sum(xstage_bytes(s, mixed, nch) for s in (1, 2, 3, 4))
+    plain = pass_bytes(mixed, ncomp, dens, nx)
+    per_stage_passes = (2 * plain if three_d else 0) + pass_bytes(mixed, ncomp, dens, nx, table)
+    return 4 * per_stage_passes + sum(xstage_bytes(s, mixed, ncomp, dens, nx) for s in (1, 2, 3, 4))
