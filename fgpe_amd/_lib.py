@@ -27,6 +27,8 @@ EXPORTS = (
     "fgpe_observables", "fgpe_adjoint_forward", "fgpe_adjoint_backward", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
     "fgpe_timing_enable",
     "fgpe_timing_read", "fgpe_autotune", "fgpe_set_variant", "fgpe_get_variant",
+    "fgpe_group_create", "fgpe_group_destroy", "fgpe_group_autotune", "fgpe_group_imag_steps",
+    "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
 )
 KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
 
@@ -84,6 +86,17 @@ def load() -> C.CDLL:
     L.fgpe_get_variant.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
     L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
     L.fgpe_timing_read.argtypes = [_vp, _dp]
+    _ip = C.POINTER(C.c_int)
+    L.fgpe_group_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]
+    L.fgpe_group_destroy.argtypes = [_vp]
+    L.fgpe_group_autotune.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_group_imag_steps.argtypes = [_vp, C.c_double, C.c_double, _dp, C.c_double, C.c_int]
+    L.fgpe_group_imag_status.argtypes = [_vp, _ip, _ip]
+    L.fgpe_group_imag_finish.argtypes = [_vp, _ip, _ip]
+    L.fgpe_graph_capture_imag.argtypes = [C.POINTER(_vp), C.c_int, C.c_double, C.c_double, _dp, C.c_double,
+                                          C.c_int, C.POINTER(_vp)]
+    L.fgpe_graph_launch.argtypes = [_vp]
+    L.fgpe_graph_destroy.argtypes = [_vp]
     _lib = L
     return L
 
@@ -264,3 +277,79 @@ class Context:
         check(load().fgpe_imag_steps(self.h, scheme, dt, beta, _dptr(np_), eps, nsteps,
                                      _dptr(hist) if history else _dp(), C.byref(done)))
         return (done.value, hist[:done.value]) if history else done.value
+
+
+class Group:
+    """Several same-shape contexts run as batched launches (fgpe_group_*).
+
+    Keeps references to its contexts; close it before them."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        arr = (_vp * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = _vp()
+        check(load().fgpe_group_create(arr, len(self.ctxs), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().fgpe_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, reps=3):
+        out = np.zeros(6)
+        check(load().fgpe_group_autotune(self.h, reps, _dptr(out)))
+        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6])}
+
+    def imag_steps(self, dt, nsteps, beta=0.0, npart=1.0, eps=0.0):
+        np_ = np.ascontiguousarray(np.broadcast_to(np.asarray(npart, dtype=np.float64), (2,)))
+        check(load().fgpe_group_imag_steps(self.h, dt, beta, _dptr(np_), eps, nsteps))
+
+    def _flags(self, fn):
+        n = len(self.ctxs)
+        it = (C.c_int * n)()
+        cv = (C.c_int * n)()
+        check(fn(self.h, it, cv))
+        return np.array(it[:], dtype=np.int64), np.array(cv[:], dtype=bool)
+
+    def imag_status(self):
+        """(iterations run, converged) per instance; the run can go on."""
+        return self._flags(load().fgpe_group_imag_status)
+
+    def imag_finish(self):
+        """(iterations run, converged) per instance, after applying the pending scales."""
+        return self._flags(load().fgpe_group_imag_finish)
+
+
+class Graph:
+    """`iters` heavy-ball iterations of every group captured as one hipGraph."""
+
+    def __init__(self, groups, dt, iters, beta=0.0, npart=1.0, eps=0.0):
+        self.groups = list(groups)
+        arr = (_vp * len(self.groups))(*[g.h for g in self.groups])
+        np_ = np.ascontiguousarray(np.broadcast_to(np.asarray(npart, dtype=np.float64), (2,)))
+        h = _vp()
+        check(load().fgpe_graph_capture_imag(arr, len(self.groups), dt, beta, _dptr(np_), eps, iters,
+                                             C.byref(h)))
+        self.h = h
+        self.iters = iters
+
+    def launch(self):
+        check(load().fgpe_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            load().fgpe_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -180,3 +180,64 @@ def twocomp_softcore_256(n=256, nsteps=2000):
     return Workload(f"twocomp_softcore_{n}", GridSpec(n, n, n, 16.0, 16.0, 16.0), 2,
                     (100.0 * f, 100.0 * f, 80.0 * f), 5e-4, nsteps, False, (1.0, 1.0, 1.0),
                     sc_U=5.0, sc_R=1.5, seed=3)
+
+
+# ---- batched ground-state sweep ------------------------------------------
+
+SWEEP_SHAPES = ((64, 64, 64), (96, 96, 96), (128, 128, 128), (64, 64, 192), (512, 512, 1))
+
+
+@dataclasses.dataclass
+class SweepInstance:
+    """One imaginary-time ground-state problem of the sweep."""
+    index: int
+    grid: GridSpec
+    a: float            # contact length, g = 4 pi a
+    omega: tuple
+    a_dd: float = 0.0
+    dip: tuple = (0.0, 0.0, 1.0)
+    sc_U: float = 0.0
+    sc_R: float = 0.0
+    sc_dim: int = 3
+    seed: int = 0
+
+    @property
+    def kind(self):
+        return "dipolar" if self.a_dd != 0.0 else "softcore"
+
+    def initial_state(self):
+        return gaussian_noisy(self.grid, self.seed, width=1.0, noise=0.01)
+
+    def trap(self):
+        return harmonic_trap(self.grid, self.omega)
+
+
+def sweep_instances(n=64, seed=5, shapes=SWEEP_SHAPES):
+    """n instances: shapes cycled through `shapes`, grid spacing 0.25 in 3D
+    (box 16 per 64 points) and 1/8 in 2D (box 64); g ~ U[50, 300]; trap
+    omega = (l, l, 1) in 3D (pancake, aspect l ~ U[0.2, 1]) and (1, l) in 2D;
+    even instances dipolar with eps_dd ~ U[0.5, 1.5] (dipoles along z in 3D,
+    along x in 2D, where a z dipole gives a constant -4 pi a_dd), odd ones a
+    soft-core term U0 ~ U[1, 10], R ~ U[1, 2] (3D or 2D transform)."""
+    u = 0.5 * (splitmix64_uniform(seed, 4 * n).reshape(n, 4) + 1.0)   # U[0, 1)
+    out = []
+    for i in range(n):
+        nx, ny, nz = shapes[i % len(shapes)]
+        if nz > 1:
+            grid = GridSpec(nx, ny, nz, nx * 0.25, ny * 0.25, nz * 0.25)
+        else:
+            grid = GridSpec(nx, ny, 1, nx / 8.0, ny / 8.0, 1.0)
+        g = 50.0 + 250.0 * u[i, 0]
+        lam = 0.2 + 0.8 * u[i, 1]
+        omega = (lam, lam, 1.0) if nz > 1 else (1.0, lam, 0.0)
+        a = g / (4 * PI)
+        inst = SweepInstance(i, grid, a, omega, seed=1000 + i)
+        if i % 2 == 0:
+            inst.a_dd = (0.5 + 1.0 * u[i, 2]) * a
+            inst.dip = (0.0, 0.0, 1.0) if nz > 1 else (1.0, 0.0, 0.0)
+        else:
+            inst.sc_U = 1.0 + 9.0 * u[i, 2]
+            inst.sc_R = 1.0 + 1.0 * u[i, 3]
+            inst.sc_dim = 3 if nz > 1 else 2
+        out.append(inst)
+    return out

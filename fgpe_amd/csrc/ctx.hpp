@@ -127,19 +127,81 @@ struct TScope {
 // supported nx and both precisions).
 inline long long dens_pitch(const fgpe_ctx* c) { return c->n[0] / 2 + 16; }
 
+inline long long n_outer(const fgpe_ctx* c, int axis) { return axis == 1 ? c->n[2] : c->n[1]; }
+
+// Arguments of the strided passes and of the k-space multiply for one context.
+inline StridedArgs strided_args(const fgpe_ctx* c, void* data, int axis, const double* scale, int nch, int ncomp,
+                                bool half, const int* stop)
+{
+    StridedArgs a{};
+    a.data = data;
+    a.nx = c->n[0];
+    a.chan_stride = c->ntot;
+    a.scale = scale;
+    a.ncomp = ncomp;
+    a.nchan = nch;
+    a.stop = stop;
+    a.nouter = (int)n_outer(c, axis);
+    const long long n0 = c->n[0], n1 = c->n[1];
+    if (axis == 1) {
+        a.stride = n0;
+        a.outer_stride = n0 * n1;
+    } else {
+        a.stride = n0 * n1;
+        a.outer_stride = n0;
+    }
+    if (half && nch > ncomp) {
+        const long long hp = dens_pitch(c);
+        a.hcols = c->n[0] / 2 + 1;
+        a.hbase = (long long)ncomp * c->ntot;
+        a.hchan = hp * n1 * c->n[2];
+        if (axis == 1) {
+            a.hstride = hp;
+            a.houter = hp * n1;
+        } else {
+            a.hstride = hp * n1;
+            a.houter = hp;
+        }
+    }
+    return a;
+}
+
+// axis: the last axis
+inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale,
+                          bool filter, const int* stop, bool half)
+{
+    LastArgs a;
+    a.s = strided_args(c, data, axis, scale, nch, ncomp, half, stop);
+    const int oaxis = axis == 1 ? 2 : 1;
+    a.kA = c->k[axis];
+    a.kX = c->k[0];
+    a.kO = c->k[oaxis];
+    const double* d = c->hp.dip;
+    a.dA = d[axis];
+    a.dX = d[0];
+    a.dO = d[oaxis];
+    a.lap_scale = 19.739208802 / (double)c->ntot;
+    a.inv_ntot = 1.0 / (double)c->ntot;
+    a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
+    a.wtab = filter ? c->wtab : nullptr;
+    return a;
+}
+
 // ---- per-precision launchers (launch_f64.hip / launch_f32.hip) --------------
 // axis: 1 = y, 2 = z.  nch channels at ctx->spec (or `data`): ncomp complex
 // channels of ntot points, then real-field channels, in the half-spectrum
 // layout when `half` is set (rows of dens_pitch elements), else like the
-// complex ones.
+// complex ones.  tab / ninst: a device table of ninst per-instance argument
+// structs for a batched launch over same-shape contexts (null: this context).
 template <typename T> struct Launch {
     static long long xblocks(fgpe_ctx* c);
     static int fft_x(fgpe_ctx* c, const void* in, void* out, int dir);
     static int strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp, const double* scale,
-                       const int* stop, bool half);
+                       const int* stop, bool half, const void* tab = nullptr, int ninst = 1);
     static int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter,
-                    const int* stop, bool half);
-    static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode);
+                    const int* stop, bool half, const void* tab = nullptr, int ninst = 1);
+    static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode, const void* tab = nullptr,
+                      int ninst = 1);
     // adjoint VJP prologue / epilogue; y64: the state is FP64 (else T)
     static int adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epilogue);
 };

@@ -16,9 +16,10 @@ using namespace fgpe;
 
 namespace fgpe {
 
-__global__ void __launch_bounds__(256) k_finalize(FinArgs f)
+__global__ void __launch_bounds__(256) k_finalize(FinArgs f0, const FinArgs* __restrict__ tab)
 {
     __shared__ double sh[6][256];
+    const FinArgs f = tab ? tab[blockIdx.x] : f0;
     if (f.stop && *f.stop) return;
     const int t = threadIdx.x;
     const int nv = 3 * f.nc;
@@ -237,25 +238,13 @@ template <typename T> int prologue(fgpe_ctx* c)
     return FGPE_OK;
 }
 
+FinArgs fin_args(fgpe_ctx* c, long long nblocks, int kind, const double* npart, double* out, double eps);
+
 int finalize(fgpe_ctx* c, long long nblocks, int kind, const double* npart, double* out, double eps)
 {
-    FinArgs f{};
-    f.partials = c->partials;
-    f.nblocks = (int)nblocks;
-    f.nc = c->hp.ncomp;
-    f.kind = kind;
-    f.dv = c->dv;
-    f.npart[0] = npart[0];
-    f.npart[1] = c->hp.ncomp == 2 ? npart[1] : npart[0];
-    f.out = out;
-    f.scale = c->scal + SC_SCALE;
-    f.mu = c->scal + SC_MU;
-    f.eprev = c->scal + SC_EPREV;
-    f.stop = kind == FIN_HB ? c->flags + FL_STOP : nullptr;
-    f.done = c->flags + FL_DONE;
-    f.eps = eps;
+    const FinArgs f = fin_args(c, nblocks, kind, npart, out, eps);
     TScope ts(c, FGPE_KCLASS_OTHER);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, f, (const FinArgs*)nullptr);
     FGPE_LAUNCHCHK("k_finalize");
     return FGPE_OK;
 }
@@ -307,6 +296,65 @@ __global__ void k_hb_init(double* scal, int* flags)
     flags[FL_DONE] = 0;
 }
 
+// Heavy-ball set-up: on a fresh run resets the iteration scalars and sets
+// old = psi (src/oneComponentGPSolver.cu:33), on a continuation only clears
+// the iteration counter; then the x prologue of the current state.
+template <typename T> int hb_prepare(fgpe_ctx* c)
+{
+    const int nc = c->hp.ncomp;
+    if (c->hb_active) {
+        hipLaunchKernelGGL(k_hb_continue, dim3(1), dim3(1), 0, c->stream, c->flags);
+        FGPE_LAUNCHCHK("k_hb_continue");
+    } else {
+        hipLaunchKernelGGL(k_hb_init, dim3(1), dim3(1), 0, c->stream, c->scal, c->flags);
+        FGPE_LAUNCHCHK("k_hb_init");
+        for (int i = 0; i < nc; ++i) {
+            if (sizeof(T) == 8) {
+                FGPE_HIPCHK(hipMemcpyAsync(c->PS[i], c->P0[i], sizeof(double2) * c->ntot, hipMemcpyDeviceToDevice,
+                                           c->stream));
+            } else {
+                const long long n = 2 * c->ntot;
+                hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                                   c->stream, (float*)c->PS[i], (const double*)c->P0[i], n);
+                FGPE_LAUNCHCHK("k_convert");
+            }
+        }
+        c->hb_active = true;
+    }
+    return prologue<T>(c);
+}
+
+template <typename T> XArgs<T> hb_xargs(fgpe_ctx* c, double t, double dt, double beta)
+{
+    XArgs<T> a = base_xargs<T>(c, t);
+    a.scale = c->scal + SC_SCALE;
+    a.dt = dt;
+    a.beta = beta;
+    a.mu = c->hp.ncomp == 2 ? c->scal + SC_MU : nullptr;
+    a.stop = c->flags + FL_STOP;
+    return a;
+}
+
+FinArgs fin_args(fgpe_ctx* c, long long nblocks, int kind, const double* npart, double* out, double eps)
+{
+    FinArgs f{};
+    f.partials = c->partials;
+    f.nblocks = (int)nblocks;
+    f.nc = c->hp.ncomp;
+    f.kind = kind;
+    f.dv = c->dv;
+    f.npart[0] = npart[0];
+    f.npart[1] = c->hp.ncomp == 2 ? npart[1] : npart[0];
+    f.out = out;
+    f.scale = c->scal + SC_SCALE;
+    f.mu = c->scal + SC_MU;
+    f.eprev = c->scal + SC_EPREV;
+    f.stop = kind == FIN_HB ? c->flags + FL_STOP : nullptr;
+    f.done = c->flags + FL_DONE;
+    f.eps = eps;
+    return f;
+}
+
 // Heavy-ball imaginary time (the reference's scheme), fully on the device: the
 // convergence test runs in k_finalize and stops every later launch.
 template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const double* npart, double eps,
@@ -317,38 +365,12 @@ template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const doub
     const bool dens = c->nch > nc;
     if (nc == 2 && dens)
         return fail(FGPE_EUNSUP, "imaginary time with two components and a k-space kernel is not available");
-    if (c->hb_active) {
-        hipLaunchKernelGGL(k_hb_continue, dim3(1), dim3(1), 0, c->stream, c->flags);
-        FGPE_LAUNCHCHK("k_hb_continue");
-    } else {
-    hipLaunchKernelGGL(k_hb_init, dim3(1), dim3(1), 0, c->stream, c->scal, c->flags);
-    FGPE_LAUNCHCHK("k_hb_init");
-    // old = psi (src/oneComponentGPSolver.cu:33)
-    for (int i = 0; i < nc; ++i) {
-        if (sizeof(T) == 8) {
-            FGPE_HIPCHK(hipMemcpyAsync(c->PS[i], c->P0[i], sizeof(double2) * c->ntot, hipMemcpyDeviceToDevice,
-                                       c->stream));
-        } else {
-            const long long n = 2 * c->ntot;
-            hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                               c->stream, (float*)c->PS[i], (const double*)c->P0[i], n);
-            FGPE_LAUNCHCHK("k_convert");
-        }
-    }
-    c->hb_active = true;
-    }
-    FGPE_RET(prologue<T>(c));
+    FGPE_RET(hb_prepare<T>(c));
     const long long nb = Launch<T>::xblocks(c);
     const int* stop = c->flags + FL_STOP;
     for (int it = 0; it < nsteps; ++it) {
         FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, stop));
-        XArgs<T> a = base_xargs<T>(c, it * dt);
-        a.scale = c->scal + SC_SCALE;
-        a.dt = dt;
-        a.beta = beta;
-        a.mu = nc == 2 ? c->scal + SC_MU : nullptr;
-        a.stop = stop;
-        FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_HB));
+        FGPE_RET(Launch<T>::xstage(c, hb_xargs<T>(c, it * dt, dt, beta), nc, dens, MODE_HB));
         FGPE_RET(finalize(c, nb, FIN_HB, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP, eps));
     }
     return FGPE_OK;
@@ -676,6 +698,192 @@ int ensure_hist(fgpe_ctx* c, size_t n)
         c->hist_n = n;
     }
     return FGPE_OK;
+}
+
+}  // namespace
+
+// ---- batched groups and captured graphs ---------------------------------
+// A group's per-instance argument tables live in one device buffer: the
+// y-forward, last-axis, y-inverse, x-stage and finalize arguments of every
+// instance, each table 256-byte aligned.
+struct fgpe_group {
+    std::vector<fgpe_ctx*> c;
+    void* tab = nullptr;   // device argument tables of the last batched run
+    size_t cap = 0;
+};
+
+struct fgpe_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    void* tab = nullptr;   // the argument tables the captured launches read
+};
+
+namespace {
+
+struct GroupTabs {
+    size_t o_sf = 0, o_l = 0, o_si = 0, o_x = 0, o_f = 0, bytes = 0;
+    std::vector<unsigned char> host;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// timing: no scale or stop in the passes and an RK stage-2 x-stage (reads P0,
+// writes PS and K, so the state survives); else one heavy-ball iteration.
+template <typename T> GroupTabs group_tabs(fgpe_group* g, double dt, double beta, const double* npart, double eps,
+                                           bool timing)
+{
+    const size_t n = g->c.size();
+    GroupTabs t;
+    t.o_sf = 0;
+    t.o_l = align256(t.o_sf + n * sizeof(StridedArgs));
+    t.o_si = align256(t.o_l + n * sizeof(LastArgs));
+    t.o_x = align256(t.o_si + n * sizeof(StridedArgs));
+    t.o_f = align256(t.o_x + n * sizeof(XArgs<T>));
+    t.bytes = align256(t.o_f + n * sizeof(FinArgs));
+    t.host.assign(t.bytes, 0);
+    for (size_t i = 0; i < n; ++i) {
+        fgpe_ctx* c = g->c[i];
+        const int la = last_axis(c);
+        const double* scale = timing ? nullptr : c->scal + SC_SCALE;
+        const int* stop = timing ? nullptr : c->flags + FL_STOP;
+        const StridedArgs sf = strided_args(c, c->spec, 1, scale, c->nch, 1, true, stop);
+        const LastArgs l = last_args(c, c->spec, la, c->nch, 1, la == 2 ? nullptr : scale, has_filter(c->hp), stop,
+                                     true);
+        const StridedArgs si = strided_args(c, c->spec, 1, nullptr, c->nch, 1, true, stop);
+        XArgs<T> x;
+        if (timing) {
+            x = base_xargs<T>(c, 0.0);
+            x.cr = 0.0;
+            x.ci = -1e-30;
+            x.stage = 2;
+        } else {
+            x = hb_xargs<T>(c, 0.0, dt, beta);
+        }
+        const FinArgs f = fin_args(c, Launch<T>::xblocks(c), FIN_HB, npart, c->scal + SC_TMP, eps);
+        std::memcpy(&t.host[t.o_sf + i * sizeof(StridedArgs)], &sf, sizeof(sf));
+        std::memcpy(&t.host[t.o_l + i * sizeof(LastArgs)], &l, sizeof(l));
+        std::memcpy(&t.host[t.o_si + i * sizeof(StridedArgs)], &si, sizeof(si));
+        std::memcpy(&t.host[t.o_x + i * sizeof(XArgs<T>)], &x, sizeof(x));
+        std::memcpy(&t.host[t.o_f + i * sizeof(FinArgs)], &f, sizeof(f));
+    }
+    return t;
+}
+
+// Kernel classes of one batched iteration (cls < 0: all, then the finalize).
+template <typename T> int group_launch(fgpe_group* g, const GroupTabs& t, const unsigned char* dtab, int cls,
+                                       int mode)
+{
+    fgpe_ctx* c0 = g->c[0];
+    const int n = (int)g->c.size();
+    const int nc = 1, nch = c0->nch;
+    const bool dens = nch > nc, filter = has_filter(c0->hp);
+    const int la = last_axis(c0);
+    if ((cls < 0 || cls == FGPE_KCLASS_STRIDED) && la == 2)
+        FGPE_RET(Launch<T>::strided(c0, c0->spec, 1, -1, nch, nc, nullptr, nullptr, true, dtab + t.o_sf, n));
+    if (cls < 0 || cls == FGPE_KCLASS_LAST)
+        FGPE_RET(Launch<T>::last(c0, c0->spec, la, nch, nc, nullptr, filter, nullptr, true, dtab + t.o_l, n));
+    if ((cls < 0 || cls == FGPE_KCLASS_STRIDED) && la == 2)
+        FGPE_RET(Launch<T>::strided(c0, c0->spec, 1, +1, nch, nc, nullptr, nullptr, true, dtab + t.o_si, n));
+    if (cls < 0 || cls == FGPE_KCLASS_XSTAGE)
+        FGPE_RET(Launch<T>::xstage(c0, XArgs<T>{}, nc, dens, mode, dtab + t.o_x, n));
+    if (cls < 0) {
+        TScope ts(c0, FGPE_KCLASS_OTHER);
+        hipLaunchKernelGGL(k_finalize, dim3((unsigned)n), dim3(256), 0, c0->stream, FinArgs{},
+                           (const FinArgs*)(dtab + t.o_f));
+        FGPE_LAUNCHCHK("k_finalize");
+    }
+    return FGPE_OK;
+}
+
+int group_check(fgpe_group* g)
+{
+    fgpe_ctx* c0 = g->c[0];
+    for (fgpe_ctx* c : g->c) {
+        FGPE_RET(ensure_ready(c));
+        if (c->hp.ncomp != 1) return fail(FGPE_EUNSUP, "group instances must have one component");
+        if (c->hp.quench_on) return fail(FGPE_EUNSUP, "group instances cannot have an a(t) ramp");
+        if (c->n[0] != c0->n[0] || c->n[1] != c0->n[1] || c->n[2] != c0->n[2])
+            return fail(FGPE_EINVAL, "group instances must share the grid shape");
+        if (c->prec != c0->prec) return fail(FGPE_EINVAL, "group instances must share the precision");
+        if (c->stream != c0->stream) return fail(FGPE_EINVAL, "group instances must share the stream");
+        if (c->nch != c0->nch) return fail(FGPE_EINVAL, "group instances must all have or all lack a k-space kernel");
+        if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "group instances need a real trap");
+    }
+    return FGPE_OK;
+}
+
+// Uploads a table set after the stream has drained (earlier launches may
+// still read the previous contents).
+int group_upload(fgpe_ctx* c0, void** dtab, size_t* cap, const GroupTabs& t)
+{
+    FGPE_HIPCHK(hipStreamSynchronize(c0->stream));
+    if (*cap < t.bytes) {
+        if (*dtab) (void)hipFree(*dtab);
+        *dtab = nullptr;
+        *cap = 0;
+        FGPE_HIPCHK(hipMalloc(dtab, t.bytes));
+        *cap = t.bytes;
+    }
+    FGPE_HIPCHK(hipMemcpy(*dtab, t.host.data(), t.bytes, hipMemcpyHostToDevice));
+    return FGPE_OK;
+}
+
+template <typename T> int group_imag(fgpe_group* g, double dt, double beta, const double* npart, double eps,
+                                     int nsteps)
+{
+    FGPE_RET(group_check(g));
+    for (fgpe_ctx* c : g->c) FGPE_RET(hb_prepare<T>(c));
+    const GroupTabs t = group_tabs<T>(g, dt, beta, npart, eps, false);
+    FGPE_RET(group_upload(g->c[0], &g->tab, &g->cap, t));
+    for (int it = 0; it < nsteps; ++it)
+        FGPE_RET(group_launch<T>(g, t, (const unsigned char*)g->tab, -1, MODE_HB));
+    return FGPE_OK;
+}
+
+template <typename T> int group_autotune(fgpe_group* g, int reps, double* out)
+{
+    FGPE_RET(group_check(g));
+    fgpe_ctx* c0 = g->c[0];
+    const double one[2] = {1.0, 1.0};
+    const GroupTabs t = group_tabs<T>(g, 0.0, 0.0, one, 0.0, true);
+    FGPE_RET(group_upload(c0, &g->tab, &g->cap, t));
+    const unsigned char* dt = (const unsigned char*)g->tab;
+    const int la = last_axis(c0);
+    hipEvent_t e0, e1;
+    FGPE_HIPCHK(hipEventCreate(&e0));
+    FGPE_HIPCHK(hipEventCreate(&e1));
+    int rc = FGPE_OK;
+    const int classes[3] = {FGPE_KCLASS_STRIDED, FGPE_KCLASS_LAST, FGPE_KCLASS_XSTAGE};
+    for (int ci = 0; ci < 3 && rc == FGPE_OK; ++ci) {
+        const int cls = classes[ci];
+        if (cls == FGPE_KCLASS_STRIDED && la != 2) continue;
+        const int axis = cls == FGPE_KCLASS_STRIDED ? 1 : (cls == FGPE_KCLASS_LAST ? la : 0);
+        double best = 1e300;
+        int bestv = 0;
+        for (int v = 0; v < 2 && rc == FGPE_OK; ++v) {
+            for (fgpe_ctx* c : g->c) c->var[cls][axis] = v;
+            rc = group_launch<T>(g, t, dt, cls, MODE_RK);   // warm-up
+            if (rc != FGPE_OK) break;
+            (void)hipEventRecord(e0, c0->stream);
+            for (int r = 0; r < reps && rc == FGPE_OK; ++r) rc = group_launch<T>(g, t, dt, cls, MODE_RK);
+            (void)hipEventRecord(e1, c0->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / reps;
+            if (out) out[2 * ci + v] = per;
+            if (per < best) {
+                best = per;
+                bestv = v;
+            }
+        }
+        for (fgpe_ctx* c : g->c) c->var[cls][axis] = bestv;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (fgpe_ctx* c : g->c) c->spec_valid = false;
+    return rc;
 }
 
 }  // namespace
@@ -1086,6 +1294,153 @@ int fgpe_timing_read(fgpe_ctx* c, double* out)
     }
     c->evused = 0;
     c->evrec.clear();
+    return FGPE_OK;
+}
+
+int fgpe_group_create(fgpe_ctx* const* ctxs, int n, fgpe_group** out)
+{
+    if (!ctxs || !out) return fail(FGPE_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (n < 1) return fail(FGPE_EINVAL, "a group needs at least one context");
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(FGPE_EINVAL, "NULL context in the group");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(FGPE_EINVAL, "a context appears twice in the group");
+    }
+    fgpe_group* g = new fgpe_group();
+    g->c.assign(ctxs, ctxs + n);
+    *out = g;
+    return FGPE_OK;
+}
+
+int fgpe_group_destroy(fgpe_group* g)
+{
+    if (!g) return FGPE_OK;
+    if (g->tab) {
+        (void)hipStreamSynchronize(g->c[0]->stream);
+        (void)hipFree(g->tab);
+    }
+    delete g;
+    return FGPE_OK;
+}
+
+int fgpe_group_autotune(fgpe_group* g, int reps, double* host_ms)
+{
+    if (!g) return fail(FGPE_EINVAL, "group is NULL");
+    if (reps < 1) return fail(FGPE_EINVAL, "reps < 1");
+    if (host_ms)
+        for (int i = 0; i < 6; ++i) host_ms[i] = 0.0;
+    return g->c[0]->mixed() ? group_autotune<float>(g, reps, host_ms) : group_autotune<double>(g, reps, host_ms);
+}
+
+int fgpe_group_imag_steps(fgpe_group* g, double dt, double beta, const double* npart, double eps, int nsteps)
+{
+    if (!g || !npart) return fail(FGPE_EINVAL, "NULL argument");
+    if (nsteps < 0) return fail(FGPE_EINVAL, "nsteps < 0");
+    return g->c[0]->mixed() ? group_imag<float>(g, dt, beta, npart, eps, nsteps)
+                            : group_imag<double>(g, dt, beta, npart, eps, nsteps);
+}
+
+int fgpe_group_imag_status(fgpe_group* g, int* iters, int* converged)
+{
+    if (!g) return fail(FGPE_EINVAL, "group is NULL");
+    const size_t n = g->c.size();
+    std::vector<int> fl(FL_COUNT * n);
+    for (size_t i = 0; i < n; ++i) {
+        fgpe_ctx* c = g->c[i];
+        FGPE_HIPCHK(hipMemcpyAsync(&fl[FL_COUNT * i], c->flags, sizeof(int) * FL_COUNT, hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    for (size_t i = 0; i < n; ++i) FGPE_HIPCHK(hipStreamSynchronize(g->c[i]->stream));
+    for (size_t i = 0; i < n; ++i) {
+        if (iters) iters[i] = fl[FL_COUNT * i + FL_DONE];
+        if (converged) converged[i] = (g->c[i]->hb_active && fl[FL_COUNT * i + FL_STOP]) ? 1 : 0;
+    }
+    return FGPE_OK;
+}
+
+int fgpe_group_imag_finish(fgpe_group* g, int* iters, int* converged)
+{
+    if (!g) return fail(FGPE_EINVAL, "group is NULL");
+    for (fgpe_ctx* c : g->c) FGPE_RET(apply_pending_scale(c));
+    return fgpe_group_imag_status(g, iters, converged);
+}
+
+int fgpe_graph_capture_imag(fgpe_group* const* groups, int ngroups, double dt, double beta,
+                            const double* npart, double eps, int iters, fgpe_graph** out)
+{
+    if (!groups || !npart || !out) return fail(FGPE_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (ngroups < 1 || iters < 1) return fail(FGPE_EINVAL, "need at least one group and one iteration");
+    hipStream_t s = groups[0]->c[0]->stream;
+    if (!s) return fail(FGPE_EINVAL, "the null stream cannot be captured; set a stream on the contexts");
+    std::vector<GroupTabs> tabs(ngroups);
+    size_t total = 0;
+    std::vector<size_t> base(ngroups);
+    for (int k = 0; k < ngroups; ++k) {
+        fgpe_group* g = groups[k];
+        if (!g) return fail(FGPE_EINVAL, "NULL group");
+        FGPE_RET(group_check(g));
+        if (g->c[0]->stream != s) return fail(FGPE_EINVAL, "all groups must share one stream");
+        for (fgpe_ctx* c : g->c) {
+            c->timing = false;
+            FGPE_RET(c->mixed() ? hb_prepare<float>(c) : hb_prepare<double>(c));
+        }
+        tabs[k] = g->c[0]->mixed() ? group_tabs<float>(g, dt, beta, npart, eps, false)
+                                   : group_tabs<double>(g, dt, beta, npart, eps, false);
+        base[k] = total;
+        total += tabs[k].bytes;
+    }
+    fgpe_graph* gr = new fgpe_graph();
+    gr->stream = s;
+    int rc = FGPE_OK;
+    if (hipMalloc(&gr->tab, total) != hipSuccess) rc = fail(FGPE_EHIP, "hipMalloc of the graph tables failed");
+    for (int k = 0; k < ngroups && rc == FGPE_OK; ++k)
+        if (hipMemcpy((unsigned char*)gr->tab + base[k], tabs[k].host.data(), tabs[k].bytes,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(FGPE_EHIP, "graph table upload failed");
+    if (rc == FGPE_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(FGPE_EHIP, "stream synchronise failed");
+    if (rc == FGPE_OK && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        rc = fail(FGPE_EHIP, "hipStreamBeginCapture failed");
+    if (rc == FGPE_OK) {
+        for (int it = 0; it < iters && rc == FGPE_OK; ++it)
+            for (int k = 0; k < ngroups && rc == FGPE_OK; ++k) {
+                const unsigned char* d = (const unsigned char*)gr->tab + base[k];
+                rc = groups[k]->c[0]->mixed() ? group_launch<float>(groups[k], tabs[k], d, -1, MODE_HB)
+                                              : group_launch<double>(groups[k], tabs[k], d, -1, MODE_HB);
+            }
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc == FGPE_OK && e != hipSuccess)
+            rc = fail(FGPE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        gr->graph = graph;
+    }
+    if (rc == FGPE_OK && hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = fail(FGPE_EHIP, "hipGraphInstantiate failed");
+    if (rc != FGPE_OK) {
+        const std::string msg = g_err;
+        fgpe_graph_destroy(gr);
+        return fail(rc, msg);
+    }
+    *out = gr;
+    return FGPE_OK;
+}
+
+int fgpe_graph_launch(fgpe_graph* g)
+{
+    if (!g || !g->exec) return fail(FGPE_EINVAL, "graph is NULL");
+    FGPE_HIPCHK(hipGraphLaunch(g->exec, g->stream));
+    return FGPE_OK;
+}
+
+int fgpe_graph_destroy(fgpe_graph* g)
+{
+    if (!g) return FGPE_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->tab) (void)hipFree(g->tab);
+    delete g;
     return FGPE_OK;
 }
 

@@ -113,7 +113,12 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
 // scale (forward only, may be null): channel c < ncomp is multiplied by
 // scale[c], a real-field channel by scale[0]^2 (the pending renormalisation
 // of the imaginary-time schemes).
+//
+// Batched launches: the pass kernels take a table of per-instance argument
+// structs (device memory, one per blockIdx.y) for several contexts of one
+// shape; with a null table they use the by-value argument.
 struct StridedArgs {
+    void* data;             // the channels (c2_t<T>)
     int nx;                 // columns of the complex channels (x extent)
     long long stride;       // element stride along the axis (complex layout)
     long long outer_stride; // stride of the outer index (complex layout)
@@ -188,12 +193,14 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
 
 template <typename T, int N, int V, int DIR>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
-k_strided(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, StridedArgs a)
+k_strided(const c2_t<T>* __restrict__ tw, StridedArgs a0, const StridedArgs* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
+    const StridedArgs a = tab ? tab[blockIdx.y] : a0;
     if (a.stop && *a.stop) return;
+    T2* __restrict__ data = (T2*)a.data;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a, p);
     T2* base = data + at.off;
@@ -255,12 +262,14 @@ __device__ __forceinline__ double vdd_of(double c_dd, double num, double k2)
 
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
-k_last(c2_t<T>* __restrict__ data, const c2_t<T>* __restrict__ tw, LastArgs a)
+k_last(const c2_t<T>* __restrict__ tw, LastArgs a0, const LastArgs* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
+    const LastArgs a = tab ? tab[blockIdx.y] : a0;
     if (a.s.stop && *a.s.stop) return;
+    T2* __restrict__ data = (T2*)a.s.data;
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a.s, p);
     T2* base = data + at.off;
@@ -375,12 +384,13 @@ template <typename T> struct XArgs {
 
 template <typename T, int N, int V, int NC, bool DENS, int MODE>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
-k_xstage(XArgs<T> a, const c2_t<T>* __restrict__ tw)
+k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     constexpr int NV = 3 * NC;
     __shared__ T2 lds[P * lds_stride<N>()];
+    const XArgs<T> a = tab ? tab[blockIdx.y] : a0;
     if (a.stop && *a.stop) return;
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
@@ -719,7 +729,8 @@ struct FinArgs {
     double eps;
 };
 
-__global__ void k_finalize(FinArgs f);   // defined in fgpe_capi.hip
+// one block per instance (blockIdx.x indexes tab when it is non-null)
+__global__ void k_finalize(FinArgs f0, const FinArgs* __restrict__ tab);   // defined in fgpe_capi.hip
 
 template <typename TD, typename TS>
 __global__ void k_convert(TD* __restrict__ dst, const TS* __restrict__ src, long long n)

@@ -14,43 +14,6 @@ template <typename T> const c2_t<T>* tw_of(fgpe_ctx* c, int axis)
     return (const c2_t<T>*)(sizeof(T) == 8 ? c->tw64[axis] : c->tw32[axis]);
 }
 
-inline long long n_outer(fgpe_ctx* c, int axis) { return axis == 1 ? c->n[2] : c->n[1]; }
-
-inline StridedArgs strided_args(fgpe_ctx* c, int axis, const double* scale, int nch, int ncomp, bool half,
-                                const int* stop)
-{
-    StridedArgs a{};
-    a.nx = c->n[0];
-    a.chan_stride = c->ntot;
-    a.scale = scale;
-    a.ncomp = ncomp;
-    a.nchan = nch;
-    a.stop = stop;
-    a.nouter = (int)n_outer(c, axis);
-    const long long n0 = c->n[0], n1 = c->n[1];
-    if (axis == 1) {
-        a.stride = n0;
-        a.outer_stride = n0 * n1;
-    } else {
-        a.stride = n0 * n1;
-        a.outer_stride = n0;
-    }
-    if (half && nch > ncomp) {
-        const long long hp = dens_pitch(c);
-        a.hcols = c->n[0] / 2 + 1;
-        a.hbase = (long long)ncomp * c->ntot;
-        a.hchan = hp * n1 * c->n[2];
-        if (axis == 1) {
-            a.hstride = hp;
-            a.houter = hp * n1;
-        } else {
-            a.hstride = hp * n1;
-            a.houter = hp;
-        }
-    }
-    return a;
-}
-
 template <typename T, int N, int V> int fft_x_n(fgpe_ctx* c, const void* in, void* out, int dir)
 {
     constexpr int P = px_of<T, N, V>();
@@ -70,75 +33,71 @@ template <typename T, int N, int V> int fft_x_n(fgpe_ctx* c, const void* in, voi
 }
 
 template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
-                                                  const double* scale, const int* stop, bool half)
+                                                  const double* scale, const int* stop, bool half, const void* tab,
+                                                  int ninst)
 {
     constexpr int P = ps_of<T, N, V>();
     if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    StridedArgs a = strided_args(c, axis, scale, nch, ncomp, half, stop);
-    dim3 grid((unsigned)strided_blocks<P>(a)), block(P * Plan<N, V>::T);
-    using T2 = c2_t<T>;
+    StridedArgs a = strided_args(c, data, axis, scale, nch, ncomp, half, stop);
+    dim3 grid((unsigned)strided_blocks<P>(a), (unsigned)ninst), block(P * Plan<N, V>::T);
+    const StridedArgs* t = (const StridedArgs*)tab;
     TScope ts(c, FGPE_KCLASS_STRIDED);
     if (dir < 0)
-        hipLaunchKernelGGL((k_strided<T, N, V, -1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
+        hipLaunchKernelGGL((k_strided<T, N, V, -1>), grid, block, 0, c->stream, tw_of<T>(c, axis), a, t);
     else
-        hipLaunchKernelGGL((k_strided<T, N, V, +1>), grid, block, 0, c->stream, (T2*)data, tw_of<T>(c, axis), a);
+        hipLaunchKernelGGL((k_strided<T, N, V, +1>), grid, block, 0, c->stream, tw_of<T>(c, axis), a, t);
     FGPE_LAUNCHCHK("k_strided");
     return FGPE_OK;
 }
 
 template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
-                                               const double* scale, bool filter, const int* stop, bool half)
+                                               const double* scale, bool filter, const int* stop, bool half,
+                                               const void* tab, int ninst)
 {
     constexpr int P = ps_of<T, N, V>();
     if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    LastArgs a;
-    a.s = strided_args(c, axis, scale, nch, ncomp, half, stop);
-    const int oaxis = axis == 1 ? 2 : 1;
-    a.kA = c->k[axis];
-    a.kX = c->k[0];
-    a.kO = c->k[oaxis];
-    const double* d = c->hp.dip;
-    a.dA = d[axis];
-    a.dX = d[0];
-    a.dO = d[oaxis];
-    a.lap_scale = 19.739208802 / (double)c->ntot;
-    a.inv_ntot = 1.0 / (double)c->ntot;
-    a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
-    a.wtab = filter ? c->wtab : nullptr;
-    dim3 grid((unsigned)strided_blocks<P>(a.s)), block(P * Plan<N, V>::T);
+    const LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half);
+    dim3 grid((unsigned)strided_blocks<P>(a.s), (unsigned)ninst), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_LAST);
-    hipLaunchKernelGGL((k_last<T, N, V>), grid, block, 0, c->stream, (c2_t<T>*)data, tw_of<T>(c, axis), a);
+    hipLaunchKernelGGL((k_last<T, N, V>), grid, block, 0, c->stream, tw_of<T>(c, axis), a, (const LastArgs*)tab);
     FGPE_LAUNCHCHK("k_last");
     return FGPE_OK;
 }
 
-template <typename T, int N, int V, int NC, bool DENS, int MODE> int xstage_t(fgpe_ctx* c, const XArgs<T>& a)
+template <typename T, int N, int V, int NC, bool DENS, int MODE>
+int xstage_t(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
 {
     constexpr int P = px_of<T, N, V>();
     const long long rows = c->ntot / N;
     if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
+    dim3 grid((unsigned)(rows / P), (unsigned)ninst), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_XSTAGE);
-    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, a, tw_of<T>(c, 0),
+                       (const XArgs<T>*)tab);
     FGPE_LAUNCHCHK("k_xstage");
     return FGPE_OK;
 }
 
-template <typename T, int N, int V, int NC, bool DENS> int xstage_m(fgpe_ctx* c, const XArgs<T>& a, int mode)
+template <typename T, int N, int V, int NC, bool DENS>
+int xstage_m(fgpe_ctx* c, const XArgs<T>& a, int mode, const void* tab, int ninst)
 {
     switch (mode) {
-    case MODE_PROLOGUE: return xstage_t<T, N, V, NC, DENS, MODE_PROLOGUE>(c, a);
-    case MODE_RK: return xstage_t<T, N, V, NC, DENS, MODE_RK>(c, a);
-    case MODE_APPLY: return xstage_t<T, N, V, NC, DENS, MODE_APPLY>(c, a);
-    case MODE_HB: return xstage_t<T, N, V, NC, DENS, MODE_HB>(c, a);
+    case MODE_PROLOGUE: return xstage_t<T, N, V, NC, DENS, MODE_PROLOGUE>(c, a, tab, ninst);
+    case MODE_RK: return xstage_t<T, N, V, NC, DENS, MODE_RK>(c, a, tab, ninst);
+    case MODE_APPLY: return xstage_t<T, N, V, NC, DENS, MODE_APPLY>(c, a, tab, ninst);
+    case MODE_HB: return xstage_t<T, N, V, NC, DENS, MODE_HB>(c, a, tab, ninst);
     }
     return fail(FGPE_EINVAL, "bad x-stage mode");
 }
 
-template <typename T, int N, int V> int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
+template <typename T, int N, int V>
+int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode, const void* tab, int ninst)
 {
-    if (nc == 1) return dens ? xstage_m<T, N, V, 1, true>(c, a, mode) : xstage_m<T, N, V, 1, false>(c, a, mode);
-    return dens ? xstage_m<T, N, V, 2, true>(c, a, mode) : xstage_m<T, N, V, 2, false>(c, a, mode);
+    if (nc == 1)
+        return dens ? xstage_m<T, N, V, 1, true>(c, a, mode, tab, ninst)
+                    : xstage_m<T, N, V, 1, false>(c, a, mode, tab, ninst);
+    return dens ? xstage_m<T, N, V, 2, true>(c, a, mode, tab, ninst)
+                : xstage_m<T, N, V, 2, false>(c, a, mode, tab, ninst);
 }
 
 template <typename T, typename YT, int N, int V, bool DENS> int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
@@ -198,26 +157,29 @@ template <typename T> int Launch<T>::fft_x(fgpe_ctx* c, const void* in, void* ou
 }
 
 template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
-                                             const double* scale, const int* stop, bool half)
+                                             const double* scale, const int* stop, bool half, const void* tab,
+                                             int ninst)
 {
-#define CALL(N_, V_) detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop, half)
+#define CALL(N_, V_) detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop, half, tab, ninst)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_STRIDED][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported strided axis length");
 }
 
 template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
-                                          const double* scale, bool filter, const int* stop, bool half)
+                                          const double* scale, bool filter, const int* stop, bool half,
+                                          const void* tab, int ninst)
 {
-#define CALL(N_, V_) detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half)
+#define CALL(N_, V_) detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half, tab, ninst)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported last axis length");
 }
 
-template <typename T> int Launch<T>::xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode)
+template <typename T> int Launch<T>::xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode,
+                                            const void* tab, int ninst)
 {
-#define CALL(N_, V_) detail::xstage_n<T, N_, V_>(c, a, nc, dens, mode)
+#define CALL(N_, V_) detail::xstage_n<T, N_, V_>(c, a, nc, dens, mode, tab, ninst)
     FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");

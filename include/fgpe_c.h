@@ -183,6 +183,48 @@ int fgpe_autotune(fgpe_ctx* ctx, int reps, double* host_ms);
 int fgpe_set_variant(fgpe_ctx* ctx, int kclass, int axis, int variant);
 int fgpe_get_variant(fgpe_ctx* ctx, int kclass, int axis, int* variant);
 
+/* ---- batched sweeps (no counterpart in the reference, which runs one
+   instance per process: src/oneComponentGPSolver.cu:26-104) ----------------
+   A group is n contexts of one shape, precision, stream and Hamiltonian kind
+   (one component, the same k-space kernel configuration; a, a_dd, the
+   soft-core term, trap and state may differ).  Its kernels run as one launch
+   per kernel class with a table of per-instance arguments, so small grids
+   fill the GPU together.  The contexts stay usable on their own; destroy the
+   group before any of its contexts. */
+typedef struct fgpe_group fgpe_group;
+int fgpe_group_create(fgpe_ctx* const* ctxs, int n, fgpe_group** out);
+int fgpe_group_destroy(fgpe_group* group);
+/* Times both plan variants of each kernel class as batched launches and sets
+   the faster on every context (host_ms: 6 doubles or NULL, as fgpe_autotune). */
+int fgpe_group_autotune(fgpe_group* group, int reps, double* host_ms);
+/* nsteps heavy-ball iterations of every instance (same scheme and stopping
+   test as fgpe_imag_steps, per instance), enqueued without a host
+   synchronisation; a converged instance does nothing in later iterations.
+   Continues a previous group run unless a state or Hamiltonian was set. */
+int fgpe_group_imag_steps(fgpe_group* group, double dt, double beta, const double* host_npart, double eps,
+                          int nsteps);
+/* Iterations each instance has run since the run started or was continued
+   (iters, n ints or NULL) and whether it has converged (converged, n ints or
+   NULL); synchronises, leaves the run state alone. */
+int fgpe_group_imag_status(fgpe_group* group, int* iters, int* converged);
+/* Ends a run: applies the pending renormalisation to every state, then
+   reports as fgpe_group_imag_status. */
+int fgpe_group_imag_finish(fgpe_group* group, int* iters, int* converged);
+
+/* ---- captured launch sequences -------------------------------------------
+   fgpe_graph_capture_imag prepares the groups (as fgpe_group_imag_steps does
+   on its first call), then captures `iters` heavy-ball iterations of all of
+   them as one hipGraph on their common stream (not the null stream).  Each
+   fgpe_graph_launch replays those iterations; the host polls with
+   fgpe_group_imag_status between launches and ends the run with
+   fgpe_group_imag_finish.  The graph keeps its own copy of the argument
+   tables; destroy it before its groups' contexts. */
+typedef struct fgpe_graph fgpe_graph;
+int fgpe_graph_capture_imag(fgpe_group* const* groups, int ngroups, double dt, double beta,
+                            const double* host_npart, double eps, int iters, fgpe_graph** out);
+int fgpe_graph_launch(fgpe_graph* graph);
+int fgpe_graph_destroy(fgpe_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif
