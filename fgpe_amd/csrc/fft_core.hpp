@@ -214,8 +214,25 @@ template <int DIR, typename T2> __device__ __forceinline__ T2 twid(const T2* __r
     return w;
 }
 
-// Stage-to-stage exchange through LDS: write / barrier / read / barrier.
-template <typename T2, int N, int V, int DIR, int S>
+// Ordering of the stage-to-stage exchange.  A pencil owned by consecutive
+// lanes of one wave (x pencils: T <= 64 threads, t = lane % T) needs only
+// wavefront-scope ordering: a wave's LDS operations are performed in program
+// order, so the fences below just keep the compiler from moving the reads
+// above the writes (and the next writes above the reads).  Pencils spread
+// over several waves (the strided tiles) need the block barrier.
+template <bool WAVE> __device__ __forceinline__ void exchange_sync()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// Stage-to-stage exchange through LDS: write / sync / read / sync.
+template <typename T2, int N, int V, int DIR, int S, bool WAVE>
 __device__ __forceinline__ void exchange(T2* v, int t, T2* lds)
 {
     constexpr int TT = Plan<N, V>::T, E = N / TT;
@@ -232,17 +249,17 @@ __device__ __forceinline__ void exchange(T2* v, int t, T2* lds)
 #pragma unroll
         for (int i = 0; i < r; ++i) lds[lds_phys(base + i * Ns)] = v[m * r + i];
     }
-    __syncthreads();
+    exchange_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < M2; ++m) {
         const int j = t + TT * m;
 #pragma unroll
         for (int i = 0; i < r2; ++i) v[m * r2 + i] = lds[lds_phys(j + i * (N / r2))];
     }
-    __syncthreads();
+    exchange_sync<WAVE>();
 }
 
-template <typename T2, int N, int V, int DIR, int S>
+template <typename T2, int N, int V, int DIR, int S, bool WAVE>
 __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
     constexpr int L = Plan<N, V>::L, TT = Plan<N, V>::T, E = N / TT;
@@ -262,23 +279,26 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __re
         Dft<r, DIR>::run(v + m * r);
     }
     if constexpr (S + 1 < L) {
-        exchange<T2, N, V, DIR, S>(v, t, lds);
-        fft_stages<T2, N, V, DIR, S + 1>(v, t, lds, tw);
+        exchange<T2, N, V, DIR, S, WAVE>(v, t, lds);
+        fft_stages<T2, N, V, DIR, S + 1, WAVE>(v, t, lds, tw);
     }
 }
 
 // Forward: v in layout A -> layout B.  Inverse: layout B -> layout A.
-// lds: this pencil's lds_stride<N>() elements.  Every thread of the block
-// must call these (they contain barriers).
-template <typename T2, int N, int V>
+// lds: this pencil's lds_stride<N>() elements.  WAVE: the pencil's T threads
+// are consecutive lanes of one wave (then only that wave must call);
+// otherwise every thread of the block must call (block barriers).
+template <typename T2, int N, int V, bool WAVE = false>
 __device__ __forceinline__ void fft_forward(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
-    fft_stages<T2, N, V, -1, 0>(v, t, lds, tw);
+    static_assert(!WAVE || (Plan<N, V>::T <= 64 && 64 % Plan<N, V>::T == 0), "pencil must fit in a wave");
+    fft_stages<T2, N, V, -1, 0, WAVE>(v, t, lds, tw);
 }
-template <typename T2, int N, int V>
+template <typename T2, int N, int V, bool WAVE = false>
 __device__ __forceinline__ void fft_inverse(T2* v, int t, T2* lds, const T2* __restrict__ tw)
 {
-    fft_stages<T2, N, V, +1, 0>(v, t, lds, tw);
+    static_assert(!WAVE || (Plan<N, V>::T <= 64 && 64 % Plan<N, V>::T == 0), "pencil must fit in a wave");
+    fft_stages<T2, N, V, +1, 0, WAVE>(v, t, lds, tw);
 }
 
 }  // namespace fgpe

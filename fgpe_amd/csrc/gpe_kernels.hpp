@@ -86,13 +86,13 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
     if (DIR < 0) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = src[elemA<N, V>(t, q)];
-        fft_forward<T2, N, V>(v, t, my, tw);
+        fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) dst[elemB<N, V>(t, q)] = v[q];
     } else {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = src[elemB<N, V>(t, q)];
-        fft_inverse<T2, N, V>(v, t, my, tw);
+        fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) dst[elemA<N, V>(t, q)] = v[q];
     }
@@ -260,6 +260,97 @@ __device__ __forceinline__ double vdd_of(double c_dd, double num, double k2)
     return c_dd * (((num / den) * (num / den)) - 1.0 / 3.0);
 }
 
+// The k-space multiply of the FP64 form.
+template <int N, int V>
+__device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastArgs& a, const TileAt& at)
+{
+    constexpr int E = Plan<N, V>::E;
+    const double kx = a.kX[at.ix], ko = a.kO[at.outer];
+    const double kxo2 = kx * kx + ko * ko;
+    if (at.ch < a.s.ncomp) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const double ka = a.kA[elemB<N, V>(t, q)];
+            const double k2 = kxo2 + ka * ka;
+            v[q] = double2{(a.lap_scale * v[q].x) * k2, (a.lap_scale * v[q].y) * k2};
+        }
+        return;
+    }
+    const double* wt = (const double*)a.wtab;
+    const int nxf = a.s.nx, nO = a.s.nouter;
+    const double kxp = a.kX[(nxf - at.ix) % nxf], kop = a.kO[(nO - (int)at.outer) % nO];
+    const double num0 = kx * a.dX + ko * a.dO;
+    const double num0p = kxp * a.dX + kop * a.dO;
+    const double kxo2p = kxp * kxp + kop * kop;
+    const bool xo_neg = (kxp == -kx) && (kop == -ko);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int ea = elemB<N, V>(t, q);
+        const double ka = a.kA[ea];
+        double vt = 0.0;
+        if (a.c_dd != 0.0) {
+            vt = vdd_of(a.c_dd, num0 + ka * a.dA, kxo2 + ka * ka);
+            const double kap = a.kA[(N - ea) % N];
+            if (!(xo_neg && kap == -ka))
+                vt = 0.5 * (vt + vdd_of(a.c_dd, num0p + kap * a.dA, kxo2p + kap * kap));
+        }
+        if (wt && at.valid) vt = vt + wt[at.sp_off + (long long)ea * a.s.stride];
+        const double f = vt * a.inv_ntot;
+        v[q] = double2{v[q].x * f, v[q].y * f};
+    }
+}
+
+// The k-space multiply of the mixed-precision form: factors in FP32 (the
+// data is FP32), same formulas and the same Nyquist symmetrisation as the
+// FP64 form.
+__device__ __forceinline__ float vdd_of_f32(float c_dd, float num, float k2)
+{
+    if (sqrtf(k2) < 1E-6f) return 0.f;
+    return c_dd * ((num * num) / k2 - (1.f / 3.f));
+}
+
+template <int N, int V>
+__device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastArgs& a, const TileAt& at)
+{
+    constexpr int E = Plan<N, V>::E;
+    const float kx = (float)a.kX[at.ix], ko = (float)a.kO[at.outer];
+    const float kxo2 = kx * kx + ko * ko;
+    if (at.ch < a.s.ncomp) {
+        const float ls = (float)a.lap_scale;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const float ka = (float)a.kA[elemB<N, V>(t, q)];
+            const float f = ls * (kxo2 + ka * ka);
+            v[q] = float2{v[q].x * f, v[q].y * f};
+        }
+        return;
+    }
+    const float* wt = (const float*)a.wtab;
+    const int nxf = a.s.nx, nO = a.s.nouter;
+    const double kxpd = a.kX[(nxf - at.ix) % nxf], kopd = a.kO[(nO - (int)at.outer) % nO];
+    const bool xo_neg = (kxpd == -a.kX[at.ix]) && (kopd == -a.kO[at.outer]);
+    const float kxp = (float)kxpd, kop = (float)kopd;
+    const float dA = (float)a.dA, dX = (float)a.dX, dO = (float)a.dO, c_dd = (float)a.c_dd;
+    const float num0 = kx * dX + ko * dO, num0p = kxp * dX + kop * dO;
+    const float kxo2p = kxp * kxp + kop * kop;
+    const float inv = (float)a.inv_ntot;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int ea = elemB<N, V>(t, q);
+        float vt = 0.f;
+        if (c_dd != 0.f) {
+            const double kad = a.kA[ea], kapd = a.kA[(N - ea) % N];
+            const float ka = (float)kad, kap = (float)kapd;
+            vt = vdd_of_f32(c_dd, num0 + ka * dA, kxo2 + ka * ka);
+            if (!(xo_neg && kapd == -kad))
+                vt = 0.5f * (vt + vdd_of_f32(c_dd, num0p + kap * dA, kxo2p + kap * kap));
+        }
+        if (wt && at.valid) vt = vt + wt[at.sp_off + (long long)ea * a.s.stride];
+        const float f = vt * inv;
+        v[q] = float2{v[q].x * f, v[q].y * f};
+    }
+}
+
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
 k_last(const c2_t<T>* __restrict__ tw, LastArgs a0, const LastArgs* __restrict__ tab)
@@ -286,49 +377,8 @@ k_last(const c2_t<T>* __restrict__ tw, LastArgs a0, const LastArgs* __restrict__
     }
     fft_forward<T2, N, V>(v, t, my, tw);
 
-    const double kx = a.kX[at.ix], ko = a.kO[at.outer];
-    const double kxo2 = kx * kx + ko * ko;
-    if (at.ch < a.s.ncomp) {
-#pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const double ka = a.kA[elemB<N, V>(t, q)];
-            const double k2 = kxo2 + ka * ka;
-            if (sizeof(T) == 8) {
-                v[q] = T2{(T)((a.lap_scale * (double)v[q].x) * k2), (T)((a.lap_scale * (double)v[q].y) * k2)};
-            } else {
-                const T f = (T)(a.lap_scale * k2);
-                v[q] = T2{v[q].x * f, v[q].y * f};
-            }
-        }
-    } else {
-        const T* wt = (const T*)a.wtab;
-        const int nxf = a.s.nx, nO = a.s.nouter;
-        const double kxp = a.kX[(nxf - at.ix) % nxf], kop = a.kO[(nO - (int)at.outer) % nO];
-        const double num0 = kx * a.dX + ko * a.dO;
-        const double num0p = kxp * a.dX + kop * a.dO;
-        const double kxo2p = kxp * kxp + kop * kop;
-        const bool xo_neg = (kxp == -kx) && (kop == -ko);
-#pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int ea = elemB<N, V>(t, q);
-            const double ka = a.kA[ea];
-            double vt = 0.0;
-            if (a.c_dd != 0.0) {
-                vt = vdd_of(a.c_dd, num0 + ka * a.dA, kxo2 + ka * ka);
-                const double kap = a.kA[(N - ea) % N];
-                if (!(xo_neg && kap == -ka))
-                    vt = 0.5 * (vt + vdd_of(a.c_dd, num0p + kap * a.dA, kxo2p + kap * kap));
-            }
-            if (wt && at.valid) vt = vt + (double)wt[at.sp_off + (long long)ea * a.s.stride];
-            const double f = vt * a.inv_ntot;
-            if (sizeof(T) == 8) {
-                v[q] = T2{(T)((double)v[q].x * f), (T)((double)v[q].y * f)};
-            } else {
-                const T ff = (T)f;
-                v[q] = T2{v[q].x * ff, v[q].y * ff};
-            }
-        }
-    }
+    if constexpr (sizeof(T) == 4) last_multiply_f32<N, V>(v, t, a, at);
+    else last_multiply_f64<N, V>(v, t, a, at);
     fft_inverse<T2, N, V>(v, t, my, tw);
     if (at.valid)
 #pragma unroll
@@ -410,7 +460,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                 v[q] = from_d2<T>(a.P0[c][row0 + elemA<N, V>(t, q)]);
                 if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
             }
-            fft_forward<T2, N, V>(v, t, my, tw);
+            fft_forward<T2, N, V, true>(v, t, my, tw);
             T2* sc = a.spec + c * a.ntot + row0;
 #pragma unroll
             for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
@@ -429,7 +479,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     v[q] = T2{w.x, -w.y};
                 }
             }
-            fft_inverse<T2, N, V>(v, t, my, tw);
+            fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
             for (int q = 0; q < E; ++q) phi[q] = v[q].x;
         }
@@ -444,96 +494,165 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
             T2* sc = a.spec + c * a.ntot + row0;
 #pragma unroll
             for (int q = 0; q < E; ++q) v[q] = sc[elemB<N, V>(t, q)];
-            fft_inverse<T2, N, V>(v, t, my, tw);   // v = KE psi_c (layout A)
+            fft_inverse<T2, N, V, true>(v, t, my, tw);   // v = KE psi_c (layout A)
             const double sig = a.scale ? a.scale[c] : 1.0;
             const double sig1 = (NC == 2 && a.scale) ? a.scale[1] : 1.0;
             const double g4s = a.g4[c][c], g2s = a.g2[c][c];
             const double g4o = a.g4[c][1 - c], g2o = a.g2[c][1 - c];
-#pragma unroll
-            for (int q = 0; q < E; ++q) {
-                const long long g = row0 + elemA<N, V>(t, q);
-                double2 po = double2{0.0, 0.0};
-                double2 ps;
-                if (from_p0) {
-                    const double2 r = a.P0[c][g];
-                    po = double2{sig * r.x, sig * r.y};
-                    ps = po;
-                } else {
-                    ps = to_d2<T>(a.PS[c][g]);
-                }
-                const double n = ps.x * ps.x + ps.y * ps.y;
-                double no = 0.0;
-                if (NC == 2) {
-                    if (c == 0) {
-                        double2 o;
-                        if (from_p0) {
-                            const double2 r = a.P0[1][g];
-                            o = double2{sig1 * r.x, sig1 * r.y};
-                        } else {
-                            o = to_d2<T>(a.PS[1][g]);
-                        }
-                        no = o.x * o.x + o.y * o.y;
-                        nsave[q] = n;
-                    } else {
-                        no = nsave[q];
-                    }
-                }
-                const double vr = (double)a.Vr[c][g];
-                const double vi = a.Vi[c] ? (double)a.Vi[c][g] : 0.0;
-                const double2 ke = to_d2<T>(v[q]);
-                double rX = (vr + g4s * n);
-                if (NC == 2) rX = rX + g4o * no;
-                if (DENS) rX += (double)phi[q];
-                const double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
-                double2 he = double2{0.0, 0.0};
-                if (MODE == MODE_APPLY || MODE == MODE_HB) {
-                    double eX = (vr + g2s * n);
-                    if (NC == 2) eX = eX + g2o * no;
-                    if (DENS) eX += 0.5 * (double)phi[q];
-                    he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
-                }
-                if (MODE == MODE_APPLY) {
-                    if (a.hmu) a.hmu[c * a.ntot + g] = h;
-                    if (a.hen) a.hen[c * a.ntot + g] = he;
-                    sums[3 * c + 0] += n;
-                    sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
-                    sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
-                } else if (MODE == MODE_HB) {
-                    const double2 old = to_d2<T>(a.PS[c][g]);
-                    const double m = a.mu ? a.mu[c] : 0.0;
-                    const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
-                                               (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
-                    a.PS[c][g] = from_d2<T>(ps);
-                    a.P0[c][g] = nw;
-                    sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
-                    sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
-                    sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
-                    v[q] = from_d2<T>(nw);
-                    if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
-                } else {
-                    if (a.stage != 1) {
+            if constexpr (!(MODE == MODE_RK && sizeof(T) == 4)) {
+    #pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    const long long g = row0 + elemA<N, V>(t, q);
+                    double2 po = double2{0.0, 0.0};
+                    double2 ps;
+                    if (from_p0) {
                         const double2 r = a.P0[c][g];
                         po = double2{sig * r.x, sig * r.y};
-                    }
-                    if (a.stage < 4) {
-                        // UpdateRKStep: psi = old + c*h ; k = h (stage 1) or k + 2h
-                        const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
-                                                   po.y + (a.cr * h.y + a.ci * h.x)};
-                        if (a.stage == 1) {
-                            a.K[c][g] = from_d2<T>(h);
-                        } else {
-                            const double2 k = to_d2<T>(a.K[c][g]);
-                            a.K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
-                        }
-                        const T2 st = from_d2<T>(nw);
-                        (a.PSo[c] ? a.PSo[c] : a.PS[c])[g] = st;
-                        v[q] = st;
+                        ps = po;
                     } else {
-                        // FinalRKStep: psi = old + c*(k + h)/6
-                        const double2 k = to_d2<T>(a.K[c][g]);
-                        const double aX = k.x + h.x, aY = k.y + h.y;
-                        const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
-                                                   po.y + (a.cr * aY + a.ci * aX) / 6.0};
+                        ps = to_d2<T>(a.PS[c][g]);
+                    }
+                    const double n = ps.x * ps.x + ps.y * ps.y;
+                    double no = 0.0;
+                    if (NC == 2) {
+                        if (c == 0) {
+                            double2 o;
+                            if (from_p0) {
+                                const double2 r = a.P0[1][g];
+                                o = double2{sig1 * r.x, sig1 * r.y};
+                            } else {
+                                o = to_d2<T>(a.PS[1][g]);
+                            }
+                            no = o.x * o.x + o.y * o.y;
+                            nsave[q] = n;
+                        } else {
+                            no = nsave[q];
+                        }
+                    }
+                    const double vr = (double)a.Vr[c][g];
+                    const double vi = a.Vi[c] ? (double)a.Vi[c][g] : 0.0;
+                    const double2 ke = to_d2<T>(v[q]);
+                    double rX = (vr + g4s * n);
+                    if (NC == 2) rX = rX + g4o * no;
+                    if (DENS) rX += (double)phi[q];
+                    const double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                    double2 he = double2{0.0, 0.0};
+                    if (MODE == MODE_APPLY || MODE == MODE_HB) {
+                        double eX = (vr + g2s * n);
+                        if (NC == 2) eX = eX + g2o * no;
+                        if (DENS) eX += 0.5 * (double)phi[q];
+                        he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
+                    }
+                    if (MODE == MODE_APPLY) {
+                        if (a.hmu) a.hmu[c * a.ntot + g] = h;
+                        if (a.hen) a.hen[c * a.ntot + g] = he;
+                        sums[3 * c + 0] += n;
+                        sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
+                        sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
+                    } else if (MODE == MODE_HB) {
+                        const double2 old = to_d2<T>(a.PS[c][g]);
+                        const double m = a.mu ? a.mu[c] : 0.0;
+                        const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
+                                                   (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
+                        a.PS[c][g] = from_d2<T>(ps);
+                        a.P0[c][g] = nw;
+                        sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
+                        sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
+                        sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
+                        v[q] = from_d2<T>(nw);
+                        if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
+                    } else {
+                        if (a.stage != 1) {
+                            const double2 r = a.P0[c][g];
+                            po = double2{sig * r.x, sig * r.y};
+                        }
+                        if (a.stage < 4) {
+                            // UpdateRKStep: psi = old + c*h ; k = h (stage 1) or k + 2h
+                            const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
+                                                       po.y + (a.cr * h.y + a.ci * h.x)};
+                            if (a.stage == 1) {
+                                a.K[c][g] = from_d2<T>(h);
+                            } else {
+                                const double2 k = to_d2<T>(a.K[c][g]);
+                                a.K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
+                            }
+                            const T2 st = from_d2<T>(nw);
+                            (a.PSo[c] ? a.PSo[c] : a.PS[c])[g] = st;
+                            v[q] = st;
+                        } else {
+                            // FinalRKStep: psi = old + c*(k + h)/6
+                            const double2 k = to_d2<T>(a.K[c][g]);
+                            const double aX = k.x + h.x, aY = k.y + h.y;
+                            const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
+                                                       po.y + (a.cr * aY + a.ci * aX) / 6.0};
+                            a.P0[c][g] = nw;
+                            v[q] = from_d2<T>(nw);
+                            if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
+                        }
+                        if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
+                    }
+                }
+            } else {
+                // Mixed precision RK stage: H, the stage state and the K
+                // accumulator in FP32; only stage 4 adds its FP32 increment
+                // c (K + h) / 6 to the FP64 state.
+                const float f4s = (float)g4s, f4o = (float)g4o;
+                const float fcr = (float)a.cr, fci = (float)a.ci;
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    const long long g = row0 + elemA<N, V>(t, q);
+                    float2 ps;
+                    if (a.stage == 1) {
+                        const double2 r = a.P0[c][g];
+                        ps = float2{(float)(sig * r.x), (float)(sig * r.y)};
+                    } else {
+                        ps = a.PS[c][g];
+                    }
+                    const float n = ps.x * ps.x + ps.y * ps.y;
+                    float no = 0.f;
+                    if (NC == 2) {
+                        if (c == 0) {
+                            float2 o;
+                            if (a.stage == 1) {
+                                const double2 r = a.P0[1][g];
+                                o = float2{(float)(sig1 * r.x), (float)(sig1 * r.y)};
+                            } else {
+                                o = a.PS[1][g];
+                            }
+                            no = o.x * o.x + o.y * o.y;
+                            nsave[q] = n;
+                        } else {
+                            no = (float)nsave[q];
+                        }
+                    }
+                    const float vr = (float)a.Vr[c][g];
+                    const float vi = a.Vi[c] ? (float)a.Vi[c][g] : 0.f;
+                    const float2 ke = v[q];
+                    float rX = vr + f4s * n;
+                    if (NC == 2) rX = rX + f4o * no;
+                    if (DENS) rX += (float)phi[q];
+                    const float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                    if (a.stage < 4) {
+                        float2 po = ps;
+                        if (a.stage != 1) {
+                            const double2 r = a.P0[c][g];
+                            po = float2{(float)(sig * r.x), (float)(sig * r.y)};
+                        }
+                        const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
+                        if (a.stage == 1) {
+                            a.K[c][g] = h;
+                        } else {
+                            const float2 k = a.K[c][g];
+                            a.K[c][g] = float2{k.x + 2.f * h.x, k.y + 2.f * h.y};
+                        }
+                        (a.PSo[c] ? a.PSo[c] : a.PS[c])[g] = nw;
+                        v[q] = nw;
+                    } else {
+                        const float2 k = a.K[c][g];
+                        const float ax = k.x + h.x, ay = k.y + h.y;
+                        const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
+                        const double2 r = a.P0[c][g];
+                        const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
                         a.P0[c][g] = nw;
                         v[q] = from_d2<T>(nw);
                         if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
@@ -542,7 +661,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                 }
             }
             if (MODE != MODE_APPLY) {
-                fft_forward<T2, N, V>(v, t, my, tw);
+                fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
                 for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
             }
@@ -554,7 +673,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
     if (DENS) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{nd[q], (T)0};
-        fft_forward<T2, N, V>(v, t, my, tw);
+        fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const int e = elemB<N, V>(t, q);
@@ -625,18 +744,18 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
         rr[q] = yx * ab.x + yy * ab.y;
         v[q] = T2{(T)ab.x, (T)ab.y};
     }
-    fft_forward<T2, N, V>(v, t, my, tw);
+    fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
     for (int q = 0; q < E; ++q) sp[row0 + elemB<N, V>(t, q)] = v[q];
     if (DENS) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)nn[q], (T)0};
-        fft_forward<T2, N, V>(v, t, my, tw);
+        fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) sp[a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)rr[q], (T)0};
-        fft_forward<T2, N, V>(v, t, my, tw);
+        fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) sp[2 * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
     }
@@ -661,18 +780,18 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
     if (DENS) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = sp[a.ntot + row0 + elemB<N, V>(t, q)];
-        fft_inverse<T2, N, V>(v, t, my, tw);
+        fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phin[q] = (double)v[q].x;
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = sp[2 * a.ntot + row0 + elemB<N, V>(t, q)];
-        fft_inverse<T2, N, V>(v, t, my, tw);
+        fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phir[q] = (double)v[q].x;
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = sp[row0 + elemB<N, V>(t, q)];
-    fft_inverse<T2, N, V>(v, t, my, tw);   // T abar
+    fft_inverse<T2, N, V, true>(v, t, my, tw);   // T abar
     double s[2] = {0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < E; ++q) {
