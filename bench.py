@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -23,6 +24,22 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+
+PMC_SUMMARY = os.path.join("profiles", "rk4_dipolar_256cubed_pmc_traffic.md")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    from the committed rocprofv3 summary of this workload, or None."""
+    path = os.path.join(ROOT, PMC_SUMMARY)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch (.+) = (\d+)$", line.strip())
+        if m and m.group(1) == kernel:
+            return int(m.group(2))
+    return None
 
 
 def run_form(torch, w, psi, V, precision, steps, warmup):
@@ -198,6 +215,10 @@ def main():
     else:
         per_pt = roofline.pass_bytes(False, 1, True, nx)
     achieved = per_pt * N / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    if dom == "xstage" and (nx, ny, nz) == (256, 256, 256):
+        v = f64["tune"]["variant"]["xstage"]
+        traffic = pmc_traffic(f"fgpe::k_xstage<double, 256, {v}, 1, true, 1>")
     step_gbs = roofline.step_bytes_per_point(False, 1, True, nx) * N * steps_s / 1e9
     mixed_steps = 1000.0 / mixed["ms"]
     l2 = float(np.sqrt(np.sum(np.abs(mixed["psi"] - f64["psi"]) ** 2) / np.sum(np.abs(f64["psi"]) ** 2)))
@@ -231,7 +252,9 @@ def main():
         "kernels_mixed_ms_avg": {k: round(v[0] / v[1], 4) for k, v in mixed["kt"].items() if v[1] > 0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": PMC_SUMMARY if traffic else None,
+                     "algorithmic_bytes_per_launch": round(per_pt * N),
                      "bytes_per_point": per_pt, "avg_launch_ms": round(avg_ms, 4)},
         "cpu_baseline": None,
     }
