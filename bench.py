@@ -93,6 +93,116 @@ def cpu_baseline(w, psi, V, nsteps, threads):
     return nsteps / dt, orc.num_threads()
 
 
+def run_adjoint(torch, args):
+    """Gradient of the final-state fidelity |<target|psi(T)>|^2 with respect to
+    the trap V, a and a_dd through the discrete adjoint, for the dipolar GPE
+    (omega = (1,1,0.25), g = 200, eps_dd = 1.3, box 16 x 16 x 32) on 128^3 over
+    T RK4 steps of dt = 5e-4 with a checkpoint every 45 steps; checked by
+    central differences (two smooth V directions, a and a_dd)."""
+    from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, Context, configs
+    torch.cuda.set_device(0)
+    n, dt, every = args.adjoint_steps, 5e-4, 45
+    grid = configs.GridSpec(128, 128, 128, 16.0, 16.0, 32.0)
+    g = 200.0
+    a, a_dd = g / (4 * configs.PI), 1.3 * g / (4 * configs.PI)
+    omega = (1.0, 1.0, 0.25)
+    V = configs.harmonic_trap(grid, omega).real
+    psi0 = configs.thomas_fermi_noisy(grid, 4, g, omega)
+    target = configs.thomas_fermi_noisy(grid, 5, g, (1.0, 1.0, 0.3))
+    kx, ky, kz = grid.kaxes()
+    dv = grid.dv
+
+    def smooth(seed):
+        u = configs.splitmix64_uniform(seed, grid.npoints).reshape(grid.shape)
+        F = np.fft.fftn(u)
+        KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+        f = np.fft.ifftn(F * np.exp(-4.0 * (KX ** 2 + KY ** 2 + KZ ** 2))).real
+        return f / np.abs(f).max()
+
+    def make(prec, Vx, ax, addx):
+        c = Context(128, 128, 128, prec)
+        c.set_kspace(kx, ky, kz, dv)
+        c.set_hamiltonian(a=(ax,), a_dd=addx, dip=(0.0, 0.0, 1.0))
+        c.upload_vext(Vx.astype(np.complex128))
+        c.upload_psi(psi0)
+        return c
+
+    def J(psiT):
+        ov = np.sum(np.conj(target) * psiT) * dv
+        return float(abs(ov) ** 2), ov
+
+    out = {}
+    for name, prec in (("f64", FGPE_PREC_F64), ("mixed", FGPE_PREC_MIXED)):
+        times = []
+        for rep in range(2):
+            c = make(prec, V, a, a_dd)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            c.adjoint_forward(dt, n, every)
+            psiT = c.download_psi()
+            Jv, ov = J(psiT)
+            lamT = torch.from_numpy(2 * ov * dv * target).cuda()
+            gV = torch.empty(grid.shape, dtype=torch.float64, device="cuda")
+            lam0 = torch.empty_like(lamT)
+            torch.cuda.synchronize()
+            gs = c.adjoint_backward(lamT.data_ptr(), lam0.data_ptr(), gV.data_ptr())
+            c.sync()
+            times.append(time.perf_counter() - t0)
+            c.close()
+        gVn = gV.cpu().numpy()
+        print(f"# adjoint {name}: J = {Jv:.10f}, {times[-1]:.2f} s per gradient", file=sys.stderr, flush=True)
+        if name == "mixed":
+            # against the FP64 adjoint: finite differences of an FP32-bulk forward are dominated by its
+            # rounding noise at these step sizes
+            ref = out["f64"]
+            cmp = [("V", float(np.sum(gVn * smooth(4 + j))), ref["directional"][j]) for j in range(2)]
+            cmp += [("a", gs[0], ref["directional"][2]), ("a_dd", gs[1], ref["directional"][3])]
+            checks = [{"dir": lab, "mixed": m, "f64": r, "rel": abs(m - r) / abs(r)} for lab, m, r in cmp]
+            for x in checks:
+                print(f"#   {x['dir']}: mixed {x['mixed']:.10e} f64 {x['f64']:.10e} rel {x['rel']:.2e}",
+                      file=sys.stderr, flush=True)
+            out[name] = {"grads_per_s": round(1.0 / times[-1], 4), "s_per_gradient": round(times[-1], 3),
+                         "J": Jv, "vs_f64_adjoint": checks, "max_rel_vs_f64": max(x["rel"] for x in checks)}
+            continue
+        # central differences of the same form's forward
+        checks = []
+        h = 1e-4
+        dirs = [("V", smooth(4 + j), 0.0, 0.0) for j in range(2)] + [("a", None, h * a, 0.0),
+                                                                      ("a_dd", None, 0.0, h * a_dd)]
+        for lab, dVx, pa, pd in dirs:
+            Js = []
+            for sgn in (1.0, -1.0):
+                Vx = V + sgn * h * dVx if dVx is not None else V
+                c = make(prec, Vx, a + sgn * pa, a_dd + sgn * pd)
+                c.rk4_steps(dt, n)
+                Js.append(J(c.download_psi())[0])
+                c.close()
+            if dVx is not None:
+                fd, ad = (Js[0] - Js[1]) / (2 * h), float(np.sum(gVn * dVx))
+            elif lab == "a":
+                fd, ad = (Js[0] - Js[1]) / (2 * pa), gs[0]
+            else:
+                fd, ad = (Js[0] - Js[1]) / (2 * pd), gs[1]
+            checks.append({"dir": lab, "adjoint": ad, "fd": fd, "rel": abs(ad - fd) / abs(fd)})
+            print(f"#   {lab}: adjoint {ad:.10e} fd {fd:.10e} rel {abs(ad - fd) / abs(fd):.2e}",
+                  file=sys.stderr, flush=True)
+        out[name] = {"grads_per_s": round(1.0 / times[-1], 4), "s_per_gradient": round(times[-1], 3), "J": Jv,
+                     "fd_checks": checks, "max_rel_vs_fd": max(x["rel"] for x in checks),
+                     "directional": [x["adjoint"] for x in checks]}
+    line = {
+        "metric": "adjoint gradients/s (fidelity wrt V, a, a_dd; 128^3 dipolar, 2000 RK4 steps, checkpoint 45)",
+        "value": out["f64"]["grads_per_s"], "unit": "gradients/s", "n_gpus": 1, "steps": n, "warmup": 1,
+        "ms_per_step": round(1e3 * out["f64"]["s_per_gradient"] / n, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Thomas-Fermi psi0 and target, harmonic trap)",
+        "config": {"workload": f"adjoint 128x128x128 dipolar, T = {n} steps", "dt": dt, "ckpt_every": every,
+                   "box": [16.0, 16.0, 32.0], "g": g, "eps_dd": 1.3, "omega": list(omega)},
+        "f64": out["f64"], "mixed": out["mixed"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_sweep(torch, args):
     """Batched imaginary-time ground-state sweep: every instance to
     |dE/E| < 1e-9 (heavy ball, dt = 2e-3, beta = 0.9), mixed precision,
@@ -163,6 +273,10 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="run the batched ground-state sweep (64 mixed-shape dipolar / soft-core "
                          "instances, one hipGraph) instead of the 256^3 RK4 benchmark")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time adjoint gradients (128^3 dipolar, 2000 RK4 steps, checkpoint every 45) "
+                         "in both forms, with central finite-difference checks")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--sweep-n", type=int, default=64)
     ap.add_argument("--sweep-max-iters", type=int, default=20000)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -187,6 +301,8 @@ def main():
     from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, configs, roofline
     if args.sweep:
         return run_sweep(torch, args)
+    if args.adjoint:
+        return run_adjoint(torch, args)
 
     nx, ny, nz = (int(v) for v in args.grid.split(","))
     w = configs.dipolar_real(nx, ny, nz, nsteps=args.steps)

@@ -25,6 +25,19 @@
 
 namespace fgpe {
 
+// Device arrays through global-address-space pointers: a pointer read from
+// an argument table in memory is otherwise generic, and generic (flat)
+// accesses also count against the LDS wait counter, which would make every
+// LDS exchange wait for outstanding HBM traffic.
+// (The host pass of hipcc only parses kernel bodies; there the alias is a
+// plain pointer, since address space 1 means something else on the host.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T> using gp = __attribute__((address_space(1))) T*;
+#else
+template <typename T> using gp = T*;
+#endif
+template <typename T> __device__ __forceinline__ gp<T> glob(T* p) { return (gp<T>)p; }
+
 template <typename T> struct c2;
 template <> struct c2<double> { using type = double2; };
 template <> struct c2<float> { using type = float2; };

@@ -19,13 +19,15 @@ namespace fgpe {
 __global__ void __launch_bounds__(256) k_finalize(FinArgs f0, const FinArgs* __restrict__ tab)
 {
     __shared__ double sh[6][256];
-    const FinArgs f = tab ? tab[blockIdx.x] : f0;
-    if (f.stop && *f.stop) return;
+    const FinArgs f = args_of(f0, tab, blockIdx.x);
+    if (f.stop && *glob(f.stop)) return;
+    const gp<const double> partials = glob(f.partials);
+    const gp<double> out = glob(f.out), scale = glob(f.scale), mu_out = glob(f.mu), eprev = glob(f.eprev);
     const int t = threadIdx.x;
     const int nv = 3 * f.nc;
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int b = t; b < f.nblocks; b += 256)
-        for (int q = 0; q < nv; ++q) s[q] += f.partials[(long long)b * nv + q];
+        for (int q = 0; q < nv; ++q) s[q] += partials[(long long)b * nv + q];
     for (int q = 0; q < 6; ++q) sh[q][t] = s[q];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -38,31 +40,31 @@ __global__ void __launch_bounds__(256) k_finalize(FinArgs f0, const FinArgs* __r
     for (int c = 0; c < f.nc; ++c) {
         const double norm = sh[3 * c][0] * f.dv;
         if (f.kind == FIN_OBS) {
-            f.out[3 * c] = norm;
-            f.out[3 * c + 1] = sh[3 * c + 1][0] * f.dv / norm;
-            f.out[3 * c + 2] = sh[3 * c + 2][0] * f.dv / norm;
+            out[3 * c] = norm;
+            out[3 * c + 1] = sh[3 * c + 1][0] * f.dv / norm;
+            out[3 * c + 2] = sh[3 * c + 2][0] * f.dv / norm;
         } else if (f.kind == FIN_RENORM) {
-            f.out[3 * c] = norm;
-            f.scale[c] = sqrt(f.npart[c] / norm);
+            out[3 * c] = norm;
+            scale[c] = sqrt(f.npart[c] / norm);
         } else {
             const double sc = sqrt(f.npart[c] / norm);
             const double narg = f.nc == 1 ? f.npart[c] : norm;
             const double mu = sc * sh[3 * c + 1][0] * f.dv / narg;
             const double en = sc * sh[3 * c + 2][0] * f.dv / narg;
-            f.out[3 * c] = norm;
-            f.out[3 * c + 1] = mu;
-            f.out[3 * c + 2] = en;
-            f.scale[c] = sc;
-            f.mu[c] = mu;
-            const double ep = f.eprev[c];
+            out[3 * c] = norm;
+            out[3 * c + 1] = mu;
+            out[3 * c + 2] = en;
+            scale[c] = sc;
+            mu_out[c] = mu;
+            const double ep = eprev[c];
             if (f.nc == 1) conv = conv && (fabs(en - ep) / ep < f.eps);
             else conv = conv && (fabs(en - ep) < f.eps);
-            f.eprev[c] = en;
+            eprev[c] = en;
         }
     }
     if (f.kind == FIN_HB) {
-        f.done[0] += 1;
-        if (conv) f.stop[0] = 1;
+        glob(f.done)[0] += 1;
+        if (conv) glob(f.stop)[0] = 1;
     }
 }
 
@@ -429,7 +431,7 @@ __global__ void __launch_bounds__(256) k_weighted_sum(const double2* __restrict_
         const double wt = kind == 0 ? w[ix] : cos(2 * kPi * w[ix] / L);
         s[0] += wt * (a[i].x * a[i].x + a[i].y * a[i].y);
     }
-    block_sum_store<1>(s, partials + blockIdx.x);
+    block_sum_store<1>(s, glob(partials) + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partials, int nb, double scale,

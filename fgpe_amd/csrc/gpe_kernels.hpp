@@ -39,13 +39,34 @@ template <typename T, int N, int V = 0> __host__ __device__ constexpr int ps_of(
     return V == 0 ? base : 2 * base;
 }
 
+// The arguments of this block: the by-value kernel argument, or this
+// instance's entry of a device table (batched launches, blockIdx.y), read
+// through the constant address space so that the loads are scalar.
+template <typename A> __device__ __forceinline__ A args_of(const A& a0, const A* tab, unsigned inst)
+{
+    if (!tab) return a0;
+    static_assert(sizeof(A) % 4 == 0, "argument structs are dword-sized");
+    struct Raw {
+        unsigned w[sizeof(A) / 4];
+    } r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    using cu = const __attribute__((address_space(4))) unsigned;
+#else
+    using cu = const unsigned;
+#endif
+    cu* src = (cu*)(tab + inst);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(A) / 4; ++i) r.w[i] = src[i];
+    return __builtin_bit_cast(A, r);
+}
+
 template <typename T> __device__ __forceinline__ double2 to_d2(c2_t<T> a) { return double2{(double)a.x, (double)a.y}; }
 template <typename T> __device__ __forceinline__ c2_t<T> from_d2(double2 a) { return c2_t<T>{(T)a.x, (T)a.y}; }
 
 // ---------------------------------------------------------------------------
 // Block reduction of NV doubles per thread into out[0..NV): wave64 shuffles,
 // then one LDS slot per wave, summed in wave order (deterministic).
-template <int NV> __device__ __forceinline__ void block_sum_store(double (&s)[NV], double* out)
+template <int NV> __device__ __forceinline__ void block_sum_store(double (&s)[NV], gp<double> out)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -156,38 +177,32 @@ struct TileAt {
 
 template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArgs& a, int p)
 {
+    // complex-layout channels first, then the half-layout ones (selects, not
+    // branches: every value is block-uniform)
     TileAt r;
     const int ntf = a.nx / P;
+    const int nth = a.hcols ? (a.hcols + P - 1) / P : 1;
     const long long nbf = (long long)(a.hcols ? a.ncomp : a.nchan) * ntf * a.nouter;
-    long long b = blockIdx.x;
-    int xt;
-    if (b < nbf) {
-        const long long per = (long long)ntf * a.nouter;
-        r.ch = (int)(b / per);
-        b -= (long long)r.ch * per;
-        xt = (int)(b % ntf);
-        r.outer = b / ntf;
-        r.ix = xt * P + p;
-        r.sp_off = r.outer * a.outer_stride + r.ix;
-        r.off = r.ch * a.chan_stride + r.sp_off;
-        r.st = a.stride;
-        r.valid = true;
-    } else {
-        b -= nbf;
-        const int nth = (a.hcols + P - 1) / P;
-        const long long per = (long long)nth * a.nouter;
-        const int hc = (int)(b / per);
-        b -= (long long)hc * per;
-        xt = (int)(b % nth);
-        r.outer = b / nth;
-        r.ch = a.ncomp + hc;
-        r.ix = xt * P + p;
-        r.valid = r.ix < a.hcols;
-        if (!r.valid) r.ix = 0;   // keeps table lookups in range; nothing is stored
-        r.sp_off = r.outer * a.outer_stride + r.ix;
-        r.off = a.hbase + hc * a.hchan + r.outer * a.houter + r.ix;
-        r.st = a.hstride;
-    }
+    const bool half = (long long)blockIdx.x >= nbf;
+    const long long b = half ? (long long)blockIdx.x - nbf : (long long)blockIdx.x;
+    const int nt = half ? nth : ntf;
+    const long long per = (long long)nt * a.nouter;
+    const int cl = (int)(b / per);            // channel within its layout
+    const long long rem = b - (long long)cl * per;
+    const int xt = (int)(rem % nt);
+    r.outer = rem / nt;
+    r.ch = half ? a.ncomp + cl : cl;
+    r.ix = xt * P + p;
+    r.valid = !half || r.ix < a.hcols;
+    if (!r.valid) r.ix = 0;   // keeps table lookups in range; nothing is stored
+    r.sp_off = r.outer * a.outer_stride + r.ix;
+    // blends by mask: a select between two struct fields would be folded into
+    // a load through a selected address, which keeps the struct in scratch
+    const long long m = half ? -1LL : 0LL;
+    const long long off_h = a.hbase + cl * a.hchan + r.outer * a.houter + r.ix;
+    const long long off_c = cl * a.chan_stride + r.sp_off;
+    r.off = (off_h & m) | (off_c & ~m);
+    r.st = (a.hstride & m) | (a.stride & ~m);
     return r;
 }
 
@@ -198,19 +213,19 @@ k_strided(const c2_t<T>* __restrict__ tw, StridedArgs a0, const StridedArgs* __r
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
-    const StridedArgs a = tab ? tab[blockIdx.y] : a0;
-    if (a.stop && *a.stop) return;
-    T2* __restrict__ data = (T2*)a.data;
+    const StridedArgs a = args_of(a0, tab, blockIdx.y);
+    if (a.stop && *glob(a.stop)) return;
+    const gp<T2> data = glob((T2*)a.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a, p);
-    T2* base = data + at.off;
+    const gp<T2> base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
     T2 v[E];
     if (DIR < 0) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = at.valid ? base[(long long)elemA<N, V>(t, q) * at.st] : T2{0, 0};
         if (a.scale) {
-            double s = a.scale[at.ch < a.ncomp ? at.ch : 0];
+            double s = glob(a.scale)[at.ch < a.ncomp ? at.ch : 0];
             if (at.ch >= a.ncomp) s = s * s;
             const T st = (T)s;
 #pragma unroll
@@ -265,20 +280,21 @@ template <int N, int V>
 __device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastArgs& a, const TileAt& at)
 {
     constexpr int E = Plan<N, V>::E;
-    const double kx = a.kX[at.ix], ko = a.kO[at.outer];
+    const gp<const double> kA = glob(a.kA), kX = glob(a.kX), kO = glob(a.kO);
+    const double kx = kX[at.ix], ko = kO[at.outer];
     const double kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const double ka = a.kA[elemB<N, V>(t, q)];
+            const double ka = kA[elemB<N, V>(t, q)];
             const double k2 = kxo2 + ka * ka;
             v[q] = double2{(a.lap_scale * v[q].x) * k2, (a.lap_scale * v[q].y) * k2};
         }
         return;
     }
-    const double* wt = (const double*)a.wtab;
+    const gp<const double> wt = glob((const double*)a.wtab);
     const int nxf = a.s.nx, nO = a.s.nouter;
-    const double kxp = a.kX[(nxf - at.ix) % nxf], kop = a.kO[(nO - (int)at.outer) % nO];
+    const double kxp = kX[(nxf - at.ix) % nxf], kop = kO[(nO - (int)at.outer) % nO];
     const double num0 = kx * a.dX + ko * a.dO;
     const double num0p = kxp * a.dX + kop * a.dO;
     const double kxo2p = kxp * kxp + kop * kop;
@@ -286,11 +302,11 @@ __device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastA
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int ea = elemB<N, V>(t, q);
-        const double ka = a.kA[ea];
+        const double ka = kA[ea];
         double vt = 0.0;
         if (a.c_dd != 0.0) {
             vt = vdd_of(a.c_dd, num0 + ka * a.dA, kxo2 + ka * ka);
-            const double kap = a.kA[(N - ea) % N];
+            const double kap = kA[(N - ea) % N];
             if (!(xo_neg && kap == -ka))
                 vt = 0.5 * (vt + vdd_of(a.c_dd, num0p + kap * a.dA, kxo2p + kap * kap));
         }
@@ -313,22 +329,23 @@ template <int N, int V>
 __device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastArgs& a, const TileAt& at)
 {
     constexpr int E = Plan<N, V>::E;
-    const float kx = (float)a.kX[at.ix], ko = (float)a.kO[at.outer];
+    const gp<const double> kA = glob(a.kA), kX = glob(a.kX), kO = glob(a.kO);
+    const float kx = (float)kX[at.ix], ko = (float)kO[at.outer];
     const float kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
         const float ls = (float)a.lap_scale;
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const float ka = (float)a.kA[elemB<N, V>(t, q)];
+            const float ka = (float)kA[elemB<N, V>(t, q)];
             const float f = ls * (kxo2 + ka * ka);
             v[q] = float2{v[q].x * f, v[q].y * f};
         }
         return;
     }
-    const float* wt = (const float*)a.wtab;
+    const gp<const float> wt = glob((const float*)a.wtab);
     const int nxf = a.s.nx, nO = a.s.nouter;
-    const double kxpd = a.kX[(nxf - at.ix) % nxf], kopd = a.kO[(nO - (int)at.outer) % nO];
-    const bool xo_neg = (kxpd == -a.kX[at.ix]) && (kopd == -a.kO[at.outer]);
+    const double kxpd = kX[(nxf - at.ix) % nxf], kopd = kO[(nO - (int)at.outer) % nO];
+    const bool xo_neg = (kxpd == -kX[at.ix]) && (kopd == -kO[at.outer]);
     const float kxp = (float)kxpd, kop = (float)kopd;
     const float dA = (float)a.dA, dX = (float)a.dX, dO = (float)a.dO, c_dd = (float)a.c_dd;
     const float num0 = kx * dX + ko * dO, num0p = kxp * dX + kop * dO;
@@ -339,7 +356,7 @@ __device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastAr
         const int ea = elemB<N, V>(t, q);
         float vt = 0.f;
         if (c_dd != 0.f) {
-            const double kad = a.kA[ea], kapd = a.kA[(N - ea) % N];
+            const double kad = kA[ea], kapd = kA[(N - ea) % N];
             const float ka = (float)kad, kap = (float)kapd;
             vt = vdd_of_f32(c_dd, num0 + ka * dA, kxo2 + ka * ka);
             if (!(xo_neg && kapd == -kad))
@@ -358,18 +375,18 @@ k_last(const c2_t<T>* __restrict__ tw, LastArgs a0, const LastArgs* __restrict__
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
-    const LastArgs a = tab ? tab[blockIdx.y] : a0;
-    if (a.s.stop && *a.s.stop) return;
-    T2* __restrict__ data = (T2*)a.s.data;
+    const LastArgs a = args_of(a0, tab, blockIdx.y);
+    if (a.s.stop && *glob(a.s.stop)) return;
+    const gp<T2> data = glob((T2*)a.s.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a.s, p);
-    T2* base = data + at.off;
+    const gp<T2> base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
     T2 v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = at.valid ? base[(long long)elemA<N, V>(t, q) * at.st] : T2{0, 0};
     if (a.s.scale) {
-        double s = a.s.scale[at.ch < a.s.ncomp ? at.ch : 0];
+        double s = glob(a.s.scale)[at.ch < a.s.ncomp ? at.ch : 0];
         if (at.ch >= a.s.ncomp) s = s * s;
         const T st = (T)s;
 #pragma unroll
@@ -440,12 +457,19 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     constexpr int NV = 3 * NC;
     __shared__ T2 lds[P * lds_stride<N>()];
-    const XArgs<T> a = tab ? tab[blockIdx.y] : a0;
-    if (a.stop && *a.stop) return;
+    const XArgs<T> a = args_of(a0, tab, blockIdx.y);
+    if (a.stop && *glob(a.stop)) return;
+    const gp<T2> spec = glob(a.spec);
+    const gp<double2> P0[2] = {glob(a.P0[0]), glob(a.P0[1])};
+    const gp<T2> PS[2] = {glob(a.PS[0]), glob(a.PS[1])};
+    const gp<T2> PSo[2] = {glob(a.PSo[0]), glob(a.PSo[1])};
+    const gp<T2> K[2] = {glob(a.K[0]), glob(a.K[1])};
+    const gp<const T> Vr[2] = {glob(a.Vr[0]), glob(a.Vr[1])};
+    const gp<const T> Vi[2] = {glob(a.Vi[0]), glob(a.Vi[1])};
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
-    T2* specd = a.specd + ((long long)blockIdx.x * P + p) * a.dpitch;   // this row's half spectrum
+    const gp<T2> specd = glob(a.specd) + ((long long)blockIdx.x * P + p) * a.dpitch;   // this row's half spectrum
 
     T2 v[E];
     T nd[E];             // density of the new state (DENS)
@@ -457,11 +481,11 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
         for (int c = 0; c < NC; ++c) {
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                v[q] = from_d2<T>(a.P0[c][row0 + elemA<N, V>(t, q)]);
+                v[q] = from_d2<T>(P0[c][row0 + elemA<N, V>(t, q)]);
                 if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
             }
             fft_forward<T2, N, V, true>(v, t, my, tw);
-            T2* sc = a.spec + c * a.ntot + row0;
+            const gp<T2> sc = spec + c * a.ntot + row0;
 #pragma unroll
             for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
         }
@@ -491,12 +515,12 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
 
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            T2* sc = a.spec + c * a.ntot + row0;
+            const gp<T2> sc = spec + c * a.ntot + row0;
 #pragma unroll
             for (int q = 0; q < E; ++q) v[q] = sc[elemB<N, V>(t, q)];
             fft_inverse<T2, N, V, true>(v, t, my, tw);   // v = KE psi_c (layout A)
-            const double sig = a.scale ? a.scale[c] : 1.0;
-            const double sig1 = (NC == 2 && a.scale) ? a.scale[1] : 1.0;
+            const double sig = a.scale ? glob(a.scale)[c] : 1.0;
+            const double sig1 = (NC == 2 && a.scale) ? glob(a.scale)[1] : 1.0;
             const double g4s = a.g4[c][c], g2s = a.g2[c][c];
             const double g4o = a.g4[c][1 - c], g2o = a.g2[c][1 - c];
             if constexpr (!(MODE == MODE_RK && sizeof(T) == 4)) {
@@ -506,11 +530,11 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     double2 po = double2{0.0, 0.0};
                     double2 ps;
                     if (from_p0) {
-                        const double2 r = a.P0[c][g];
+                        const double2 r = P0[c][g];
                         po = double2{sig * r.x, sig * r.y};
                         ps = po;
                     } else {
-                        ps = to_d2<T>(a.PS[c][g]);
+                        ps = to_d2<T>(PS[c][g]);
                     }
                     const double n = ps.x * ps.x + ps.y * ps.y;
                     double no = 0.0;
@@ -518,10 +542,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (c == 0) {
                             double2 o;
                             if (from_p0) {
-                                const double2 r = a.P0[1][g];
+                                const double2 r = P0[1][g];
                                 o = double2{sig1 * r.x, sig1 * r.y};
                             } else {
-                                o = to_d2<T>(a.PS[1][g]);
+                                o = to_d2<T>(PS[1][g]);
                             }
                             no = o.x * o.x + o.y * o.y;
                             nsave[q] = n;
@@ -529,8 +553,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             no = nsave[q];
                         }
                     }
-                    const double vr = (double)a.Vr[c][g];
-                    const double vi = a.Vi[c] ? (double)a.Vi[c][g] : 0.0;
+                    const double vr = (double)Vr[c][g];
+                    const double vi = Vi[c] ? (double)Vi[c][g] : 0.0;
                     const double2 ke = to_d2<T>(v[q]);
                     double rX = (vr + g4s * n);
                     if (NC == 2) rX = rX + g4o * no;
@@ -544,18 +568,18 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
                     }
                     if (MODE == MODE_APPLY) {
-                        if (a.hmu) a.hmu[c * a.ntot + g] = h;
-                        if (a.hen) a.hen[c * a.ntot + g] = he;
+                        if (a.hmu) glob(a.hmu)[c * a.ntot + g] = h;
+                        if (a.hen) glob(a.hen)[c * a.ntot + g] = he;
                         sums[3 * c + 0] += n;
                         sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
                         sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
                     } else if (MODE == MODE_HB) {
-                        const double2 old = to_d2<T>(a.PS[c][g]);
-                        const double m = a.mu ? a.mu[c] : 0.0;
+                        const double2 old = to_d2<T>(PS[c][g]);
+                        const double m = a.mu ? glob(a.mu)[c] : 0.0;
                         const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
                                                    (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
-                        a.PS[c][g] = from_d2<T>(ps);
-                        a.P0[c][g] = nw;
+                        PS[c][g] = from_d2<T>(ps);
+                        P0[c][g] = nw;
                         sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
                         sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
                         sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
@@ -563,7 +587,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
                     } else {
                         if (a.stage != 1) {
-                            const double2 r = a.P0[c][g];
+                            const double2 r = P0[c][g];
                             po = double2{sig * r.x, sig * r.y};
                         }
                         if (a.stage < 4) {
@@ -571,21 +595,21 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
                                                        po.y + (a.cr * h.y + a.ci * h.x)};
                             if (a.stage == 1) {
-                                a.K[c][g] = from_d2<T>(h);
+                                K[c][g] = from_d2<T>(h);
                             } else {
-                                const double2 k = to_d2<T>(a.K[c][g]);
-                                a.K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
+                                const double2 k = to_d2<T>(K[c][g]);
+                                K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
                             }
                             const T2 st = from_d2<T>(nw);
-                            (a.PSo[c] ? a.PSo[c] : a.PS[c])[g] = st;
+                            (PSo[c] ? PSo[c] : PS[c])[g] = st;
                             v[q] = st;
                         } else {
                             // FinalRKStep: psi = old + c*(k + h)/6
-                            const double2 k = to_d2<T>(a.K[c][g]);
+                            const double2 k = to_d2<T>(K[c][g]);
                             const double aX = k.x + h.x, aY = k.y + h.y;
                             const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
                                                        po.y + (a.cr * aY + a.ci * aX) / 6.0};
-                            a.P0[c][g] = nw;
+                            P0[c][g] = nw;
                             v[q] = from_d2<T>(nw);
                             if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                         }
@@ -603,10 +627,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     const long long g = row0 + elemA<N, V>(t, q);
                     float2 ps;
                     if (a.stage == 1) {
-                        const double2 r = a.P0[c][g];
+                        const double2 r = P0[c][g];
                         ps = float2{(float)(sig * r.x), (float)(sig * r.y)};
                     } else {
-                        ps = a.PS[c][g];
+                        ps = PS[c][g];
                     }
                     const float n = ps.x * ps.x + ps.y * ps.y;
                     float no = 0.f;
@@ -614,10 +638,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (c == 0) {
                             float2 o;
                             if (a.stage == 1) {
-                                const double2 r = a.P0[1][g];
+                                const double2 r = P0[1][g];
                                 o = float2{(float)(sig1 * r.x), (float)(sig1 * r.y)};
                             } else {
-                                o = a.PS[1][g];
+                                o = PS[1][g];
                             }
                             no = o.x * o.x + o.y * o.y;
                             nsave[q] = n;
@@ -625,8 +649,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             no = (float)nsave[q];
                         }
                     }
-                    const float vr = (float)a.Vr[c][g];
-                    const float vi = a.Vi[c] ? (float)a.Vi[c][g] : 0.f;
+                    const float vr = (float)Vr[c][g];
+                    const float vi = Vi[c] ? (float)Vi[c][g] : 0.f;
                     const float2 ke = v[q];
                     float rX = vr + f4s * n;
                     if (NC == 2) rX = rX + f4o * no;
@@ -635,25 +659,25 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     if (a.stage < 4) {
                         float2 po = ps;
                         if (a.stage != 1) {
-                            const double2 r = a.P0[c][g];
+                            const double2 r = P0[c][g];
                             po = float2{(float)(sig * r.x), (float)(sig * r.y)};
                         }
                         const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
                         if (a.stage == 1) {
-                            a.K[c][g] = h;
+                            K[c][g] = h;
                         } else {
-                            const float2 k = a.K[c][g];
-                            a.K[c][g] = float2{k.x + 2.f * h.x, k.y + 2.f * h.y};
+                            const float2 k = K[c][g];
+                            K[c][g] = float2{k.x + 2.f * h.x, k.y + 2.f * h.y};
                         }
-                        (a.PSo[c] ? a.PSo[c] : a.PS[c])[g] = nw;
+                        (PSo[c] ? PSo[c] : PS[c])[g] = nw;
                         v[q] = nw;
                     } else {
-                        const float2 k = a.K[c][g];
+                        const float2 k = K[c][g];
                         const float ax = k.x + h.x, ay = k.y + h.y;
                         const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
-                        const double2 r = a.P0[c][g];
+                        const double2 r = P0[c][g];
                         const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
-                        a.P0[c][g] = nw;
+                        P0[c][g] = nw;
                         v[q] = from_d2<T>(nw);
                         if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                     }
@@ -667,7 +691,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
             }
         }
         if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm))
-            block_sum_store<NV>(sums, a.partials + (long long)NV * blockIdx.x);
+            block_sum_store<NV>(sums, glob(a.partials) + (long long)NV * blockIdx.x);
         if (MODE == MODE_APPLY) return;
     }
     if (DENS) {
@@ -724,8 +748,9 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
-    T2* sp = (T2*)a.spec;
-    const Y2* y = (const Y2*)a.y;
+    const gp<T2> sp = glob((T2*)a.spec);
+    const gp<const Y2> y = glob((const Y2*)a.y);
+    const gp<double2> abar = glob(a.abar);
     T2 v[E];
     double nn[E], rr[E];
 #pragma unroll
@@ -733,10 +758,10 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
         const long long g = row0 + elemA<N, V>(t, q);
         double2 ab;
         if (a.src) {
-            ab = zmul(a.alpha, a.src[g]);
-            a.abar[g] = ab;
+            ab = zmul(a.alpha, glob(a.src)[g]);
+            abar[g] = ab;
         } else {
-            ab = a.abar[g];
+            ab = abar[g];
         }
         const Y2 yr = y[g];
         const double yx = (double)yr.x, yy = (double)yr.y;
@@ -772,9 +797,11 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
-    T2* sp = (T2*)a.spec;
-    const Y2* y = (const Y2*)a.y;
-    const T* Vr = (const T*)a.Vr;
+    const gp<T2> sp = glob((T2*)a.spec);
+    const gp<const Y2> y = glob((const Y2*)a.y);
+    const gp<const T> Vr = glob((const T*)a.Vr);
+    const gp<double2> lam = glob(a.lam), gb_out = glob(a.gb_out);
+    const gp<double> vbar = glob(a.vbar);
     T2 v[E];
     double phin[E], phir[E];
     if (DENS) {
@@ -798,7 +825,7 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
         const long long g = row0 + elemA<N, V>(t, q);
         const Y2 yr = y[g];
         const double2 yv = double2{(double)yr.x, (double)yr.y};
-        const double2 ab = a.abar[g];
+        const double2 ab = glob(a.abar)[g];
         const double n = yv.x * yv.x + yv.y * yv.y;
         const double r = yv.x * ab.x + yv.y * ab.y;
         const double pn = DENS ? phin[q] : 0.0;
@@ -806,19 +833,19 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
         const double w = (double)Vr[g] + a.g4 * n + pn;
         const double c2 = 2.0 * (a.g4 * r + pr);
         const double2 G = double2{(double)v[q].x + w * ab.x + c2 * yv.x, (double)v[q].y + w * ab.y + c2 * yv.y};
-        const double2 li = a.lam_in[g];
-        const double2 base = a.init ? li : a.lam[g];
-        a.lam[g] = double2{base.x + G.x, base.y + G.y};
+        const double2 li = glob(a.lam_in)[g];
+        const double2 base = a.init ? li : lam[g];
+        lam[g] = double2{base.x + G.x, base.y + G.y};
         if (a.gb_out) {
             const double2 u = zmul(a.ga, li), wv = zmul(a.gg, G);
-            a.gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
+            gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
         }
-        a.vbar[g] += r;
+        vbar[g] += r;
         s[0] += n * r;
         s[1] += pn * r;
     }
     double tot[2] = {s[0], s[1]};
-    block_sum_store<2>(tot, a.partials + 2 * (long long)blockIdx.x + 0);
+    block_sum_store<2>(tot, glob(a.partials) + 2 * (long long)blockIdx.x + 0);
 }
 
 // ---------------------------------------------------------------------------
