@@ -75,9 +75,10 @@ def run_form(torch, w, psi, V, precision, steps, warmup):
     obs = ctx.observables(t)
     out = ctx.download_psi()
     chosen = {k: v for k, v in ctx.variants().items() if k[1] == {"xstage": 0, "strided": 1, "last": 2}[k[0]]}
+    nslab = ctx.slabs()
     ctx.close()
     return ms / steps, kt, obs, out, {"autotune_ms": {k: [round(x, 4) for x in v] for k, v in tune.items()},
-                                      "variant": {k[0]: v for k, v in chosen.items()}}
+                                      "variant": {k[0]: v for k, v in chosen.items()}, "slabs": nslab}
 
 
 def cpu_baseline(w, psi, V, nsteps, threads):

@@ -26,7 +26,8 @@ EXPORTS = (
     "fgpe_fft3d_c2c_f64", "fgpe_fft3d_c2c_f32", "fgpe_laplacian_f64", "fgpe_apply_h",
     "fgpe_observables", "fgpe_adjoint_forward", "fgpe_adjoint_backward", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
     "fgpe_timing_enable",
-    "fgpe_timing_read", "fgpe_autotune", "fgpe_set_variant", "fgpe_get_variant",
+    "fgpe_timing_read", "fgpe_autotune", "fgpe_set_variant", "fgpe_get_variant", "fgpe_set_slabs",
+    "fgpe_get_slabs",
     "fgpe_group_create", "fgpe_group_destroy", "fgpe_group_autotune", "fgpe_group_imag_steps",
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
 )
@@ -84,6 +85,8 @@ def load() -> C.CDLL:
     L.fgpe_autotune.argtypes = [_vp, C.c_int, _dp]
     L.fgpe_set_variant.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.fgpe_get_variant.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.fgpe_set_slabs.argtypes = [_vp, C.c_int]
+    L.fgpe_get_slabs.argtypes = [_vp, C.POINTER(C.c_int)]
     L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
     L.fgpe_timing_read.argtypes = [_vp, _dp]
     _ip = C.POINTER(C.c_int)
@@ -244,10 +247,20 @@ class Context:
         return bool(f.value)
 
     def autotune(self, reps=5):
-        """Time both plan variants per kernel class; returns {class: (ms v0, ms v1)}."""
-        out = np.zeros(6)
+        """Time both plan variants per kernel class, then 1/2/4 column slabs of
+        the y/last/y sequence; returns {class: (ms v0, ms v1), "slabs": (ms 1, 2, 4)}."""
+        out = np.zeros(9)
         check(load().fgpe_autotune(self.h, reps, _dptr(out)))
-        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6])}
+        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6]),
+                "slabs": tuple(out[6:9])}
+
+    def set_slabs(self, nslab: int):
+        check(load().fgpe_set_slabs(self.h, nslab))
+
+    def slabs(self) -> int:
+        v = C.c_int()
+        check(load().fgpe_get_slabs(self.h, C.byref(v)))
+        return v.value
 
     def set_variant(self, kclass: int, axis: int, variant: int):
         check(load().fgpe_set_variant(self.h, kclass, axis, variant))
@@ -303,7 +316,7 @@ class Group:
             pass
 
     def autotune(self, reps=3):
-        out = np.zeros(6)
+        out = np.zeros(9)
         check(load().fgpe_group_autotune(self.h, reps, _dptr(out)))
         return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6])}
 

@@ -98,6 +98,7 @@ struct fgpe_ctx {
     void* tw64[3] = {nullptr, nullptr, nullptr};
     void* tw32[3] = {nullptr, nullptr, nullptr};
     int var[FGPE_NKCLASS][3] = {};   // plan variant per kernel class and axis
+    int nslab = 1;                   // column slabs of the y / last / y pass sequence
 
     // launch timing (fgpe_timing_enable)
     bool timing = false;
@@ -127,6 +128,23 @@ struct TScope {
 // supported nx and both precisions).
 inline long long dens_pitch(const fgpe_ctx* c) { return c->n[0] / 2 + 16; }
 
+// Slab s of nslab of the columns of a strided-pass argument set: complex
+// columns in equal parts (multiples of the tile width P), half-layout columns
+// in near-equal parts.  nslab = 1 leaves the arguments unchanged.
+inline StridedArgs slab_of(StridedArgs a, int s, int nslab)
+{
+    if (nslab <= 1) return a;
+    const int w = a.nx / nslab;
+    a.x0 = s * w;
+    a.ncols = w;
+    if (a.hcols) {
+        const int hw = (a.hcols + nslab - 1) / nslab;
+        a.hx0 = s * hw;
+        a.hcols = s * hw >= a.hcols ? 0 : (a.hcols - s * hw < hw ? a.hcols - s * hw : hw);
+    }
+    return a;
+}
+
 inline long long n_outer(const fgpe_ctx* c, int axis) { return axis == 1 ? c->n[2] : c->n[1]; }
 
 // Arguments of the strided passes and of the k-space multiply for one context.
@@ -136,6 +154,7 @@ inline StridedArgs strided_args(const fgpe_ctx* c, void* data, int axis, const d
     StridedArgs a{};
     a.data = data;
     a.nx = c->n[0];
+    a.ncols = c->n[0];
     a.chan_stride = c->ntot;
     a.scale = scale;
     a.ncomp = ncomp;
@@ -197,9 +216,11 @@ template <typename T> struct Launch {
     static long long xblocks(fgpe_ctx* c);
     static int fft_x(fgpe_ctx* c, const void* in, void* out, int dir);
     static int strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp, const double* scale,
-                       const int* stop, bool half, const void* tab = nullptr, int ninst = 1);
+                       const int* stop, bool half, const void* tab = nullptr, int ninst = 1, int slab = 0,
+                       int nslab = 1);
     static int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter,
-                    const int* stop, bool half, const void* tab = nullptr, int ninst = 1);
+                    const int* stop, bool half, const void* tab = nullptr, int ninst = 1, int slab = 0,
+                    int nslab = 1);
     static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode, const void* tab = nullptr,
                       int ninst = 1);
     // adjoint VJP prologue / epilogue; y64: the state is FP64 (else T)

@@ -173,9 +173,13 @@ template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, i
 {
     const bool filter = has_filter(c->hp);
     if (last_axis(c) == 2) {
-        FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop, half));
-        FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop, half));
-        FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop, half));
+        // slab by slab of columns: a slab's three passes run while it is still in the memory-side cache
+        const int ns = half ? c->nslab : 1;
+        for (int sl = 0; sl < ns; ++sl) {
+            FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop, half, nullptr, 1, sl, ns));
+            FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop, half, nullptr, 1, sl, ns));
+            FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop, half, nullptr, 1, sl, ns));
+        }
     } else {
         FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop, half));
     }
@@ -683,6 +687,40 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
             if (per < best) { best = per; bestv = v; }
         }
         c->var[cls][axis] = bestv;
+    }
+    // column slabs of the y / last / y sequence (3D only): 1, 2 or 4, timed
+    // as whole RK stages (passes + x stage), since the x stage between two
+    // pass sequences decides what the memory-side cache still holds
+    if (la == 2 && rc == FGPE_OK) {
+        const int cand[3] = {1, 2, 4};
+        double best = 1e300;
+        int bests = 1;
+        auto stage = [&]() -> int {
+            FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
+            return run(FGPE_KCLASS_XSTAGE);
+        };
+        for (int k = 0; k < 3 && rc == FGPE_OK; ++k) {
+            c->nslab = cand[k];
+            int r = stage();   // warm-up
+            if (r == FGPE_EUNSUP) continue;   // slab width not a multiple of a tile width
+            if (r != FGPE_OK) {
+                rc = r;
+                break;
+            }
+            (void)hipEventRecord(e0, c->stream);
+            for (int i = 0; i < reps && rc == FGPE_OK; ++i) rc = stage();
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / reps;
+            if (out) out[6 + k] = per;
+            if (per < best) {
+                best = per;
+                bests = cand[k];
+            }
+        }
+        c->nslab = bests;
     }
     c->timing = timing;
     (void)hipEventDestroy(e0);
@@ -1237,6 +1275,8 @@ int fgpe_autotune(fgpe_ctx* c, int reps, double* host_ms)
 {
     if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
     if (reps < 1) return fail(FGPE_EINVAL, "reps must be positive");
+    if (host_ms)
+        for (int i = 0; i < 9; ++i) host_ms[i] = 0.0;
     c->hb_active = false;
     return c->mixed() ? autotune<float>(c, reps, host_ms) : autotune<double>(c, reps, host_ms);
 }
@@ -1257,6 +1297,22 @@ int fgpe_get_variant(fgpe_ctx* c, int kclass, int axis, int* variant)
     if (kclass < 0 || kclass >= FGPE_NKCLASS || axis < 0 || axis > 2)
         return fail(FGPE_EINVAL, "kclass or axis out of range");
     *variant = c->var[kclass][axis];
+    return FGPE_OK;
+}
+
+int fgpe_set_slabs(fgpe_ctx* c, int nslab)
+{
+    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (nslab != 1 && nslab != 2 && nslab != 4) return fail(FGPE_EINVAL, "nslab must be 1, 2 or 4");
+    if (c->n[0] % nslab) return fail(FGPE_EUNSUP, "nx not divisible by nslab");
+    c->nslab = nslab;
+    return FGPE_OK;
+}
+
+int fgpe_get_slabs(fgpe_ctx* c, int* nslab)
+{
+    if (!c || !nslab) return fail(FGPE_EINVAL, "NULL argument");
+    *nslab = c->nslab;
     return FGPE_OK;
 }
 
@@ -1331,7 +1387,7 @@ int fgpe_group_autotune(fgpe_group* g, int reps, double* host_ms)
     if (!g) return fail(FGPE_EINVAL, "group is NULL");
     if (reps < 1) return fail(FGPE_EINVAL, "reps < 1");
     if (host_ms)
-        for (int i = 0; i < 6; ++i) host_ms[i] = 0.0;
+        for (int i = 0; i < 9; ++i) host_ms[i] = 0.0;
     return g->c[0]->mixed() ? group_autotune<float>(g, reps, host_ms) : group_autotune<double>(g, reps, host_ms);
 }
 

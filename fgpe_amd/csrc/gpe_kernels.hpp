@@ -131,6 +131,11 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
 // pitch hpitch.  Those channels take ceil(hcols/P) tiles per outer index; the
 // lanes of a last partial tile compute on zeros and store nothing.
 //
+// A launch may cover a slab of columns only: complex columns [x0, x0 + ncols)
+// and half-layout columns [hx0, hx0 + hcols), so that the y-forward, last-axis
+// and y-inverse passes of one slab run back to back while the slab is still
+// in the memory-side cache.
+//
 // scale (forward only, may be null): channel c < ncomp is multiplied by
 // scale[c], a real-field channel by scale[0]^2 (the pending renormalisation
 // of the imaginary-time schemes).
@@ -140,14 +145,16 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
 // shape; with a null table they use the by-value argument.
 struct StridedArgs {
     void* data;             // the channels (c2_t<T>)
-    int nx;                 // columns of the complex channels (x extent)
+    int nx;                 // x extent of the grid
+    int ncols, x0;          // complex columns of this launch
+    int hx0;                // first half-layout column of this launch
     long long stride;       // element stride along the axis (complex layout)
     long long outer_stride; // stride of the outer index (complex layout)
     long long chan_stride;  // elements per complex-layout channel
     int nouter;             // extent of the outer index
     int ncomp;              // complex channels
     int nchan;              // channels in all
-    int hcols;              // > 0: real channels in half-spectrum layout
+    int hcols;              // > 0: real channels in half-spectrum layout, columns of this launch
     long long hstride;      // half layout: element stride along the axis
     long long houter;       //              stride of the outer index
     long long hbase;        //              offset of the first real channel
@@ -160,7 +167,7 @@ struct StridedArgs {
 // half-layout ones.
 template <int P> __host__ __device__ inline long long strided_blocks(const StridedArgs& a)
 {
-    const long long ntf = a.nx / P;
+    const long long ntf = a.ncols / P;
     if (a.hcols == 0) return (long long)a.nchan * ntf * a.nouter;
     const long long nth = (a.hcols + P - 1) / P;
     return ((long long)a.ncomp * ntf + (long long)(a.nchan - a.ncomp) * nth) * a.nouter;
@@ -180,7 +187,7 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     // complex-layout channels first, then the half-layout ones (selects, not
     // branches: every value is block-uniform)
     TileAt r;
-    const int ntf = a.nx / P;
+    const int ntf = a.ncols / P;
     const int nth = a.hcols ? (a.hcols + P - 1) / P : 1;
     const long long nbf = (long long)(a.hcols ? a.ncomp : a.nchan) * ntf * a.nouter;
     const bool half = (long long)blockIdx.x >= nbf;
@@ -192,8 +199,9 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     const int xt = (int)(rem % nt);
     r.outer = rem / nt;
     r.ch = half ? a.ncomp + cl : cl;
-    r.ix = xt * P + p;
-    r.valid = !half || r.ix < a.hcols;
+    const int loc = xt * P + p;   // column within this launch
+    r.valid = !half || loc < a.hcols;
+    r.ix = (half ? a.hx0 : a.x0) + loc;
     if (!r.valid) r.ix = 0;   // keeps table lookups in range; nothing is stored
     r.sp_off = r.outer * a.outer_stride + r.ix;
     // blends by mask: a select between two struct fields would be folded into

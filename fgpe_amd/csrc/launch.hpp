@@ -34,11 +34,11 @@ template <typename T, int N, int V> int fft_x_n(fgpe_ctx* c, const void* in, voi
 
 template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
                                                   const double* scale, const int* stop, bool half, const void* tab,
-                                                  int ninst)
+                                                  int ninst, int slab, int nslab)
 {
     constexpr int P = ps_of<T, N, V>();
-    if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    StridedArgs a = strided_args(c, data, axis, scale, nch, ncomp, half, stop);
+    StridedArgs a = slab_of(strided_args(c, data, axis, scale, nch, ncomp, half, stop), slab, nslab);
+    if (a.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a), (unsigned)ninst), block(P * Plan<N, V>::T);
     const StridedArgs* t = (const StridedArgs*)tab;
     TScope ts(c, FGPE_KCLASS_STRIDED);
@@ -52,11 +52,12 @@ template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int a
 
 template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
                                                const double* scale, bool filter, const int* stop, bool half,
-                                               const void* tab, int ninst)
+                                               const void* tab, int ninst, int slab, int nslab)
 {
     constexpr int P = ps_of<T, N, V>();
-    if (c->n[0] % P) return fail(FGPE_EUNSUP, "nx not a multiple of the strided tile width");
-    const LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half);
+    LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half);
+    a.s = slab_of(a.s, slab, nslab);
+    if (a.s.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a.s), (unsigned)ninst), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_LAST);
     hipLaunchKernelGGL((k_last<T, N, V>), grid, block, 0, c->stream, tw_of<T>(c, axis), a, (const LastArgs*)tab);
@@ -158,9 +159,10 @@ template <typename T> int Launch<T>::fft_x(fgpe_ctx* c, const void* in, void* ou
 
 template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, int dir, int nch, int ncomp,
                                              const double* scale, const int* stop, bool half, const void* tab,
-                                             int ninst)
+                                             int ninst, int slab, int nslab)
 {
-#define CALL(N_, V_) detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop, half, tab, ninst)
+#define CALL(N_, V_) \
+    detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop, half, tab, ninst, slab, nslab)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_STRIDED][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported strided axis length");
@@ -168,9 +170,10 @@ template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, 
 
 template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
                                           const double* scale, bool filter, const int* stop, bool half,
-                                          const void* tab, int ninst)
+                                          const void* tab, int ninst, int slab, int nslab)
 {
-#define CALL(N_, V_) detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half, tab, ninst)
+#define CALL(N_, V_) \
+    detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half, tab, ninst, slab, nslab)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported last axis length");

@@ -177,11 +177,17 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
    exists, 256-thread x blocks, 256-byte row segments).  fgpe_autotune times
    both for the strided (y), last-axis and x-stage classes on the context's
    own buffers (reps launches each; the state is preserved) and keeps the
-   faster; host_ms (6 doubles or NULL) gets the times per class and variant.
-   Needs the Hamiltonian, trap and state set. */
+   faster; then, on 3D grids, it times whole RK stages with the y / last / y
+   sequence split into 1, 2 or 4 column slabs (a slab's three passes run back
+   to back while it is in the memory-side cache) and keeps the fastest.  host_ms (9 doubles or
+   NULL) gets the times per class and variant (6), then per slab count (3;
+   0 where not applicable).  Needs the Hamiltonian, trap and state set. */
 int fgpe_autotune(fgpe_ctx* ctx, int reps, double* host_ms);
 int fgpe_set_variant(fgpe_ctx* ctx, int kclass, int axis, int variant);
 int fgpe_get_variant(fgpe_ctx* ctx, int kclass, int axis, int* variant);
+/* Column slabs of the y / last / y sequence: 1, 2 or 4 (nx divisible). */
+int fgpe_set_slabs(fgpe_ctx* ctx, int nslab);
+int fgpe_get_slabs(fgpe_ctx* ctx, int* nslab);
 
 /* ---- batched sweeps (no counterpart in the reference, which runs one
    instance per process: src/oneComponentGPSolver.cu:26-104) ----------------
@@ -195,7 +201,8 @@ typedef struct fgpe_group fgpe_group;
 int fgpe_group_create(fgpe_ctx* const* ctxs, int n, fgpe_group** out);
 int fgpe_group_destroy(fgpe_group* group);
 /* Times both plan variants of each kernel class as batched launches and sets
-   the faster on every context (host_ms: 6 doubles or NULL, as fgpe_autotune). */
+   the faster on every context (host_ms: 9 doubles or NULL, as fgpe_autotune;
+   groups run unslabbed, the last 3 stay 0). */
 int fgpe_group_autotune(fgpe_group* group, int reps, double* host_ms);
 /* nsteps heavy-ball iterations of every instance (same scheme and stopping
    test as fgpe_imag_steps, per instance), enqueued without a host
