@@ -457,8 +457,18 @@ template <typename T> struct XArgs {
     const int* stop;        // nonzero: the loop has converged, do nothing
 };
 
+// Minimum waves per SIMD the register allocator targets for an x-stage
+// kernel: 4 for the FP32 RK stages of the variant-1 plans up to N = 256
+// (256: 138 -> 128 VGPRs with 3 spilled, 9% faster; shorter pencils fit
+// without spills); 1 (no target) elsewhere, where forcing it spills heavily.
+template <typename T, int N, int V, int NC, int MODE> constexpr int xstage_waves()
+{
+    return (sizeof(T) == 4 && V == 1 && NC == 1 && MODE == MODE_RK && N <= 256 && Plan<N, V>::E <= 8) ? 4 : 1;
+}
+
 template <typename T, int N, int V, int NC, bool DENS, int MODE>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
+__attribute__((amdgpu_waves_per_eu(xstage_waves<T, N, V, NC, MODE>())))
 k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict__ tab)
 {
     using T2 = c2_t<T>;
