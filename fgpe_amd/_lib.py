@@ -37,7 +37,7 @@ KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
 class HParams(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
                 ("quench", C.c_double * 6), ("a_dd", C.c_double), ("dip", C.c_double * 3),
-                ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int)]
+                ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int), ("vpx", C.c_double * 2)]
 
 
 class FgpeError(RuntimeError):
@@ -160,11 +160,13 @@ class Context:
         check(load().fgpe_set_kspace(self.h, _dptr(kx), _dptr(ky), _dptr(kz), float(dv)))
 
     def set_hamiltonian(self, ncomp=1, a=(0.0,), quench=None, a_dd=0.0, dip=(0.0, 0.0, 1.0),
-                        sc_U=0.0, sc_R=0.0, sc_dim=3):
+                        sc_U=0.0, sc_R=0.0, sc_dim=3, vpx=(0.0, 0.0)):
         aa = tuple(a) + (0.0,) * (3 - len(a))
         q = (0.0,) * 6 if quench is None else tuple(quench)
+        vv = tuple(vpx) + (0.0,) * (2 - len(vpx))
         hp = HParams(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
-                     (C.c_double * 6)(*q), a_dd, (C.c_double * 3)(*dip), sc_U, sc_R, sc_dim)
+                     (C.c_double * 6)(*q), a_dd, (C.c_double * 3)(*dip), sc_U, sc_R, sc_dim,
+                     (C.c_double * 2)(*vv))
         check(load().fgpe_set_hamiltonian(self.h, C.byref(hp)))
         self.ncomp = ncomp
 

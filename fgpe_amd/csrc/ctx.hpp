@@ -185,9 +185,10 @@ inline StridedArgs strided_args(const fgpe_ctx* c, void* data, int axis, const d
     return a;
 }
 
-// axis: the last axis
+// axis: the last axis; vpx: include the moving-frame term of the
+// Hamiltonian's full form (fgpe_hparams::vpx)
 inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale,
-                          bool filter, const int* stop, bool half)
+                          bool filter, const int* stop, bool half, bool vpx = false)
 {
     LastArgs a;
     a.s = strided_args(c, data, axis, scale, nch, ncomp, half, stop);
@@ -203,6 +204,7 @@ inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int 
     a.inv_ntot = 1.0 / (double)c->ntot;
     a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
     a.wtab = filter ? c->wtab : nullptr;
+    for (int i = 0; i < 2; ++i) a.vpx[i] = vpx ? c->hp.vpx[i] / (double)c->ntot : 0.0;
     return a;
 }
 
@@ -220,7 +222,7 @@ template <typename T> struct Launch {
                        int nslab = 1);
     static int last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp, const double* scale, bool filter,
                     const int* stop, bool half, const void* tab = nullptr, int ninst = 1, int slab = 0,
-                    int nslab = 1);
+                    int nslab = 1, bool vpx = false);
     static int xstage(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode, const void* tab = nullptr,
                       int ninst = 1);
     // adjoint VJP prologue / epilogue; y64: the state is FP64 (else T)

@@ -168,8 +168,9 @@ template <typename T> int fft3d(fgpe_ctx* c, const void* in, void* out, int dir)
 // y-forward / last axis / y-inverse on the spectral channels.  `scale` is
 // applied by the first forward pass after a prologue or a final stage.
 // half: the real-field channels are in the half-spectrum layout.
+// vpx: the moving-frame term of the Hamiltonian's full form is on.
 template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, int nc, const double* scale,
-                                             const int* stop, bool half)
+                                             const int* stop, bool half, bool vpx = false)
 {
     const bool filter = has_filter(c->hp);
     if (last_axis(c) == 2) {
@@ -177,20 +178,20 @@ template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, i
         const int ns = half ? c->nslab : 1;
         for (int sl = 0; sl < ns; ++sl) {
             FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop, half, nullptr, 1, sl, ns));
-            FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop, half, nullptr, 1, sl, ns));
+            FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop, half, nullptr, 1, sl, ns, vpx));
             FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop, half, nullptr, 1, sl, ns));
         }
     } else {
-        FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop, half));
+        FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop, half, nullptr, 1, 0, 1, vpx));
     }
     return FGPE_OK;
 }
 
 // The solver's channels: the components, then the total density as a half
 // spectrum when a k-space kernel is on.
-template <typename T> int spectral_passes(fgpe_ctx* c, const double* scale, const int* stop)
+template <typename T> int spectral_passes(fgpe_ctx* c, const double* scale, const int* stop, bool vpx = false)
 {
-    return spectral_passes_on<T>(c, c->spec, c->nch, c->hp.ncomp, scale, stop, true);
+    return spectral_passes_on<T>(c, c->spec, c->nch, c->hp.ncomp, scale, stop, true, vpx);
 }
 
 template <typename T> XArgs<T> base_xargs(fgpe_ctx* c, double t)
@@ -272,7 +273,9 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
             // PARS[T] at each stage (src/oneComponentGPSolver.cu:175,184)
             const double frac = s == 1 ? 0.0 : (s == 4 ? 1.0 : 0.5);
             const double t = s == 1 ? t0 + it * dt : t0 + (it + frac) * dt;
-            FGPE_RET(spectral_passes<T>(c, s == 1 ? scale : nullptr, nullptr));
+            // the moving-frame term rides with alg_calcHpsi, i.e. stage 4 of the
+            // one-component step (src/oneComponentGPSolver.cu:170-187)
+            FGPE_RET(spectral_passes<T>(c, s == 1 ? scale : nullptr, nullptr, s == 4 && nc == 1));
             XArgs<T> a = base_xargs<T>(c, t);
             const double cf = (s <= 2) ? -0.5 * dt : -1.0 * dt;
             if (imaginary) { a.cr = cf; a.ci = 0.0; }
@@ -375,7 +378,7 @@ template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const doub
     const long long nb = Launch<T>::xblocks(c);
     const int* stop = c->flags + FL_STOP;
     for (int it = 0; it < nsteps; ++it) {
-        FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, stop));
+        FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, stop, true));
         FGPE_RET(Launch<T>::xstage(c, hb_xargs<T>(c, it * dt, dt, beta), nc, dens, MODE_HB));
         FGPE_RET(finalize(c, nb, FIN_HB, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP, eps));
     }
@@ -386,7 +389,7 @@ template <typename T> int apply_h(fgpe_ctx* c, double t, void* hmu, void* hen, d
 {
     FGPE_RET(ensure_ready(c));
     FGPE_RET(prologue<T>(c));
-    FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, nullptr));
+    FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, nullptr, true));
     XArgs<T> a = base_xargs<T>(c, t);
     a.scale = c->scal + SC_SCALE;
     a.hmu = (double2*)hmu;
@@ -479,6 +482,7 @@ int adj_check(fgpe_ctx* c)
     if (c->hp.quench_on) return fail(FGPE_EUNSUP, "the adjoint needs a constant a (no a(t) ramp)");
     if (c->hp.sc_U != 0.0) return fail(FGPE_EUNSUP, "the adjoint supports the dipolar k-space kernel only");
     if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
+    if (c->hp.vpx[0] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
     return FGPE_OK;
 }
 
@@ -789,7 +793,7 @@ template <typename T> GroupTabs group_tabs(fgpe_group* g, double dt, double beta
         const int* stop = timing ? nullptr : c->flags + FL_STOP;
         const StridedArgs sf = strided_args(c, c->spec, 1, scale, c->nch, 1, true, stop);
         const LastArgs l = last_args(c, c->spec, la, c->nch, 1, la == 2 ? nullptr : scale, has_filter(c->hp), stop,
-                                     true);
+                                     true, !timing);
         const StridedArgs si = strided_args(c, c->spec, 1, nullptr, c->nch, 1, true, stop);
         XArgs<T> x;
         if (timing) {

@@ -273,6 +273,7 @@ struct LastArgs {
     double inv_ntot;    // 1 / Ntot
     double c_dd;        // 12 pi a_dd (0: no dipolar term)
     const void* wtab;   // T table of Vt (complex-layout spatial index) or null
+    double vpx[2];      // moving-frame factor vpx_c / Ntot per component (0: off)
 };
 
 // 12 pi a_dd ((k.d)^2/|k|^2 - 1/3), 0 for |k| < 1e-6 (src/parallel.cu:336-349)
@@ -281,6 +282,16 @@ __device__ __forceinline__ double vdd_of(double c_dd, double num, double k2)
     const double den = sqrt(k2);
     if (den < 1E-6) return 0.0;
     return c_dd * (((num / den) * (num / den)) - 1.0 / 3.0);
+}
+
+// The moving-frame factor of complex channel ch (0 or 1).  Blended
+// arithmetically: an indexed or selected field would be turned into a load
+// through a computed address, which keeps the whole argument struct in
+// scratch memory.
+__device__ __forceinline__ double vpx_of(const LastArgs& a, int ch)
+{
+    const double v0 = a.vpx[0], v1 = a.vpx[1];
+    return v0 + (double)(ch != 0) * (v1 - v0);
 }
 
 // The k-space multiply of the FP64 form.
@@ -292,11 +303,13 @@ __device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastA
     const double kx = kX[at.ix], ko = kO[at.outer];
     const double kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
+        const double w = vpx_of(a, at.ch) * kx;   // moving frame: - vpx k_x / N
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const double ka = kA[elemB<N, V>(t, q)];
             const double k2 = kxo2 + ka * ka;
-            v[q] = double2{(a.lap_scale * v[q].x) * k2, (a.lap_scale * v[q].y) * k2};
+            // w = 0 leaves (lap_scale v) k2 unchanged (a zero product is exact)
+            v[q] = double2{(a.lap_scale * v[q].x) * k2 - w * v[q].x, (a.lap_scale * v[q].y) * k2 - w * v[q].y};
         }
         return;
     }
@@ -342,10 +355,11 @@ __device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastAr
     const float kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
         const float ls = (float)a.lap_scale;
+        const float w = (float)(vpx_of(a, at.ch) * kX[at.ix]);   // moving frame: - vpx k_x / N
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const float ka = (float)kA[elemB<N, V>(t, q)];
-            const float f = ls * (kxo2 + ka * ka);
+            const float f = ls * (kxo2 + ka * ka) - w;
             v[q] = float2{v[q].x * f, v[q].y * f};
         }
         return;

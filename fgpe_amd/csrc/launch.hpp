@@ -52,10 +52,10 @@ template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int a
 
 template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
                                                const double* scale, bool filter, const int* stop, bool half,
-                                               const void* tab, int ninst, int slab, int nslab)
+                                               const void* tab, int ninst, int slab, int nslab, bool vpx)
 {
     constexpr int P = ps_of<T, N, V>();
-    LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half);
+    LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half, vpx);
     a.s = slab_of(a.s, slab, nslab);
     if (a.s.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a.s), (unsigned)ninst), block(P * Plan<N, V>::T);
@@ -170,10 +170,10 @@ template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, 
 
 template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int nch, int ncomp,
                                           const double* scale, bool filter, const int* stop, bool half,
-                                          const void* tab, int ninst, int slab, int nslab)
+                                          const void* tab, int ninst, int slab, int nslab, bool vpx)
 {
 #define CALL(N_, V_) \
-    detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half, tab, ninst, slab, nslab)
+    detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half, tab, ninst, slab, nslab, vpx)
     FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis], CALL)
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported last axis length");

@@ -16,10 +16,9 @@ BEConeComponent::BEConeComponent(Params& par, int precision, bool do_run) : OneC
     hparams.ncomp = 1;
     hparams.quench_on = 1;
     hparams.quench[0] = p.a;
-    if (p.getBool("trotate")) {
-        std::cerr << "#ERROR  trotate (the moving-frame -v.p term) is not available in this build" << std::endl;
-        exit(1);
-    }
+    // moving frame: alg_addVPx with 2 pi omega / (omho aho)
+    // (hamiltonians/BEC_oneComponent.cu:98-101)
+    if (p.getBool("trotate")) hparams.vpx[0] = 2 * pi * p.getDouble("omega") / (p.omho * p.aho);
     if (do_run) run();
 }
 

@@ -13,9 +13,10 @@ BECtwoComponent::BECtwoComponent(Params2& par, int precision) : TwoComponentGPSo
     hparams.a[0] = p.a11;
     hparams.a[1] = p.a22;
     hparams.a[2] = p.a12;
+    // moving frame per component (hamiltonians/BEC_twoComponent.cu:125-129)
     if (p.getBool("trotate")) {
-        std::cerr << "#ERROR  trotate (the moving-frame -v.p term) is not available in this build" << std::endl;
-        exit(1);
+        hparams.vpx[0] = 2 * pi * p.getDouble("omega") / (p.omho1 * p.aho1);
+        hparams.vpx[1] = 2 * pi * p.getDouble("omega") / (p.omho2 * p.aho2);
     }
     if (p.getBool("treal")) {
         std::cout << "# Running real time evolution" << std::endl;

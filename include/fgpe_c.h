@@ -62,6 +62,16 @@ typedef struct fgpe_hparams {
     double sc_U;        /* soft-core strength; 0 disables                      */
     double sc_R;        /* soft-core radius                                    */
     int sc_dim;         /* 1, 2 or 3: which soft-core transform                */
+    double vpx[2];      /* moving-frame term -vpx_c p_x / (2 pi) per component,
+                           i.e. -vpx_c F^-1[k_x F(psi_c)] / N (alg_addVPx,
+                           src/baseHamiltonian.cu:198-211; trotate:
+                           vpx = 2 pi omega / (omho aho), BEC_oneComponent.cu:98-101,
+                           BEC_twoComponent.cu:125-129); 0 disables.  It enters
+                           where the reference's alg_calcHpsi does: heavy ball,
+                           observables and H applications, and only stage 4 of
+                           the one-component RK4 step (stages 1-3 use
+                           alg_calcHpsiMU, src/oneComponentGPSolver.cu:170-187;
+                           the two-component RK4 uses only alg_calcHpsiMU). */
 } fgpe_hparams;
 
 /* ---- context (replaces Cuda::Cuda, src/parallel.cu:14-113) ------------- */

@@ -320,8 +320,34 @@ double orc_quench_a(double time, const double prms[6])
    Imag(V) of component 2 is V2.y (the reference's BECHamiltonian_2 reads
    V1.y at :208, which only matters for complex traps).
    Phi (the dipolar / soft-core term) acts on the total density. */
+/* The moving-frame term of alg_addVPx (src/baseHamiltonian.cu:198-211):
+   out += (-vpx / N) F^-1[ k_x F(psi) ]  (BECVPx, src/parallel.cu:283-292;
+   AppendArray, hpp/parallel.hpp:231-237). */
+static void add_vpx(const orc_grid *g, double vpx, const double *psi, double *out)
+{
+    long long N = (long long)g->nx * g->ny * g->nz;
+    double *aux = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(aux, psi, sizeof(double) * 2 * N);
+    orc_fft3d(g->nx, g->ny, g->nz, aux, -1);
+    for (long long idx = 0; idx < N; ++idx) {
+        double kx = g->kx[idx % g->nx];
+        aux[2 * idx] *= kx;
+        aux[2 * idx + 1] *= kx;
+    }
+    orc_fft3d(g->nx, g->ny, g->nz, aux, +1);
+    double f = -vpx / (double)N;
+    for (long long i = 0; i < 2 * N; ++i) out[i] += f * aux[i];
+    free(aux);
+}
+
 void orc_hpsi(const orc_grid *g, const orc_ham *h, double t,
               const double *psi, const double *V, double *hmu, double *hen)
+{
+    orc_hpsi_vpx(g, h, t, psi, V, hmu, hen, hen != NULL);
+}
+
+void orc_hpsi_vpx(const orc_grid *g, const orc_ham *h, double t, const double *psi,
+                  const double *V, double *hmu, double *hen, int vpx)
 {
     long long N = (long long)g->nx * g->ny * g->nz;
     int nc = h->ncomp;
@@ -341,7 +367,12 @@ void orc_hpsi(const orc_grid *g, const orc_ham *h, double t,
         free(dens);
     }
     double *ke = (double *)malloc(sizeof(double) * 2 * N * nc);
-    for (int c = 0; c < nc; ++c) orc_laplace(g, psi + 2 * N * c, ke + 2 * N * c);
+    for (int c = 0; c < nc; ++c) {
+        orc_laplace(g, psi + 2 * N * c, ke + 2 * N * c);
+        /* the term lands in both H and the energy operator, like the
+           reference's AppendArray on d_hpsi and d_hpsi_en */
+        if (vpx && h->vpx[c] != 0.0) add_vpx(g, h->vpx[c], psi + 2 * N * c, ke + 2 * N * c);
+    }
 
     if (nc == 1) {
         double a = h->quench_on ? orc_quench_a(t, h->quench) : h->a[0];
@@ -474,7 +505,11 @@ void orc_rk4(const orc_grid *g, const orc_ham *h, double *psi, const double *V,
         rk_update(n2, psi, old, hp, fr, fi);
 
         t = t0 + (it + 1) * dt;
-        orc_hpsi(g, h, t, psi, V, hp, NULL);
+        /* stage 4 of the one-component step is alg_calcHpsi (with the
+           moving-frame term), stages 1-3 alg_calcHpsiMU
+           (src/oneComponentGPSolver.cu:170-187); two components use the MU
+           form throughout (src/twoComponentGPSolver.cu:157-175) */
+        orc_hpsi_vpx(g, h, t, psi, V, hp, NULL, h->ncomp == 1);
         for (long long i = 0; i < n2; ++i) {
             double aX = k[2 * i] + hp[2 * i];
             double aY = k[2 * i + 1] + hp[2 * i + 1];

@@ -28,6 +28,7 @@ typedef struct orc_ham {
     int    quench_on;        /* 1: a(t) from the d_quench_as ramp              */
     double quench[6];        /* a0, a1, a2, t1, t2, t3                          */
     const double *vtilde;    /* real k-space kernel Vt(k) (N doubles) or NULL   */
+    double vpx[2];           /* moving-frame factor per component (0: off)      */
 } orc_ham;
 
 /* ---- building blocks ---------------------------------------------------- */
@@ -43,9 +44,13 @@ double orc_quench_a(double t, const double prms[6]);
 
 /* ---- Hamiltonian application ------------------------------------------- */
 /* psi, V, hmu, hen: ncomp consecutive complex arrays of N points each.
-   hen may be NULL (the "MU" kernels). */
+   hen may be NULL (the "MU" kernels, alg_calcHpsiMU: no moving-frame term);
+   with hen the full form (alg_calcHpsi: with it). */
 void   orc_hpsi(const orc_grid *g, const orc_ham *h, double t,
                 const double *psi, const double *V, double *hmu, double *hen);
+/* The same with the moving-frame term chosen explicitly. */
+void   orc_hpsi_vpx(const orc_grid *g, const orc_ham *h, double t, const double *psi,
+                    const double *V, double *hmu, double *hen, int vpx);
 
 /* ---- reductions --------------------------------------------------------- */
 double orc_norm(const orc_grid *g, const double *psi);

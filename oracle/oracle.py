@@ -30,7 +30,7 @@ class OrcGrid(C.Structure):
 
 class OrcHam(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
-                ("quench", C.c_double * 6), ("vtilde", _dp)]
+                ("quench", C.c_double * 6), ("vtilde", _dp), ("vpx", C.c_double * 2)]
 
 
 def build():
@@ -108,14 +108,16 @@ class Grid:
 
 
 class Ham:
-    def __init__(self, ncomp=1, a=(0.0, 0.0, 0.0), quench=None, vtilde=None):
+    def __init__(self, ncomp=1, a=(0.0, 0.0, 0.0), quench=None, vtilde=None, vpx=(0.0, 0.0)):
         self.ncomp = ncomp
         self.vtilde = None if vtilde is None else np.ascontiguousarray(vtilde, dtype=np.float64)
         q = (0.0,) * 6 if quench is None else tuple(quench)
         aa = tuple(a) + (0.0,) * (3 - len(a))
+        vv = tuple(vpx) + (0.0,) * (2 - len(vpx))
         self._s = OrcHam(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
                          (C.c_double * 6)(*q),
-                         _p(self.vtilde) if self.vtilde is not None else _dp())
+                         _p(self.vtilde) if self.vtilde is not None else _dp(),
+                         (C.c_double * 2)(*vv))
 
 
 def kaxis(n, dk, kmax, active=True):

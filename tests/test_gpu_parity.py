@@ -84,6 +84,7 @@ HAMS = {
     "dipolar": dict(a=(1.2,), a_dd=1.5, dip=(0.0, 0.6, 0.8)),
     "softcore": dict(a=(0.4,), sc_U=5.0, sc_R=1.5, sc_dim=3),
     "dip+sc": dict(a=(0.4,), a_dd=0.6, dip=(0.0, 0.0, 1.0), sc_U=2.0, sc_R=1.2, sc_dim=3),
+    "moving": dict(a=(0.7,), a_dd=0.5, dip=(0.0, 0.0, 1.0), vpx=(1.3,)),
 }
 
 
@@ -92,7 +93,7 @@ def oracle_ham(c, h):
     if h.get("a_dd") or h.get("sc_U"):
         vt = c.vtilde(h.get("a_dd", 0.0), h.get("dip", (0, 0, 1)),
                       (h.get("sc_dim", 3), h.get("sc_R"), h.get("sc_U")) if h.get("sc_U") else None)
-    return orc.Ham(1, h["a"], quench=h.get("quench"), vtilde=vt)
+    return orc.Ham(1, h["a"], quench=h.get("quench"), vtilde=vt, vpx=h.get("vpx", (0.0, 0.0)))
 
 
 @pytest.mark.parametrize("name", list(HAMS))
@@ -139,7 +140,7 @@ def run_both(c, h, psi, V, dt, nsteps, precision=0, imaginary=False):
     return gpu, obs
 
 
-@pytest.mark.parametrize("name", ["contact", "dipolar", "softcore", "quench"])
+@pytest.mark.parametrize("name", ["contact", "dipolar", "softcore", "quench", "moving"])
 def test_rk4_real_time_f64(name):
     h = dict(HAMS[name])
     c = Case(32, 32, 64, L=(10.0, 10.0, 20.0))
@@ -216,6 +217,30 @@ def test_heavy_ball_one_component_f64():
     assert np.abs(hist - href).max() <= 1e-11 * np.abs(href).max()
 
 
+def test_heavy_ball_moving_frame_f64():
+    """Heavy ball with the moving-frame term (alg_calcHpsi applies it every
+    iteration): same converging iteration, state and history as the oracle."""
+    from fgpe_amd import FGPE_IMAG_HEAVY_BALL, configs
+    c = Case(32, 32, 32, L=(12.0, 12.0, 12.0))
+    h = dict(a=(0.05,), a_dd=0.03, dip=(0.0, 0.0, 1.0), vpx=(0.8,))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 1.3))
+    psi = configs.gaussian_noisy(c.spec, 6, width=1.2)
+    dt, beta, npart, eps, n = 2e-3, 0.9, 1.0, 1e-7, 3000
+    ref, nref, href = orc.imag_heavy_ball(c.grid, oracle_ham(c, h), psi, V, dt, beta, npart, eps, n)
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        done, hist = ctx.imag_steps(dt, n, scheme=FGPE_IMAG_HEAVY_BALL, beta=beta, npart=npart,
+                                    eps=eps, history=True)
+        gpu = ctx.download_psi()
+    assert nref < n
+    assert done == nref, (done, nref)
+    err = rel_max(gpu, ref)
+    assert err < 1e-11, err
+    assert np.abs(hist - href).max() <= 1e-11 * np.abs(href).max()
+
+
 def two_comp_case():
     from fgpe_amd import configs
     c = Case(32, 32, 64, L=(12.0, 12.0, 16.0))
@@ -229,16 +254,18 @@ def two_comp_ham(c, h):
     vt = None
     if h.get("sc_U"):
         vt = c.vtilde(0.0, sc=(h["sc_dim"], h["sc_R"], h["sc_U"]))
-    return orc.Ham(2, h["a"], vtilde=vt)
+    return orc.Ham(2, h["a"], vtilde=vt, vpx=h.get("vpx", (0.0, 0.0)))
 
 
-@pytest.mark.parametrize("name", ["contact", "softcore"])
+@pytest.mark.parametrize("name", ["contact", "softcore", "moving"])
 def test_two_component_rk4_f64(name):
     c, V, psi = two_comp_case()
     f = 1.0 / (4 * np.pi)
     h = dict(ncomp=2, a=(100 * f, 100 * f, 80 * f))
     if name == "softcore":
         h.update(sc_U=5.0, sc_R=1.5, sc_dim=3)
+    if name == "moving":   # two-component RK4 uses the MU form only: no moving-frame term
+        h.update(vpx=(1.1, -0.6))
     dt, n = 5e-4, 10
     with c.ctx() as ctx:
         ctx.set_hamiltonian(**h)

@@ -191,6 +191,62 @@ def test_hpsi_plane_wave_closed_form():
     assert np.abs(hen - (ke + v0 + 2 * np.pi * a * A * A) * psi).max() < 1e-14
 
 
+def test_hpsi_moving_frame_plane_wave():
+    """-vpx F^-1[k_x F psi]/N on a plane wave e^{2 pi i m.x/L} is -vpx (m_x/L) psi
+    in the full form (alg_calcHpsi) and absent from the MU form."""
+    L = (8.0, 8.0, 8.0)
+    g = grid(8, 8, 8, L)
+    X, Y, Z = coords(g, L)
+    m = (3, -1, 2)
+    psi = 0.5 * np.exp(2j * np.pi * (m[0] * X + m[1] * Y + m[2] * Z) / 8.0)
+    V = np.zeros(g.shape, complex)
+    lam = 1.7
+    h = orc.Ham(1, (0.0,), vpx=(lam,))
+    hmu, hen = orc.hpsi(g, h, 0.0, psi, V)
+    ke = LIT * (9 + 1 + 4) / 64.0
+    assert np.abs(hmu - (ke - lam * 3 / 8.0) * psi).max() < 1e-13
+    assert np.abs(hen - (ke - lam * 3 / 8.0) * psi).max() < 1e-13
+    mu_only = orc.hpsi(g, h, 0.0, psi, V, energy=False)
+    assert np.abs(mu_only - ke * psi).max() < 1e-13
+    # two components: each its own factor
+    h2 = orc.Ham(2, (0.0, 0.0, 0.0), vpx=(lam, -0.4))
+    two = np.stack([psi, 2.0 * psi])
+    hm2, he2 = orc.hpsi(g, h2, 0.0, two, np.stack([V, V]))
+    assert np.abs(hm2[0] - (ke - lam * 3 / 8.0) * psi).max() < 1e-13
+    assert np.abs(hm2[1] - (ke + 0.4 * 3 / 8.0) * 2.0 * psi).max() < 1e-13
+
+
+def test_rk4_moving_frame_enters_stage_four_only():
+    """One component: stages 1-3 apply the MU form (eps0 = kinetic), stage 4
+    the full form (eps1 = eps0 - vpx k_x); on a plane wave the step multiplies
+    by 1 + (-i dt / 6)(eps0 (1 + 2 a1 + 2 a2) + eps1 a3), a1 = 1 + z/2,
+    a2 = 1 + z a1 / 2, a3 = 1 + z a2, z = -i dt eps0.  Two components never
+    apply it in RK4."""
+    L = (8.0, 8.0, 8.0)
+    g = grid(8, 8, 8, L)
+    X, Y, Z = coords(g, L)
+    psi = np.exp(2j * np.pi * (2 * X - Y) / 8.0)
+    V = np.zeros(g.shape, complex)
+    dt, n, lam = 0.01, 4, 3.0
+    e0 = LIT * 5 / 64.0
+    e1 = e0 - lam * 2 / 8.0
+    z = -1j * dt * e0
+    a1 = 1 + z / 2
+    a2 = 1 + z * a1 / 2
+    a3 = 1 + z * a2
+    A = 1 + (-1j * dt / 6) * (e0 * (1 + 2 * a1 + 2 * a2) + e1 * a3)
+    out = orc.rk4(g, orc.Ham(1, (0.0,), vpx=(lam,)), psi, V, dt, n)
+    assert np.abs(out - A ** n * psi).max() < 1e-13
+    # without the quirk the amplification would be the full-form RK4 polynomial
+    zf = -1j * dt * e1
+    full = 1 + zf + zf ** 2 / 2 + zf ** 3 / 6 + zf ** 4 / 24
+    assert np.abs(out - full ** n * psi).max() > 1e-6
+    two = orc.rk4(g, orc.Ham(2, (0.0, 0.0, 0.0), vpx=(lam, lam)), np.stack([psi, psi]),
+                  np.stack([V, V]), dt, n)
+    R0 = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert np.abs(two[0] - R0 ** n * psi).max() < 1e-13
+
+
 def test_quench_ramp():
     prms = (1.0, 2.0, 3.0, 1.0, 2.0, 4.0)
     assert orc.quench_a(0.0, prms) == 3.0      # t <= 0 falls through to a2
