@@ -76,9 +76,11 @@ def run_form(torch, w, psi, V, precision, steps, warmup):
     out = ctx.download_psi()
     chosen = {k: v for k, v in ctx.variants().items() if k[1] == {"xstage": 0, "strided": 1, "last": 2}[k[0]]}
     nslab = ctx.slabs()
+    order = ctx.order()
     ctx.close()
     return ms / steps, kt, obs, out, {"autotune_ms": {k: [round(x, 4) for x in v] for k, v in tune.items()},
-                                      "variant": {k[0]: v for k, v in chosen.items()}, "slabs": nslab}
+                                      "variant": {k[0]: v for k, v in chosen.items()}, "slabs": nslab,
+                                      "order": order}
 
 
 def cpu_baseline(w, psi, V, nsteps, threads):

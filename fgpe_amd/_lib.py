@@ -27,7 +27,7 @@ EXPORTS = (
     "fgpe_observables", "fgpe_adjoint_forward", "fgpe_adjoint_backward", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
     "fgpe_timing_enable",
     "fgpe_timing_read", "fgpe_autotune", "fgpe_set_variant", "fgpe_get_variant", "fgpe_set_slabs",
-    "fgpe_get_slabs",
+    "fgpe_get_slabs", "fgpe_set_order", "fgpe_get_order",
     "fgpe_group_create", "fgpe_group_destroy", "fgpe_group_autotune", "fgpe_group_imag_steps",
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
 )
@@ -87,6 +87,8 @@ def load() -> C.CDLL:
     L.fgpe_get_variant.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
     L.fgpe_set_slabs.argtypes = [_vp, C.c_int]
     L.fgpe_get_slabs.argtypes = [_vp, C.POINTER(C.c_int)]
+    L.fgpe_set_order.argtypes = [_vp, C.c_int]
+    L.fgpe_get_order.argtypes = [_vp, C.POINTER(C.c_int)]
     L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
     L.fgpe_timing_read.argtypes = [_vp, _dp]
     _ip = C.POINTER(C.c_int)
@@ -262,6 +264,14 @@ class Context:
     def slabs(self) -> int:
         v = C.c_int()
         check(load().fgpe_get_slabs(self.h, C.byref(v)))
+        return v.value
+
+    def set_order(self, mask: int):
+        check(load().fgpe_set_order(self.h, mask))
+
+    def order(self) -> int:
+        v = C.c_int()
+        check(load().fgpe_get_order(self.h, C.byref(v)))
         return v.value
 
     def set_variant(self, kclass: int, axis: int, variant: int):

@@ -99,6 +99,7 @@ struct fgpe_ctx {
     void* tw32[3] = {nullptr, nullptr, nullptr};
     int var[FGPE_NKCLASS][3] = {};   // plan variant per kernel class and axis
     int nslab = 1;                   // column slabs of the y / last / y pass sequence
+    int order = 0;                   // bit k set: pass k (y fwd, last, y inv, x) runs its blocks reversed
 
     // launch timing (fgpe_timing_enable)
     bool timing = false;

@@ -726,6 +726,33 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         }
         c->nslab = bests;
     }
+    // block order of the four passes of a stage (reversed or not per pass),
+    // timed the same way; keeps the fastest of the 16 combinations
+    if (rc == FGPE_OK) {
+        double best = 1e300;
+        int besto = 0;
+        auto stage = [&]() -> int {
+            FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
+            return run(FGPE_KCLASS_XSTAGE);
+        };
+        for (int o = 0; o < 16 && rc == FGPE_OK; ++o) {
+            if (la != 2 && (o & 5)) continue;   // 2D grids have no y passes
+            c->order = o;
+            rc = stage();   // warm-up
+            if (rc != FGPE_OK) break;
+            (void)hipEventRecord(e0, c->stream);
+            for (int i = 0; i < reps && rc == FGPE_OK; ++i) rc = stage();
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms / reps < best) {
+                best = ms / reps;
+                besto = o;
+            }
+        }
+        c->order = besto;
+    }
     c->timing = timing;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1301,6 +1328,21 @@ int fgpe_get_variant(fgpe_ctx* c, int kclass, int axis, int* variant)
     if (kclass < 0 || kclass >= FGPE_NKCLASS || axis < 0 || axis > 2)
         return fail(FGPE_EINVAL, "kclass or axis out of range");
     *variant = c->var[kclass][axis];
+    return FGPE_OK;
+}
+
+int fgpe_set_order(fgpe_ctx* c, int mask)
+{
+    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (mask < 0 || mask > 15) return fail(FGPE_EINVAL, "order mask must be in 0..15");
+    c->order = mask;
+    return FGPE_OK;
+}
+
+int fgpe_get_order(fgpe_ctx* c, int* mask)
+{
+    if (!c || !mask) return fail(FGPE_EINVAL, "NULL argument");
+    *mask = c->order;
     return FGPE_OK;
 }
 

@@ -161,6 +161,7 @@ struct StridedArgs {
     long long hchan;        //              elements per real channel
     const double* scale;    // per-component pending scale (forward only) or null
     const int* stop;        // nonzero: skip (a converged imaginary-time loop)
+    int rev;                // 1: blocks take the tiles in reverse order
 };
 
 // Blocks of one strided launch: complex-layout channels first, then the
@@ -190,8 +191,11 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     const int ntf = a.ncols / P;
     const int nth = a.hcols ? (a.hcols + P - 1) / P : 1;
     const long long nbf = (long long)(a.hcols ? a.ncomp : a.nchan) * ntf * a.nouter;
-    const bool half = (long long)blockIdx.x >= nbf;
-    const long long b = half ? (long long)blockIdx.x - nbf : (long long)blockIdx.x;
+    // the tile this block takes: in launch order, or reversed so that the pass
+    // starts on what the previous pass wrote last (still in the memory-side cache)
+    const long long bx = a.rev ? (long long)gridDim.x - 1 - blockIdx.x : (long long)blockIdx.x;
+    const bool half = bx >= nbf;
+    const long long b = half ? bx - nbf : bx;
     const int nt = half ? nth : ntf;
     const long long per = (long long)nt * a.nouter;
     const int cl = (int)(b / per);            // channel within its layout
@@ -469,6 +473,7 @@ template <typename T> struct XArgs {
     double dt, beta;        // HB
     const double* mu;       // HB: mu_c of the previous iteration (null: 0)
     const int* stop;        // nonzero: the loop has converged, do nothing
+    int rev;                // 1: blocks take the row groups in reverse order
 };
 
 // Minimum waves per SIMD the register allocator targets for an x-stage
@@ -499,9 +504,12 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
     const gp<const T> Vr[2] = {glob(a.Vr[0]), glob(a.Vr[1])};
     const gp<const T> Vi[2] = {glob(a.Vi[0]), glob(a.Vi[1])};
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
-    const long long row0 = ((long long)blockIdx.x * P + p) * N;
+    // row group of this block (reversed: the kernel starts on what the passes
+    // before it wrote last); partial sums stay indexed by the row group
+    const unsigned bx = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const long long row0 = ((long long)bx * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
-    const gp<T2> specd = glob(a.specd) + ((long long)blockIdx.x * P + p) * a.dpitch;   // this row's half spectrum
+    const gp<T2> specd = glob(a.specd) + ((long long)bx * P + p) * a.dpitch;   // this row's half spectrum
 
     T2 v[E];
     T nd[E];             // density of the new state (DENS)
@@ -723,7 +731,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
             }
         }
         if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm))
-            block_sum_store<NV>(sums, glob(a.partials) + (long long)NV * blockIdx.x);
+            block_sum_store<NV>(sums, glob(a.partials) + (long long)NV * bx);
         if (MODE == MODE_APPLY) return;
     }
     if (DENS) {

@@ -38,6 +38,7 @@ template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int a
 {
     constexpr int P = ps_of<T, N, V>();
     StridedArgs a = slab_of(strided_args(c, data, axis, scale, nch, ncomp, half, stop), slab, nslab);
+    a.rev = (c->order >> (dir < 0 ? 0 : 2)) & 1;
     if (a.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a), (unsigned)ninst), block(P * Plan<N, V>::T);
     const StridedArgs* t = (const StridedArgs*)tab;
@@ -57,6 +58,7 @@ template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis
     constexpr int P = ps_of<T, N, V>();
     LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half, vpx);
     a.s = slab_of(a.s, slab, nslab);
+    a.s.rev = (c->order >> 1) & 1;
     if (a.s.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a.s), (unsigned)ninst), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_LAST);
@@ -72,8 +74,10 @@ int xstage_t(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
     const long long rows = c->ntot / N;
     if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
     dim3 grid((unsigned)(rows / P), (unsigned)ninst), block(P * Plan<N, V>::T);
+    XArgs<T> b = a;
+    b.rev = (c->order >> 3) & 1;
     TScope ts(c, FGPE_KCLASS_XSTAGE);
-    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, a, tw_of<T>(c, 0),
+    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, b, tw_of<T>(c, 0),
                        (const XArgs<T>*)tab);
     FGPE_LAUNCHCHK("k_xstage");
     return FGPE_OK;

@@ -198,6 +198,13 @@ int fgpe_get_variant(fgpe_ctx* ctx, int kclass, int axis, int* variant);
 /* Column slabs of the y / last / y sequence: 1, 2 or 4 (nx divisible). */
 int fgpe_set_slabs(fgpe_ctx* ctx, int nslab);
 int fgpe_get_slabs(fgpe_ctx* ctx, int* nslab);
+/* Block order of the passes of a stage: bit 0 y forward, bit 1 last axis,
+   bit 2 y inverse, bit 3 x stage; a set bit runs that pass's blocks in
+   reverse so it starts on what the pass before it wrote last.  Results do
+   not depend on it.  fgpe_autotune picks it last (16 combinations timed as
+   whole RK stages). */
+int fgpe_set_order(fgpe_ctx* ctx, int mask);
+int fgpe_get_order(fgpe_ctx* ctx, int* mask);
 
 /* ---- batched sweeps (no counterpart in the reference, which runs one
    instance per process: src/oneComponentGPSolver.cu:26-104) ----------------
