@@ -164,8 +164,8 @@ struct StridedArgs {
     int rev;                // 1: blocks take the tiles in reverse order
 };
 
-// Blocks of one strided launch: complex-layout channels first, then the
-// half-layout ones.
+// Blocks of one strided launch: per outer index the complex-layout tiles,
+// then the half-layout ones.
 template <int P> __host__ __device__ inline long long strided_blocks(const StridedArgs& a)
 {
     const long long ntf = a.ncols / P;
@@ -185,23 +185,27 @@ struct TileAt {
 
 template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArgs& a, int p)
 {
-    // complex-layout channels first, then the half-layout ones (selects, not
-    // branches: every value is block-uniform)
+    // Tiles are enumerated outer index by outer index, all channels of one
+    // outer index together (complex-layout tiles, then half-layout ones):
+    // the x stage writes both channels of a row together, so a pass that
+    // starts where it ended finds both in the memory-side cache.  Selects,
+    // not branches: every value is block-uniform.
     TileAt r;
     const int ntf = a.ncols / P;
     const int nth = a.hcols ? (a.hcols + P - 1) / P : 1;
-    const long long nbf = (long long)(a.hcols ? a.ncomp : a.nchan) * ntf * a.nouter;
+    const int nfc = a.hcols ? a.ncomp : a.nchan;          // complex-layout channels
+    const long long nfo = (long long)nfc * ntf;           // their tiles per outer index
+    const long long per = nfo + (a.hcols ? (long long)(a.nchan - a.ncomp) * nth : 0);
     // the tile this block takes: in launch order, or reversed so that the pass
     // starts on what the previous pass wrote last (still in the memory-side cache)
     const long long bx = a.rev ? (long long)gridDim.x - 1 - blockIdx.x : (long long)blockIdx.x;
-    const bool half = bx >= nbf;
-    const long long b = half ? bx - nbf : bx;
+    r.outer = bx / per;
+    const long long w = bx - r.outer * per;
+    const bool half = w >= nfo;
+    const long long wl = half ? w - nfo : w;
     const int nt = half ? nth : ntf;
-    const long long per = (long long)nt * a.nouter;
-    const int cl = (int)(b / per);            // channel within its layout
-    const long long rem = b - (long long)cl * per;
-    const int xt = (int)(rem % nt);
-    r.outer = rem / nt;
+    const int cl = (int)(wl / nt);            // channel within its layout
+    const int xt = (int)(wl - (long long)cl * nt);
     r.ch = half ? a.ncomp + cl : cl;
     const int loc = xt * P + p;   // column within this launch
     r.valid = !half || loc < a.hcols;
