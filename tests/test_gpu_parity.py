@@ -452,3 +452,29 @@ def test_block_order_is_bitwise_neutral(precision):
         print(f"order {mask}: max |psi - psi(order 0)| = {err:.3e}")
         assert err == 0.0, (mask, err)
         assert np.array_equal(outs[mask][1], outs[0][1]), (mask, outs[mask][1], outs[0][1])
+
+
+@pytest.mark.parametrize("shape", [(96, 32, 64), (192, 64, 32), (512, 32, 32), (32, 96, 192),
+                                   (64, 64, 1), (96, 192, 1)])
+def test_rk4_dipolar_softcore_all_plans_f64(shape):
+    """Full RK4 steps through every pencil length's plan (radix-3 lengths, the
+    512 plan, 2D grids), with a tilted dipole (the half-spectrum density and
+    its Nyquist-plane symmetrisation) plus a soft-core term: 1e-12 against the
+    oracle."""
+    from fgpe_amd import configs
+    nx, ny, nz = shape
+    L = (12.0, 10.0, 14.0 if nz > 1 else 1.0)
+    c = Case(nx, ny, nz, L=L)
+    h = dict(a=(0.6,), a_dd=0.5, dip=(0.48, 0.6, 0.64) if nz > 1 else (0.6, 0.8, 0.0),
+             sc_U=1.5, sc_R=1.1, sc_dim=3 if nz > 1 else 2)
+    V = configs.harmonic_trap(c.spec, (1.0, 1.1, 0.9))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(V)) + 0.01 * c.rand_psi(56), c.dv)
+    dt, n = 5e-4, 4
+    gpu, obs = run_both(c, h, psi, V, dt, n)
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, dt, n)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, (shape, err)
+    hmu, hen = orc.hpsi(c.grid, oracle_ham(c, h), n * dt, ref, V)
+    nrm = orc.norm(c.grid, ref)
+    mu, en = orc.obs2(c.grid, nrm, ref, hmu, hen)
+    assert abs(obs[2] - en) < 1e-12 * abs(en), (shape, obs[2], en)
