@@ -59,6 +59,7 @@ struct fgpe_ctx {
     size_t bytes = 0;
 
     double* k[3] = {nullptr, nullptr, nullptr};
+    float2* k32[3] = {nullptr, nullptr, nullptr};  // the same axes in FP32 as (k[i], k[(n - i) % n])
     double dv = 0.0;
     bool kset = false;
 
@@ -197,6 +198,9 @@ inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int 
     a.kA = c->k[axis];
     a.kX = c->k[0];
     a.kO = c->k[oaxis];
+    a.kA32 = c->k32[axis];
+    a.kX32 = c->k32[0];
+    a.kO32 = c->k32[oaxis];
     const double* d = c->hp.dip;
     a.dA = d[axis];
     a.dX = d[0];

@@ -990,6 +990,7 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     const size_t N = (size_t)c->ntot;
     for (int a = 0; a < 3 && r == FGPE_OK; ++a) {
         r = dalloc(c, (void**)&c->k[a], sizeof(double) * c->n[a]);
+        if (r == FGPE_OK) r = dalloc(c, (void**)&c->k32[a], sizeof(float2) * c->n[a]);
         if (r == FGPE_OK) r = dalloc(c, &c->tw64[a], sizeof(double2) * c->n[a]);
         if (r == FGPE_OK) r = dalloc(c, &c->tw32[a], sizeof(float2) * c->n[a]);
         if (r == FGPE_OK) {
@@ -1029,7 +1030,7 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
 {
     if (!c) return FGPE_OK;
     (void)hipDeviceSynchronize();
-    void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
+    void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
                     c->aux, c->xbuf, c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
@@ -1069,8 +1070,13 @@ int fgpe_set_kspace(fgpe_ctx* c, const double* kx, const double* ky, const doubl
 {
     if (!c || !kx || !ky || !kz) return fail(FGPE_EINVAL, "NULL argument");
     const double* src[3] = {kx, ky, kz};
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         FGPE_HIPCHK(hipMemcpy(c->k[a], src[a], sizeof(double) * c->n[a], hipMemcpyHostToDevice));
+        const int n = c->n[a];
+        std::vector<float2> f(n);
+        for (int i = 0; i < n; ++i) f[i] = float2{(float)src[a][i], (float)src[a][(n - i) % n]};
+        FGPE_HIPCHK(hipMemcpy(c->k32[a], f.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    }
     c->dv = dv;
     c->kset = true;
     c->spec_valid = false;

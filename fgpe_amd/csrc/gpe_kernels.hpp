@@ -282,6 +282,9 @@ struct LastArgs {
     double c_dd;        // 12 pi a_dd (0: no dipolar term)
     const void* wtab;   // T table of Vt (complex-layout spatial index) or null
     double vpx[2];      // moving-frame factor vpx_c / Ntot per component (0: off)
+    const float2* kA32; // the three k axes rounded to FP32 as pairs (k[i], k[(n - i) % n])
+    const float2* kX32;
+    const float2* kO32;
 };
 
 // 12 pi a_dd ((k.d)^2/|k|^2 - 1/3), 0 for |k| < 1e-6 (src/parallel.cu:336-349)
@@ -345,61 +348,74 @@ __device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastA
     }
 }
 
-// The k-space multiply of the mixed-precision form: factors in FP32 (the
-// data is FP32), same formulas and the same Nyquist symmetrisation as the
-// FP64 form.
+// The k-space multiply of the mixed-precision form: factors in FP32 from
+// FP32 copies of the k axes (the data is FP32), same formulas and the same
+// Nyquist symmetrisation as the FP64 form, the partner test k' = -k made
+// on the FP32 values.
+// |k| < 1e-6 tested as k^2 < 1e-12; the quotient through the hardware
+// reciprocal (1 ulp), well inside the FP32 bulk's own rounding.
 __device__ __forceinline__ float vdd_of_f32(float c_dd, float num, float k2)
 {
-    if (sqrtf(k2) < 1E-6f) return 0.f;
-    return c_dd * ((num * num) / k2 - (1.f / 3.f));
+    if (k2 < 1E-12f) return 0.f;
+    return c_dd * ((num * num) * __builtin_amdgcn_rcpf(k2) - (1.f / 3.f));
 }
 
 template <int N, int V>
 __device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastArgs& a, const TileAt& at)
 {
     constexpr int E = Plan<N, V>::E;
-    const gp<const double> kA = glob(a.kA), kX = glob(a.kX), kO = glob(a.kO);
-    const float kx = (float)kX[at.ix], ko = (float)kO[at.outer];
+    const gp<const float2> kA = glob(a.kA32);
+    const float2 kxp2 = glob(a.kX32)[at.ix], kop2 = glob(a.kO32)[at.outer];
+    const float kx = kxp2.x, ko = kop2.x;
     const float kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
         const float ls = (float)a.lap_scale;
-        const float w = (float)(vpx_of(a, at.ch) * kX[at.ix]);   // moving frame: - vpx k_x / N
+        const float w = (float)vpx_of(a, at.ch) * kx;   // moving frame: - vpx k_x / N
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const float ka = (float)kA[elemB<N, V>(t, q)];
+            const float ka = kA[elemB<N, V>(t, q)].x;
             const float f = ls * (kxo2 + ka * ka) - w;
             v[q] = float2{v[q].x * f, v[q].y * f};
         }
         return;
     }
-    const gp<const float> wt = glob((const float*)a.wtab);
-    const int nxf = a.s.nx, nO = a.s.nouter;
-    const double kxpd = kX[(nxf - at.ix) % nxf], kopd = kO[(nO - (int)at.outer) % nO];
-    const bool xo_neg = (kxpd == -kX[at.ix]) && (kopd == -kO[at.outer]);
-    const float kxp = (float)kxpd, kop = (float)kopd;
+    // Vt rows are addressed from one base: every in-grid offset fits 32 bits.
+    const gp<const float> wt = (a.wtab && at.valid) ? glob((const float*)a.wtab) + at.sp_off : nullptr;
+    const float kxp = kxp2.y, kop = kop2.y;
+    const bool xo_neg = (kxp == -kx) && (kop == -ko);
     const float dA = (float)a.dA, dX = (float)a.dX, dO = (float)a.dO, c_dd = (float)a.c_dd;
     const float num0 = kx * dX + ko * dO, num0p = kxp * dX + kop * dO;
     const float kxo2p = kxp * kxp + kop * kop;
     const float inv = (float)a.inv_ntot;
+    const int st = (int)a.s.stride;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int ea = elemB<N, V>(t, q);
         float vt = 0.f;
         if (c_dd != 0.f) {
-            const double kad = kA[ea], kapd = kA[(N - ea) % N];
-            const float ka = (float)kad, kap = (float)kapd;
+            const float2 kk = kA[ea];
+            const float ka = kk.x, kap = kk.y;
             vt = vdd_of_f32(c_dd, num0 + ka * dA, kxo2 + ka * ka);
-            if (!(xo_neg && kapd == -kad))
+            if (!(xo_neg && kap == -ka))
                 vt = 0.5f * (vt + vdd_of_f32(c_dd, num0p + kap * dA, kxo2p + kap * kap));
         }
-        if (wt && at.valid) vt = vt + wt[at.sp_off + (long long)ea * a.s.stride];
+        if (wt) vt = vt + wt[ea * st];
         const float f = vt * inv;
         v[q] = float2{v[q].x * f, v[q].y * f};
     }
 }
 
+// Waves per SIMD the FP32 radix-16 last-axis pass is compiled for: four
+// (at most 128 VGPRs) keeps four blocks per CU in flight while the
+// register-resident pencil waits on HBM.
+template <typename T, int N, int V> constexpr int last_waves()
+{
+    return (sizeof(T) == 4 && N == 256 && V == 0) ? 4 : 1;
+}
+
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
+__attribute__((amdgpu_waves_per_eu(last_waves<T, N, V>())))
 k_last(const c2_t<T>* __restrict__ tw, LastArgs a0, const LastArgs* __restrict__ tab)
 {
     using T2 = c2_t<T>;
