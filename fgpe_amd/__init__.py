@@ -5,4 +5,5 @@ include/fgpe_c.h); this package holds its build recipe, a ctypes binding and
 the workload definitions.
 """
 from ._lib import (FGPE_IMAG_HEAVY_BALL, FGPE_IMAG_RK4_RENORM, FGPE_PREC_F64,  # noqa: F401
-                   FGPE_PREC_MIXED, Context, FgpeError, Graph, Group, load)
+                   FGPE_PREC_MIXED, INIT_HARMONIC, INIT_TUBE, INIT_VOID, LATTICES, Context, FgpeError, Geom,
+                   Graph, Group, load)

@@ -30,7 +30,12 @@ EXPORTS = (
     "fgpe_get_slabs", "fgpe_set_order", "fgpe_get_order",
     "fgpe_group_create", "fgpe_group_destroy", "fgpe_group_autotune", "fgpe_group_imag_steps",
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
+    "fgpe_psi_init", "fgpe_psi_add_noise", "fgpe_psi_imprint_vortex_tube", "fgpe_psi_force_lattice",
+    "fgpe_psi_normalize", "fgpe_vext_init", "fgpe_vext_init_box", "fgpe_vext_add_cos", "fgpe_vext_commit",
 )
+# initialiser kinds and lattices (fgpe_psi_init, fgpe_vext_init, fgpe_psi_force_lattice)
+INIT_HARMONIC, INIT_TUBE, INIT_VOID = 0, 1, 2
+LATTICES = ("1d", "2d_triangular", "2d_square", "3d_hex", "3d_fcc", "3d_bcc", "3d_sc")
 KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
 
 
@@ -38,6 +43,15 @@ class HParams(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
                 ("quench", C.c_double * 6), ("a_dd", C.c_double), ("dip", C.c_double * 3),
                 ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int), ("vpx", C.c_double * 2)]
+
+
+class Geom(C.Structure):
+    """Cell centres x_i = -xmax + i dx + dx/2 per axis."""
+    _fields_ = [("xmax", C.c_double * 3), ("dx", C.c_double * 3)]
+
+    @classmethod
+    def of(cls, xmax, dx):
+        return cls((C.c_double * 3)(*map(float, xmax)), (C.c_double * 3)(*map(float, dx)))
 
 
 class FgpeError(RuntimeError):
@@ -102,6 +116,16 @@ def load() -> C.CDLL:
                                           C.c_int, C.POINTER(_vp)]
     L.fgpe_graph_launch.argtypes = [_vp]
     L.fgpe_graph_destroy.argtypes = [_vp]
+    _gp = C.POINTER(Geom)
+    L.fgpe_psi_init.argtypes = [_vp, C.c_int, C.c_int, _gp, _dp, C.c_double, C.c_double, C.c_ulonglong]
+    L.fgpe_psi_add_noise.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_ulonglong, C.c_uint]
+    L.fgpe_psi_imprint_vortex_tube.argtypes = [_vp, C.c_int, _gp, C.c_double]
+    L.fgpe_psi_force_lattice.argtypes = [_vp, C.c_int, C.c_int, _gp, C.c_double, C.c_double, _dp, C.c_double]
+    L.fgpe_psi_normalize.argtypes = [_vp, C.c_int, C.c_double, _gp, _dp]
+    L.fgpe_vext_init.argtypes = [_vp, C.c_int, C.c_int, _gp, _dp]
+    L.fgpe_vext_init_box.argtypes = [_vp, C.c_int, _gp, C.c_double, C.c_double, _dp]
+    L.fgpe_vext_add_cos.argtypes = [_vp, C.c_int, _gp, C.c_double, C.c_double]
+    L.fgpe_vext_commit.argtypes = [_vp, C.c_int, _dp]
     _lib = L
     return L
 
@@ -200,6 +224,47 @@ class Context:
         a = a.reshape(-1)
         assert a.size == self.N
         check(load().fgpe_upload_vext(self.h, comp, _dptr(a)))
+
+    # ---- initial state and trap built on the device (fgpe_psi_*, fgpe_vext_*)
+    def init_psi(self, geom: Geom, kind=INIT_HARMONIC, ax=(1.0, 1.0, 1.0), rand=(0.0, 0.0), seed=0, comp=0):
+        axa = (C.c_double * 3)(*map(float, ax))
+        check(load().fgpe_psi_init(self.h, comp, kind, C.byref(geom), axa, float(rand[0]), float(rand[1]),
+                                   int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def psi_add_noise(self, amp=(0.0, 0.0), seed=0, stream=1, comp=0):
+        check(load().fgpe_psi_add_noise(self.h, comp, float(amp[0]), float(amp[1]),
+                                        int(seed) & 0xFFFFFFFFFFFFFFFF, int(stream)))
+
+    def psi_imprint_vortex_tube(self, geom: Geom, ncirc: float, comp=0):
+        check(load().fgpe_psi_imprint_vortex_tube(self.h, comp, C.byref(geom), float(ncirc)))
+
+    def psi_force_lattice(self, geom: Geom, lattice: str, amp: float, k: float, off=(0.0, 0.0, 0.0),
+                          npart=1.0, comp=0):
+        o = (C.c_double * 3)(*map(float, off))
+        check(load().fgpe_psi_force_lattice(self.h, comp, LATTICES.index(lattice), C.byref(geom), float(amp),
+                                            float(k), o, float(npart)))
+
+    def psi_normalize(self, geom: Geom, npart: float, comp=0) -> float:
+        out = C.c_double()
+        check(load().fgpe_psi_normalize(self.h, comp, float(npart), C.byref(geom), C.byref(out)))
+        return out.value
+
+    def vext_init(self, geom: Geom, kind=INIT_HARMONIC, ax=(1.0, 1.0, 1.0), comp=0):
+        axa = (C.c_double * 3)(*map(float, ax))
+        check(load().fgpe_vext_init(self.h, comp, kind, C.byref(geom), axa))
+
+    def vext_init_box(self, geom: Geom, V: float, B: float, w, comp=0):
+        wa = (C.c_double * 3)(*map(float, w))
+        check(load().fgpe_vext_init_box(self.h, comp, C.byref(geom), float(V), float(B), wa))
+
+    def vext_add_cos(self, geom: Geom, A: float, n: float, comp=0):
+        check(load().fgpe_vext_add_cos(self.h, comp, C.byref(geom), float(A), float(n)))
+
+    def vext_commit(self, comp=0, download=False):
+        """Make the assembled trap component comp's; returns it (N complex) if asked."""
+        out = np.empty(self.N, np.complex128) if download else None
+        check(load().fgpe_vext_commit(self.h, comp, _dptr(out) if download else None))
+        return out
 
     def psi_device_ptr(self, comp=0) -> int:
         p = _vp()

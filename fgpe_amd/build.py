@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("launch_f64.hip", "launch_f32.hip", "fgpe_capi.hip")]
+           ("launch_f64.hip", "launch_f32.hip", "fgpe_capi.hip", "fgpe_init.hip")]
 BUILD_DIR = os.path.join(HERE, "_build")
 
 

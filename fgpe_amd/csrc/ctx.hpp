@@ -95,6 +95,7 @@ struct fgpe_ctx {
     double adj_dt = 0.0, adj_t0 = 0.0;
     bool adj_ready = false;
     double* xbuf = nullptr;   // x cell centres of fgpe_calc_cos
+    double* vstage[2] = {nullptr, nullptr};   // FP64 trap being assembled (fgpe_vext_*)
 
     void* tw64[3] = {nullptr, nullptr, nullptr};
     void* tw32[3] = {nullptr, nullptr, nullptr};
@@ -117,6 +118,9 @@ namespace fgpe {
 // ---- launch timing: HIP events around each launch when enabled -------------
 long t_begin(fgpe_ctx* c);
 void t_end(fgpe_ctx* c, long k, int cls);
+// Device allocation charged to the context (fgpe_ctx_device_bytes).
+int dalloc(fgpe_ctx* c, void** p, size_t bytes);
+void dfree(fgpe_ctx* c, void** p, size_t bytes);
 struct TScope {
     fgpe_ctx* c;
     int cls;

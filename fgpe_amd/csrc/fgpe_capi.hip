@@ -93,6 +93,26 @@ void t_end(fgpe_ctx* c, long k, int cls)
     c->evrec.emplace_back(cls, (size_t)k);
 }
 
+int dalloc(fgpe_ctx* c, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(FGPE_EHIP, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    c->bytes += bytes;
+    return FGPE_OK;
+}
+
+void dfree(fgpe_ctx* c, void** p, size_t bytes)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        c->bytes -= bytes;
+        *p = nullptr;
+    }
+}
+
 }  // namespace fgpe
 
 namespace {
@@ -119,26 +139,6 @@ template <typename T> std::vector<c2_t<T>> twiddle_table(int n)
         w[m] = c2_t<T>{(T)(double)cosl(th), (T)(-(double)sinl(th))};
     }
     return w;
-}
-
-int dalloc(fgpe_ctx* c, void** p, size_t bytes)
-{
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(FGPE_EHIP, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    }
-    c->bytes += bytes;
-    return FGPE_OK;
-}
-
-void dfree(fgpe_ctx* c, void** p, size_t bytes)
-{
-    if (*p) {
-        (void)hipFree(*p);
-        c->bytes -= bytes;
-        *p = nullptr;
-    }
 }
 
 int last_axis(fgpe_ctx* c) { return c->n[2] > 1 ? 2 : 1; }
@@ -1033,7 +1033,7 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
-                    c->aux, c->xbuf, c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
+                    c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
                     c->lam[1], c->lam[2], c->pacc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -5,6 +5,8 @@
 
 #include <cstring>
 
+#include "../hpp/device_init.hpp"
+
 BEConeComponent::BEConeComponent(Params& par, int precision, bool do_run) : OneComponentGPSolver(par, precision)
 {
     p = par;
@@ -52,6 +54,11 @@ BECDipolar::BECDipolar(Params& par, int precision) : BEConeComponent(par, precis
 
 void BEConeComponent::InitPsi()
 {
+    if (deviceInitPsi(p)) {
+        devicePsi(ctx, p, 0, 0, p.getBool("tpi_vortex"), this->psi);
+        output.WritePsiInit(this->psi, p.Npoints);
+        return;
+    }
     Psi psi_(p, "");
     if (p.getBool("tpi_vortex")) psi_.imprintVortexTube(p, p.getDouble("api_prc"));
     output.WritePsiInit(psi_.getPsi(), p.Npoints);
@@ -60,6 +67,13 @@ void BEConeComponent::InitPsi()
 
 void BEConeComponent::InitVext()
 {
+    if (deviceInitVext(p)) {
+        // addProtocolPotential(p, 0): A = aseed
+        const bool seed = p.getBool("tvseed");
+        deviceVext(ctx, p, 0, 0, seed, seed ? p.getDouble("aseed") : 0.0, this->vext);
+        output.WriteVext(this->vext, p.Npoints);
+        return;
+    }
     Vext vext_(p, "");
     if (p.getBool("tvseed")) vext_.addProtocolPotential(p, 0);
     output.WriteVext(vext_.getVext(), p.Npoints);

@@ -4,6 +4,8 @@
 
 #include <cstring>
 
+#include "../hpp/device_init.hpp"
+
 BECtwoComponent::BECtwoComponent(Params2& par, int precision) : TwoComponentGPSolver(par, precision)
 {
     p = par;
@@ -29,6 +31,11 @@ BECtwoComponent::BECtwoComponent(Params2& par, int precision) : TwoComponentGPSo
 
 void BECtwoComponent::InitPsi()
 {
+    if (deviceInitPsi(p)) {
+        devicePsi(ctx, p, 0, 1, p.getBool("tpi_vortex"), this->psi1);
+        devicePsi(ctx, p, 1, 2, p.getBool("tpi_vortex"), this->psi2);
+        return;
+    }
     Psi psi_1(p, 1);
     Psi psi_2(p, 2);
     if (p.getBool("tpi_vortex")) {
@@ -41,6 +48,15 @@ void BECtwoComponent::InitPsi()
 
 void BECtwoComponent::InitVext()
 {
+    if (deviceInitVext(p)) {
+        const bool seed = p.getBool("tvseed");
+        for (int c = 1; c <= 2; ++c)
+            deviceVext(ctx, p, c - 1, c, seed, seed ? Vext::weightedProtocolAmplitude(p, c) : 0.0,
+                       c == 1 ? this->vext1 : this->vext2);
+        output.WriteVext(this->vext1, p.Npoints, "1");
+        output.WriteVext(this->vext2, p.Npoints, "2");
+        return;
+    }
     Vext vext_1(p, 1);
     Vext vext_2(p, 2);
     if (p.getBool("tvseed")) {

@@ -1,5 +1,6 @@
 // psi.hpp -- initial wave function (hpp/psi.hpp): Gaussian or tube with
-// seeded noise, vortex phase imprint, normalisation, read from psi_final.wdat.
+// seeded noise, vortex phase imprint, lattice profiles, normalisation, read
+// from psi_final.wdat.
 #ifndef FGPE_HOST_PSI_HPP
 #define FGPE_HOST_PSI_HPP
 
@@ -25,6 +26,22 @@ public:
     void addNoise(complex& noise_amp, const int& length);
     void imprintVortexTube(const ParamsBase& p, const double& Ncirc);
     void readPsi(const std::string& prefix, const int& length);
+
+    // additive lattice profiles, each followed by normalize(p, 1)
+    void force3DHEXLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                           const double& offy, const double& offz, const int& type = 1);
+    void force3DFCCLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                           const double& offy, const double& offz, const int& type = 1);
+    void force3DBCCLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                           const double& offy, const double& offz, const int& type = 1);
+    void force3DSCLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                          const double& offy, const double& offz, const int& type = 1);
+    void force2DSquareLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                              const double& offy, const int& type = 1);
+    void force2DTriangularLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                                  const double& offy, const int& type = 1);
+    void force1DLattice(const ParamsBase& p, const double& amp, const double& k, const double& off,
+                        const int& type = 1);
 
     inline int iX(int index, int NX, int) { return index % NX; }
     inline int iY(int index, int NX, int NY) { return (index / NX) % NY; }

@@ -1,4 +1,4 @@
-// vext.hpp -- external potential (hpp/vext.hpp): harmonic or tube trap,
+// vext.hpp -- external potential (hpp/vext.hpp): harmonic, tube or box trap,
 // cosine seeding protocols, read from vext.wdat.
 #ifndef FGPE_HOST_VEXT_HPP
 #define FGPE_HOST_VEXT_HPP
@@ -11,6 +11,7 @@ class Vext {
 public:
     void initTube(const ParamsBase& p, const int& type);
     void initHarmonic(const ParamsBase& p, const int& type);
+    void initBox(const ParamsBase& p, const int& type);
     void initVoid(const ParamsBase& p, const int& type);
 
     Vext(const Params& p, const std::string& type);
@@ -23,6 +24,8 @@ public:
     void readVext(const std::string& prefix, const int& length);
     void addProtocolPotential(const ParamsBase& p, const int& type, const int sign = 1);
     void addWeightedProtocolPotential(const Params2& p, const int& type);
+    // the amplitude A of addWeightedProtocolPotential (prints it, as the reference)
+    static double weightedProtocolAmplitude(const Params2& p, const int& type);
     void addOpticalLattice(const ParamsBase& p, const int& type);
 
     inline int iX(int index, int NX, int) { return index % NX; }

@@ -1,4 +1,4 @@
-// psi.cpp -- Psi initialisers (src/psi.cu:7-116, 135-176, 351-395).
+// psi.cpp -- Psi initialisers (src/psi.cu:7-395).
 #include "../hpp/psi.hpp"
 
 #include <cstdio>
@@ -99,6 +99,115 @@ void Psi::addNoise(complex& noise_amp, const int& length)
                                 std::imag(noise_amp) * distribution(generator));
         psi[i] = (complex(1, 0) + noise) * psi[i];
     }
+}
+
+// Lattice profiles added to the real part with the mean amplitude
+// sqrt(npart / dV), then normalised to component 1's particle number, as the
+// reference does for either component.
+void Psi::force1DLattice(const ParamsBase& p, const double& amp, const double& k, const double& off, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - off;
+        double force = amp * (1 - std::cos(k * x)) / 2.;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
+}
+
+void Psi::force2DTriangularLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                                   const double& offy, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    const double h = std::sqrt(3.0) / 2;
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - offx;
+        double y = getY(p, i) - offy;
+        double force = amp * (std::cos(k * x) + std::cos(k * (-0.5 * x + h * y)) + std::cos(k * (-0.5 * x - h * y))) / 3;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
+}
+
+void Psi::force2DSquareLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                               const double& offy, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - offx;
+        double y = getY(p, i) - offy;
+        double force = amp * (std::cos(k * y) + std::cos(k * x)) / 2;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
+}
+
+void Psi::force3DHEXLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                            const double& offy, const double& offz, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    const double r3 = std::sqrt(3.);
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - offx, y = getY(p, i) - offy, z = getZ(p, i) - offz;
+        double x1 = (-x + r3 * y) / 2., x4 = (-x - r3 * y) / 2.;
+        double force = amp * (std::cos(k * x1) + std::cos(k * x) + std::cos(k * z) + std::cos(k * x4)) / 4.0;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
+}
+
+void Psi::force3DFCCLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                            const double& offy, const double& offz, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    const double r3 = std::sqrt(3.);
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - offx, y = getY(p, i) - offy, z = getZ(p, i) - offz;
+        // the nine projections, (x+y+z) three times over as in the reference
+        const double u[9] = {(x + y - z) / r3, (y + z - x) / r3, (z + x - y) / r3,
+                             (x - y - z) / r3, (y - z - x) / r3, (z - x - y) / r3,
+                             (x + y + z) / r3, (y + z + x) / r3, (z + x + y) / r3};
+        double s = std::cos(k * u[0]);
+        for (int j = 1; j < 9; ++j) s = s + std::cos(k * u[j]);
+        double force = amp * s / 9.0;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
+}
+
+void Psi::force3DBCCLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                            const double& offy, const double& offz, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    const double r2 = std::sqrt(2.);
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - offx, y = getY(p, i) - offy, z = getZ(p, i) - offz;
+        const double u[6] = {(x + y) / r2, (z + y) / r2, (z + x) / r2, (x - y) / r2, (z - y) / r2, (z - x) / r2};
+        double s = std::cos(k * u[0]);
+        for (int j = 1; j < 6; ++j) s = s + std::cos(k * u[j]);
+        double force = amp * s / 6;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
+}
+
+void Psi::force3DSCLattice(const ParamsBase& p, const double& amp, const double& k, const double& offx,
+                           const double& offy, const double& offz, const int& type)
+{
+    double npart = (type == 1) ? p._npart(1) : p._npart(2);
+    double avg = std::pow(npart / (p.DX[0] * p.DX[1] * p.DX[2]), 0.5);
+    for (int i = 0; i < p.Npoints; ++i) {
+        double x = getX(p, i) - offx, y = getY(p, i) - offy, z = getZ(p, i) - offz;
+        double force = amp * (std::cos(k * z) + std::cos(k * y) + std::cos(k * x)) / 3;
+        psi[i] = force * avg + psi[i];
+    }
+    normalize(p, 1);
 }
 
 void Psi::normalize(const ParamsBase& p, const int& type)

@@ -1,4 +1,4 @@
-// vext.cpp -- Vext initialisers (src/vext.cu:6-85, 122-218).
+// vext.cpp -- Vext initialisers (src/vext.cu:6-218).
 #include "../hpp/vext.hpp"
 
 #include <cstdio>
@@ -55,12 +55,38 @@ void Vext::initHarmonic(const ParamsBase& p, const int& type)
     }
 }
 
+// tanh box of half-widths xbox/aho, ybox/aho, zbox/aho (the reference reads
+// _aho(0): for two components that is component 2's length)
+void Vext::initBox(const ParamsBase& p, const int&)
+{
+    const double V = p.getDouble("Vbox");
+    const double xx = p.getDouble("xbox") / p._aho(0);
+    const double yy = p.getDouble("ybox") / p._aho(0);
+    const double zz = p.getDouble("zbox") / p._aho(0);
+    const double B = p.getDouble("Bbox");
+    for (int i = 0; i < p.Npoints; ++i) {
+        const double x = getX(p, i), y = getY(p, i), z = getZ(p, i);
+        vext[i] = 0.5 * V *
+                  ((std::tanh(B * (x - xx)) - std::tanh(B * (x + xx))) *
+                   (std::tanh(B * (y - yy)) - std::tanh(B * (y + yy))) *
+                   (std::tanh(B * (z - zz)) - std::tanh(B * (z + zz))));
+    }
+}
+
 void Vext::initVoid(const ParamsBase& p, const int&)
 {
     for (int i = 0; i < p.Npoints; ++i) vext[i] = 0;
 }
 
 void Vext::addWeightedProtocolPotential(const Params2& p, const int& type)
+{
+    const double A = weightedProtocolAmplitude(p, type);
+    double L = 2 * p.XMAX[0];
+    double n = p.getDouble("nseed");
+    for (int i = 0; i < p.Npoints; ++i) vext[i] += A * std::cos(n * 2 * pi / L * getX(p, i));
+}
+
+double Vext::weightedProtocolAmplitude(const Params2& p, const int& type)
 {
     const double n1 = p.npart1 / p.V;
     const double n2 = p.npart2 / p.V;
@@ -76,9 +102,7 @@ void Vext::addWeightedProtocolPotential(const Params2& p, const int& type)
         else A = (type == 1) ? 2 * p.getDouble("aseed") * xi / (1 - xi) : 2 * p.getDouble("aseed") / (1 - xi);
     }
     std::cout << "# The lambda imbalance: " << A << std::endl;
-    double L = 2 * p.XMAX[0];
-    double n = p.getDouble("nseed");
-    for (int i = 0; i < p.Npoints; ++i) vext[i] += A * std::cos(n * 2 * pi / L * getX(p, i));
+    return A;
 }
 
 void Vext::addProtocolPotential(const ParamsBase& p, const int&, const int sign)

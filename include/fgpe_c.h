@@ -103,6 +103,70 @@ int fgpe_upload_vext(fgpe_ctx* ctx, int comp, const double* host_vext);
 /* Device address of component comp's FP64 state (N complex doubles). */
 int fgpe_psi_device_ptr(fgpe_ctx* ctx, int comp, void** dev_psi);
 
+/* ---- initial state and trap built on the device ------------------------ */
+/* Cell centres x_i = -xmax + i dx + dx/2 per axis (Psi::getX/getY/getZ,
+   src/psi.cu:134-150; Vext::getX/getY/getZ, src/vext.cu:182-199). */
+typedef struct fgpe_geom {
+    double xmax[3];
+    double dx[3];
+} fgpe_geom;
+
+enum { FGPE_INIT_HARMONIC = 0, FGPE_INIT_TUBE = 1, FGPE_INIT_VOID = 2 };
+enum {
+    FGPE_LATTICE_1D = 0,          /* Psi::force1DLattice        src/psi.cu:179-193 */
+    FGPE_LATTICE_2D_TRIANGULAR,   /* Psi::force2DTriangularLattice :196-213 */
+    FGPE_LATTICE_2D_SQUARE,       /* Psi::force2DSquareLattice  :217-232 */
+    FGPE_LATTICE_3D_HEX,          /* Psi::force3DHEXLattice     :236-262 */
+    FGPE_LATTICE_3D_FCC,          /* Psi::force3DFCCLattice     :265-297 */
+    FGPE_LATTICE_3D_BCC,          /* Psi::force3DBCCLattice     :301-328 */
+    FGPE_LATTICE_3D_SC            /* Psi::force3DSCLattice      :331-347 */
+};
+
+/* psi_comp = (1 + noise) * envelope at every point: envelope
+   exp(-0.05 (x^2 + y^2 + z^2)) on x/ax (HARMONIC, Psi::initHarmonic,
+   src/psi.cu:100-116), the same without x (TUBE, Psi::initTube :80-97) or
+   1 (VOID, Psi::initVoid :119-131); noise = (rand_re u1, rand_im u2) with u
+   uniform on [-1, 1).  The reference draws u from a sequential
+   std::mt19937 stream; here u comes from the counter-based Philox4x32-10
+   generator (Salmon et al., SC'11) keyed by seed, counter (point index,
+   stream 0), so a point's noise does not depend on the others.  With zero
+   amplitudes the result is the reference's.  Not normalised. */
+int fgpe_psi_init(fgpe_ctx* ctx, int comp, int kind, const fgpe_geom* g, const double ax[3],
+                  double rand_re, double rand_im, unsigned long long seed);
+/* psi *= 1 + (amp_re u1, amp_im u2), u from Philox stream `stream` (>= 1;
+   Psi::addNoise, src/psi.cu:167-176). */
+int fgpe_psi_add_noise(fgpe_ctx* ctx, int comp, double amp_re, double amp_im, unsigned long long seed,
+                       unsigned int stream);
+/* psi *= exp(i 2 pi ncirc x / (2 xmax_x)) (Psi::imprintVortexTube,
+   src/psi.cu:155-164). */
+int fgpe_psi_imprint_vortex_tube(fgpe_ctx* ctx, int comp, const fgpe_geom* g, double ncirc);
+/* psi += force(x - off) * sqrt(npart / (dx dy dz)) with the lattice's
+   force (FGPE_LATTICE_*); the reference then normalises with component
+   1's particle number, which the caller does with fgpe_psi_normalize. */
+int fgpe_psi_force_lattice(fgpe_ctx* ctx, int comp, int lattice, const fgpe_geom* g, double amp, double k,
+                           const double off[3], double npart);
+/* psi *= sqrt(npart / (sum |psi|^2 dx dy dz)); host_norm (may be NULL)
+   receives sum |psi|^2 dx dy dz afterwards (Psi::normalize,
+   src/psi.cu:351-372). */
+int fgpe_psi_normalize(fgpe_ctx* ctx, int comp, double npart, const fgpe_geom* g, double* host_norm);
+
+/* Trap: the real FP64 potential is assembled on the device and enters the
+   Hamiltonian at fgpe_vext_commit.  HARMONIC 0.5 ((x/ax)^2/ax^2 + ...)
+   (Vext::initHarmonic, src/vext.cu:72-85), TUBE without x (Vext::initTube
+   :60-70), VOID 0 (:112-118). */
+int fgpe_vext_init(fgpe_ctx* ctx, int comp, int kind, const fgpe_geom* g, const double ax[3]);
+/* Box trap 0.5 V prod_a (tanh(B (x_a - w_a)) - tanh(B (x_a + w_a))) on the
+   raw cell centres (Vext::initBox, src/vext.cu:88-109; w_a = xbox/aho ...). */
+int fgpe_vext_init_box(fgpe_ctx* ctx, int comp, const fgpe_geom* g, double V, double B, const double w[3]);
+/* V += A cos(2 pi n x / (2 xmax_x)) (Vext::addProtocolPotential,
+   addWeightedProtocolPotential with its host-side amplitude, and
+   addOpticalLattice, src/vext.cu:122-179). */
+int fgpe_vext_add_cos(fgpe_ctx* ctx, int comp, const fgpe_geom* g, double A, double n);
+/* Make the assembled potential component comp's trap (rounded to the
+   bulk precision); host_vext (may be NULL) receives it as N complex
+   doubles, for the Vext output file. */
+int fgpe_vext_commit(fgpe_ctx* ctx, int comp, double* host_vext);
+
 /* ---- building blocks on device arrays ---------------------------------- */
 /* 3D C2C transform of N complex values, unnormalised; direction -1 forward
    (CUFFT_FORWARD), +1 inverse (cufftExecZ2Z, src/baseHamiltonian.cu:89,95).

@@ -71,6 +71,31 @@ int    orc_imag_heavy_ball(const orc_grid *g, const orc_ham *h, double *psi,
 int    orc_num_threads(void);
 void   orc_set_threads(int n);
 
+/* ---- initial state and trap (fgpe_oracle_init.c) ------------------------ */
+typedef struct orc_geom {
+    double xmax[3];          /* cell centres -xmax + i dx + dx/2 per axis     */
+    double dx[3];
+    int n[3];
+} orc_geom;
+
+#include <stdint.h>
+void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void   orc_noise(long long i, uint32_t stream, uint64_t seed, double *u1, double *u2);
+/* kind 0 harmonic, 1 tube, 2 void */
+void   orc_psi_init(const orc_geom *g, int kind, const double ax[3], double rr, double ci,
+                    uint64_t seed, double *psi);
+void   orc_psi_add_noise(const orc_geom *g, double rr, double ci, uint64_t seed, uint32_t stream,
+                         double *psi);
+void   orc_psi_vortex(const orc_geom *g, double ncirc, double *psi);
+/* lattice 0 1D, 1 2D triangular, 2 2D square, 3 HEX, 4 FCC, 5 BCC, 6 SC */
+double orc_force(int lattice, double x, double y, double z, double amp, double k);
+void   orc_psi_force(const orc_geom *g, int lattice, double amp, double k, const double off[3],
+                     double npart, double *psi);
+double orc_psi_normalize(const orc_geom *g, double npart, double *psi);
+void   orc_vext_init(const orc_geom *g, int kind, const double ax[3], double *v);
+void   orc_vext_box(const orc_geom *g, double V, double B, const double w[3], double *v);
+void   orc_vext_add_cos(const orc_geom *g, double A, double n, double *v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,10 @@ class OrcGrid(C.Structure):
                 ("kx", _dp), ("ky", _dp), ("kz", _dp), ("dv", C.c_double)]
 
 
+class OrcGeom(C.Structure):
+    _fields_ = [("xmax", C.c_double * 3), ("dx", C.c_double * 3), ("n", C.c_int * 3)]
+
+
 class OrcHam(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
                 ("quench", C.c_double * 6), ("vtilde", _dp), ("vpx", C.c_double * 2)]
@@ -66,6 +70,21 @@ def lib():
         L.orc_imag_heavy_ball.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
+        _gp = C.POINTER(OrcGeom)
+        _u32p = C.POINTER(C.c_uint32)
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_noise.argtypes = [C.c_longlong, C.c_uint32, C.c_uint64, _dp, _dp]
+        L.orc_psi_init.argtypes = [_gp, C.c_int, _dp, C.c_double, C.c_double, C.c_uint64, _dp]
+        L.orc_psi_add_noise.argtypes = [_gp, C.c_double, C.c_double, C.c_uint64, C.c_uint32, _dp]
+        L.orc_psi_vortex.argtypes = [_gp, C.c_double, _dp]
+        L.orc_force.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_force.restype = C.c_double
+        L.orc_psi_force.argtypes = [_gp, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp]
+        L.orc_psi_normalize.argtypes = [_gp, C.c_double, _dp]
+        L.orc_psi_normalize.restype = C.c_double
+        L.orc_vext_init.argtypes = [_gp, C.c_int, _dp, _dp]
+        L.orc_vext_box.argtypes = [_gp, C.c_double, C.c_double, _dp, _dp]
+        L.orc_vext_add_cos.argtypes = [_gp, C.c_double, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -217,3 +236,90 @@ def imag_heavy_ball(g: Grid, h: Ham, psi, V, dt, beta, npart, eps, niter):
                                   _p(V.view(np.float64)), dt, beta, npart, eps, niter,
                                   _p(hist))
     return a, n, hist[:n]
+
+
+# ---- initial state and trap (fgpe_oracle_init.c) --------------------------
+LATTICES = ("1d", "2d_triangular", "2d_square", "3d_hex", "3d_fcc", "3d_bcc", "3d_sc")
+
+
+def geom(n, xmax, dx) -> OrcGeom:
+    return OrcGeom((C.c_double * 3)(*map(float, xmax)), (C.c_double * 3)(*map(float, dx)),
+                   (C.c_int * 3)(*map(int, n)))
+
+
+def _d3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def noise(i, stream, seed):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_noise(int(i), int(stream), int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def _ntot(g: OrcGeom):
+    return int(g.n[0]) * int(g.n[1]) * int(g.n[2])
+
+
+def psi_init(g: OrcGeom, kind, ax=(1.0, 1.0, 1.0), rand=(0.0, 0.0), seed=0):
+    """kind 0 harmonic, 1 tube, 2 void (src/psi.cu:80-131), Philox noise."""
+    out = np.empty(_ntot(g), np.complex128)
+    lib().orc_psi_init(C.byref(g), int(kind), _p(_d3(ax)), float(rand[0]), float(rand[1]),
+                       int(seed) & 0xFFFFFFFFFFFFFFFF, _p(out.view(np.float64)))
+    return out
+
+
+def psi_add_noise(g: OrcGeom, psi, amp, seed, stream):
+    out = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    lib().orc_psi_add_noise(C.byref(g), float(amp[0]), float(amp[1]), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                            int(stream), _p(out.view(np.float64)))
+    return out
+
+
+def psi_vortex(g: OrcGeom, psi, ncirc):
+    out = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    lib().orc_psi_vortex(C.byref(g), float(ncirc), _p(out.view(np.float64)))
+    return out
+
+
+def force(lattice, x, y, z, amp, k):
+    return lib().orc_force(LATTICES.index(lattice), float(x), float(y), float(z), float(amp), float(k))
+
+
+def psi_force(g: OrcGeom, psi, lattice, amp, k, off, npart):
+    out = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    lib().orc_psi_force(C.byref(g), LATTICES.index(lattice), float(amp), float(k), _p(_d3(off)),
+                        float(npart), _p(out.view(np.float64)))
+    return out
+
+
+def psi_normalize(g: OrcGeom, psi, npart):
+    out = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    nrm = lib().orc_psi_normalize(C.byref(g), float(npart), _p(out.view(np.float64)))
+    return out, nrm
+
+
+def vext_init(g: OrcGeom, kind, ax=(1.0, 1.0, 1.0)):
+    out = np.empty(_ntot(g), np.float64)
+    lib().orc_vext_init(C.byref(g), int(kind), _p(_d3(ax)), _p(out))
+    return out
+
+
+def vext_box(g: OrcGeom, V, B, w):
+    out = np.empty(_ntot(g), np.float64)
+    lib().orc_vext_box(C.byref(g), float(V), float(B), _p(_d3(w)), _p(out))
+    return out
+
+
+def vext_add_cos(g: OrcGeom, v, A, n):
+    out = np.ascontiguousarray(v, dtype=np.float64).copy()
+    lib().orc_vext_add_cos(C.byref(g), float(A), float(n), _p(out))
+    return out
