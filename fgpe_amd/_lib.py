@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfgpe.so")
+# FGPE_LIBRARY selects another build of the library (e.g. one with a user
+# pointwise term, libfgpe_user_example.so)
+LIB_PATH = os.environ.get("FGPE_LIBRARY") or os.path.join(_HERE, "libfgpe.so")
 
 FGPE_PREC_F64 = 0
 FGPE_PREC_MIXED = 1
@@ -32,6 +34,7 @@ EXPORTS = (
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
     "fgpe_psi_init", "fgpe_psi_add_noise", "fgpe_psi_imprint_vortex_tube", "fgpe_psi_force_lattice",
     "fgpe_psi_normalize", "fgpe_vext_init", "fgpe_vext_init_box", "fgpe_vext_add_cos", "fgpe_vext_commit",
+    "fgpe_user_term",
 )
 # initialiser kinds and lattices (fgpe_psi_init, fgpe_vext_init, fgpe_psi_force_lattice)
 INIT_HARMONIC, INIT_TUBE, INIT_VOID = 0, 1, 2
@@ -42,7 +45,8 @@ KCLASS_NAMES = ("xstage", "strided", "last", "fft_x", "other")
 class HParams(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
                 ("quench", C.c_double * 6), ("a_dd", C.c_double), ("dip", C.c_double * 3),
-                ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int), ("vpx", C.c_double * 2)]
+                ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int), ("vpx", C.c_double * 2),
+                ("user", C.c_double * 8)]
 
 
 class Geom(C.Structure):
@@ -56,6 +60,13 @@ class Geom(C.Structure):
 
 class FgpeError(RuntimeError):
     pass
+
+
+def user_term_enabled() -> bool:
+    """True when the loaded library was built with a user pointwise term."""
+    v = C.c_int()
+    check(load().fgpe_user_term(C.byref(v)))
+    return bool(v.value)
 
 
 _lib = None
@@ -126,6 +137,7 @@ def load() -> C.CDLL:
     L.fgpe_vext_init_box.argtypes = [_vp, C.c_int, _gp, C.c_double, C.c_double, _dp]
     L.fgpe_vext_add_cos.argtypes = [_vp, C.c_int, _gp, C.c_double, C.c_double]
     L.fgpe_vext_commit.argtypes = [_vp, C.c_int, _dp]
+    L.fgpe_user_term.argtypes = [C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -186,13 +198,14 @@ class Context:
         check(load().fgpe_set_kspace(self.h, _dptr(kx), _dptr(ky), _dptr(kz), float(dv)))
 
     def set_hamiltonian(self, ncomp=1, a=(0.0,), quench=None, a_dd=0.0, dip=(0.0, 0.0, 1.0),
-                        sc_U=0.0, sc_R=0.0, sc_dim=3, vpx=(0.0, 0.0)):
+                        sc_U=0.0, sc_R=0.0, sc_dim=3, vpx=(0.0, 0.0), user=()):
         aa = tuple(a) + (0.0,) * (3 - len(a))
         q = (0.0,) * 6 if quench is None else tuple(quench)
         vv = tuple(vpx) + (0.0,) * (2 - len(vpx))
+        uu = tuple(map(float, user)) + (0.0,) * (8 - len(user))
         hp = HParams(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
                      (C.c_double * 6)(*q), a_dd, (C.c_double * 3)(*dip), sc_U, sc_R, sc_dim,
-                     (C.c_double * 2)(*vv))
+                     (C.c_double * 2)(*vv), (C.c_double * 8)(*uu))
         check(load().fgpe_set_hamiltonian(self.h, C.byref(hp)))
         self.ncomp = ncomp
 

@@ -28,9 +28,20 @@ SOURCES = [os.path.join(HERE, "csrc", f) for f in
 BUILD_DIR = os.path.join(HERE, "_build")
 
 
-def _compile(src: str, verbose: bool) -> str:
-    obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+# Library variants: name -> extra flags.  "user_example" compiles the example
+# user pointwise term of csrc/user_term.hpp (the tests' custom Hamiltonian).
+VARIANTS = {"": [], "user_example": ["-DFGPE_USER_TERM_EXAMPLE"]}
+
+
+def lib_path(variant: str = "") -> str:
+    return LIB if not variant else os.path.join(HERE, f"libfgpe_{variant}.so")
+
+
+def _compile(src: str, verbose: bool, variant: str = "") -> str:
+    bdir = os.path.join(BUILD_DIR, variant) if variant else BUILD_DIR
+    os.makedirs(bdir, exist_ok=True)
+    obj = os.path.join(bdir, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *HIP_FLAGS, *VARIANTS[variant], "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -40,16 +51,21 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
-def build_lib(verbose: bool = False) -> str:
+def build_lib(verbose: bool = False, variants=("", "user_example")) -> str:
+    """Compile libfgpe.so and its variants (all sources in parallel)."""
     os.makedirs(BUILD_DIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    jobs = [(v, src) for v in variants for src in SOURCES]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(lambda j: _compile(j[1], verbose, j[0]), jobs))
+    for v in variants:
+        out = lib_path(v)
+        tmp = out + ".tmp"
+        mine = [o for (vv, _), o in zip(jobs, objs) if vv == v]
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *mine, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, out)
     return LIB
 
 

@@ -222,6 +222,9 @@ template <typename T> XArgs<T> base_xargs(fgpe_ctx* c, double t)
             }
     }
     a.partials = c->partials;
+    a.ny = c->n[1];
+    a.t = t;
+    for (int i = 0; i < 8; ++i) a.user[i] = h.user[i];
     return a;
 }
 
@@ -483,6 +486,7 @@ int adj_check(fgpe_ctx* c)
     if (c->hp.sc_U != 0.0) return fail(FGPE_EUNSUP, "the adjoint supports the dipolar k-space kernel only");
     if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
     if (c->hp.vpx[0] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
+    if (UserTerm::enabled) return fail(FGPE_EUNSUP, "the adjoint does not know the user term's derivative");
     return FGPE_OK;
 }
 
@@ -961,6 +965,13 @@ template <typename T> int group_autotune(fgpe_group* g, int reps, double* out)
 
 // =========================================================================
 extern "C" {
+
+int fgpe_user_term(int* enabled)
+{
+    if (!enabled) return fail(FGPE_EINVAL, "NULL argument");
+    *enabled = UserTerm::enabled ? 1 : 0;
+    return FGPE_OK;
+}
 
 const char* fgpe_last_error(void) { return g_err.c_str(); }
 

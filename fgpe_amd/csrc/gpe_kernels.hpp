@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_core.hpp"
+#include "user_term.hpp"
 
 namespace fgpe {
 
@@ -494,6 +495,9 @@ template <typename T> struct XArgs {
     const double* mu;       // HB: mu_c of the previous iteration (null: 0)
     const int* stop;        // nonzero: the loop has converged, do nothing
     int rev;                // 1: blocks take the row groups in reverse order
+    int ny;                 // grid rows per z plane (the user term's grid index)
+    double t;               // time of this H application (the user term)
+    double user[8];         // the user term's parameters
 };
 
 // Minimum waves per SIMD the register allocator targets for an x-stage
@@ -503,6 +507,15 @@ template <typename T> struct XArgs {
 template <typename T, int N, int V, int NC, int MODE> constexpr int xstage_waves()
 {
     return (sizeof(T) == 4 && V == 1 && NC == 1 && MODE == MODE_RK && N <= 256 && Plan<N, V>::E <= 8) ? 4 : 1;
+}
+
+// The user term's inputs at linear index g (rows of N points along x).
+template <int N, typename T>
+__device__ __forceinline__ UserTermIn user_in(const XArgs<T>& a, long long g, int c, double pr, double pi, double n,
+                                              double no)
+{
+    const long long r = g / N;
+    return UserTermIn{a.t, (int)(g - r * N), (int)(r % a.ny), (int)(r / a.ny), c, pr, pi, n, no, a.user};
 }
 
 template <typename T, int N, int V, int NC, bool DENS, int MODE>
@@ -619,13 +632,19 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     double rX = (vr + g4s * n);
                     if (NC == 2) rX = rX + g4o * no;
                     if (DENS) rX += (double)phi[q];
-                    const double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                    double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
                     double2 he = double2{0.0, 0.0};
                     if (MODE == MODE_APPLY || MODE == MODE_HB) {
                         double eX = (vr + g2s * n);
                         if (NC == 2) eX = eX + g2o * no;
                         if (DENS) eX += 0.5 * (double)phi[q];
                         he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
+                    }
+                    if constexpr (UserTerm::enabled) {
+                        const UserTermOut w = UserTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
+                        h = double2{h.x + (w.wr * ps.x - w.wi * ps.y), h.y + (w.wr * ps.y + w.wi * ps.x)};
+                        if (MODE == MODE_APPLY || MODE == MODE_HB)
+                            he = double2{he.x + (w.er * ps.x - w.ei * ps.y), he.y + (w.er * ps.y + w.ei * ps.x)};
                     }
                     if (MODE == MODE_APPLY) {
                         if (a.hmu) glob(a.hmu)[c * a.ntot + g] = h;
@@ -715,7 +734,11 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     float rX = vr + f4s * n;
                     if (NC == 2) rX = rX + f4o * no;
                     if (DENS) rX += (float)phi[q];
-                    const float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                    float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                    if constexpr (UserTerm::enabled) {
+                        const UserTermOut w = UserTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
+                        h = float2{h.x + (float)(w.wr * ps.x - w.wi * ps.y), h.y + (float)(w.wr * ps.y + w.wi * ps.x)};
+                    }
                     if (a.stage < 4) {
                         float2 po = ps;
                         if (a.stage != 1) {

@@ -10,6 +10,15 @@ SingleTemplate::SingleTemplate(Params& par, int precision) : BEConeComponent(par
         hparams.a[0] = p.a;                      contact 4 pi a |psi|^2
         hparams.a_dd = p.add;                    dipolar kernel along hparams.dip
         hparams.sc_U = ...; hparams.sc_R = ...;  soft-core kernel
+        Any other pointwise term is written once in the library's
+        fgpe_amd/csrc/user_term.hpp (UserTerm::eval, rebuild libfgpe.so);
+        its eight parameters come from hparams.user, here from the input
+        keys user_prm0 ... user_prm7 when present.
     */
+    const InputVariables iv = p.getIV();
+    for (int i = 0; i < 8; ++i) {
+        const std::string key = "user_prm" + std::to_string(i);
+        if (iv.getVariable(key) != "None") hparams.user[i] = p.getDouble(key);
+    }
     run();
 }

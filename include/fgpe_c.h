@@ -72,7 +72,20 @@ typedef struct fgpe_hparams {
                            the one-component RK4 step (stages 1-3 use
                            alg_calcHpsiMU, src/oneComponentGPSolver.cu:170-187;
                            the two-component RK4 uses only alg_calcHpsiMU). */
+    double user[8];     /* parameters of the library's user pointwise term
+                           (fgpe_amd/csrc/user_term.hpp), read by its functor */
 } fgpe_hparams;
+
+/* Custom Hamiltonians (the plugin kernels SingleTemplate_template /
+   SingleTemplate_MU_template, hamiltonians/single_template.cu:333-373, that
+   a user edits): the library compiles one user functor into the fused
+   pointwise stage of every H application, which adds w psi_c to H psi_c
+   and w_en psi_c to the energy operator, w from (t, grid index, component,
+   psi_c, densities, user[8]).  Editing fgpe_amd/csrc/user_term.hpp and
+   rebuilding the library replaces it; the shipped library has none.
+   *enabled = 1 when this build has one.  Adjoint gradients reject a build
+   with a user term (its derivative is not known to the library). */
+int fgpe_user_term(int* enabled);
 
 /* ---- context (replaces Cuda::Cuda, src/parallel.cu:14-113) ------------- */
 int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out);

@@ -346,6 +346,44 @@ void orc_hpsi(const orc_grid *g, const orc_ham *h, double t,
     orc_hpsi_vpx(g, h, t, psi, V, hmu, hen, hen != NULL);
 }
 
+/* The example user pointwise term of the library's user_term.hpp (a
+   modulated contact coupling, three-body loss and a ramp along z):
+   w = u0 sin(u1 t) n + u3 iz - i u2 n^2 / 2, w_en with half the coupling. */
+static void user_example(const orc_ham *h, double t, int iz, double n, double *wr, double *wi, double *er,
+                         double *ei)
+{
+    const double dg = h->user[0] * sin(h->user[1] * t);
+    const double loss = -0.5 * h->user[2] * n * n;
+    const double ramp = h->user[3] * iz;
+    *wr = dg * n + ramp;
+    *wi = loss;
+    *er = 0.5 * dg * n + ramp;
+    *ei = loss;
+}
+
+static void add_user(const orc_grid *g, const orc_ham *h, double t, int nc, const double *psi, double *hmu,
+                     double *hen)
+{
+    const long long N = (long long)g->nx * g->ny * g->nz;
+    for (int c = 0; c < nc; ++c) {
+        const double *p = psi + 2 * N * c;
+        for (long long i = 0; i < N; ++i) {
+            const int iz = (int)(i / ((long long)g->nx * g->ny));
+            const double pr = p[2 * i], pi_ = p[2 * i + 1];
+            double wr, wi, er, ei;
+            user_example(h, t, iz, pr * pr + pi_ * pi_, &wr, &wi, &er, &ei);
+            double *o = hmu + 2 * N * c;
+            o[2 * i] += wr * pr - wi * pi_;
+            o[2 * i + 1] += wr * pi_ + wi * pr;
+            if (hen) {
+                double *e = hen + 2 * N * c;
+                e[2 * i] += er * pr - ei * pi_;
+                e[2 * i + 1] += er * pi_ + ei * pr;
+            }
+        }
+    }
+}
+
 void orc_hpsi_vpx(const orc_grid *g, const orc_ham *h, double t, const double *psi,
                   const double *V, double *hmu, double *hen, int vpx)
 {
@@ -424,6 +462,7 @@ void orc_hpsi_vpx(const orc_grid *g, const orc_ham *h, double t, const double *p
             }
         }
     }
+    if (h->user_example) add_user(g, h, t, nc, psi, hmu, hen);
     free(ke);
     free(phi);
 }

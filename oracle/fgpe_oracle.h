@@ -29,6 +29,8 @@ typedef struct orc_ham {
     double quench[6];        /* a0, a1, a2, t1, t2, t3                          */
     const double *vtilde;    /* real k-space kernel Vt(k) (N doubles) or NULL   */
     double vpx[2];           /* moving-frame factor per component (0: off)      */
+    int    user_example;     /* 1: the example user pointwise term              */
+    double user[8];          /* its parameters                                  */
 } orc_ham;
 
 /* ---- building blocks ---------------------------------------------------- */

@@ -385,3 +385,48 @@ def test_two_component_hpsi_uniform():
     assert np.abs(hmu[0] - ref1).max() < 1e-14 and np.abs(hmu[1] - ref2).max() < 1e-14
     e1 = (0.1 + 0.05j + 2 * np.pi * (a11 * n1 + a12 * n2)) * A[0]
     assert np.abs(hen[0] - e1).max() < 1e-14
+
+
+# ---- the example user pointwise term (fgpe_amd/csrc/user_term.hpp) --------
+def test_user_term_loss_decay_closed_form():
+    """Three-body loss alone on a uniform state: dn/dt = -K3 n^3, so
+    n(t) = n0 / sqrt(1 + 2 K3 n0^2 t); the phase stays put."""
+    g = grid(4, 4, 4)
+    A0 = 0.8 + 0.3j
+    K3, dt, n = 0.6, 1e-3, 400
+    out = orc.rk4(g, orc.Ham(1, (0.0,), user=(0.0, 0.0, K3, 0.0)), np.full(g.shape, A0),
+                  np.zeros(g.shape, complex), dt, n)
+    n0, t = abs(A0) ** 2, n * dt
+    assert np.abs(np.abs(out) ** 2 - n0 / np.sqrt(1 + 2 * K3 * n0 ** 2 * t)).max() < 1e-11
+    assert np.abs(np.angle(out) - np.angle(A0)).max() < 1e-13
+
+
+def test_user_term_modulated_coupling_phase_closed_form():
+    """dg(t) = u0 sin(u1 t) alone on a uniform state: |psi| is kept and the
+    phase is -n u0 (1 - cos(u1 t)) / u1 (the stage times t, t + dt/2, t + dt
+    enter through RK4)."""
+    g = grid(4, 4, 4)
+    A0 = 0.6 - 0.2j
+    u0, u1, dt, n = 1.3, 2.1, 1e-3, 500
+    out = orc.rk4(g, orc.Ham(1, (0.0,), user=(u0, u1)), np.full(g.shape, A0), np.zeros(g.shape, complex),
+                  dt, n)
+    nn, t = abs(A0) ** 2, n * dt
+    ref = A0 * np.exp(-1j * nn * u0 * (1 - np.cos(u1 * t)) / u1)
+    assert np.abs(out - ref).max() < 1e-11
+
+
+def test_user_term_ramp_and_energy_halves():
+    """On a uniform state (no kinetic term) the ramp adds u3 iz psi to both
+    operators, the modulated coupling dg n psi to H and dg n psi / 2 to H_en."""
+    g = grid(4, 6, 8)
+    A0 = 0.7 + 0.1j
+    psi = np.full(g.shape, A0)
+    V = np.zeros(g.shape, complex)
+    u = (0.9, 1.7, 0.0, 0.25)
+    t = 0.4
+    h0, e0 = orc.hpsi(g, orc.Ham(1, (0.0,)), t, psi, V)
+    h1, e1 = orc.hpsi(g, orc.Ham(1, (0.0,), user=u), t, psi, V)
+    iz = np.arange(g.nz)[:, None, None] * np.ones(g.shape)
+    dg = u[0] * np.sin(u[1] * t) * abs(A0) ** 2
+    assert np.abs((h1 - h0) - (dg + u[3] * iz) * psi).max() < 1e-14
+    assert np.abs((e1 - e0) - (0.5 * dg + u[3] * iz) * psi).max() < 1e-14
