@@ -18,6 +18,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fft_core.hpp"
 #include "user_term.hpp"
 
@@ -509,6 +511,36 @@ template <typename T, int N, int V, int NC, int MODE> constexpr int xstage_waves
     return (sizeof(T) == 4 && V == 1 && NC == 1 && MODE == MODE_RK && N <= 256 && Plan<N, V>::E <= 8) ? 4 : 1;
 }
 
+// Accesses of the x stage's pointwise operands (state, stage state, RK
+// accumulator, trap) are non-temporal: these streams are read or written
+// once per stage, and left to allocate in L2 and the memory-side cache they
+// evict the spectral channels that the x stage itself and the passes around
+// it re-read (measured at 256^3: x stage -8%, step +5% FP64 / +6% mixed).
+// Every access of these arrays in the kernel goes through here, so they are
+// always reached through the same (native vector) type.
+template <typename V> __device__ __forceinline__ V ldn(gp<V> p, long long i)
+{
+    if constexpr (std::is_arithmetic_v<V>) {
+        return __builtin_nontemporal_load(&p[i]);
+    } else {
+        using NV = typename V::Native_vec_;
+        return __builtin_bit_cast(V, __builtin_nontemporal_load((gp<const NV>)(&p[i])));
+    }
+}
+template <typename V> __device__ __forceinline__ V ldn(gp<const V> p, long long i)
+{
+    return __builtin_nontemporal_load(&p[i]);
+}
+template <typename V> __device__ __forceinline__ void stn(gp<V> p, long long i, const V& v)
+{
+    if constexpr (std::is_arithmetic_v<V>) {
+        __builtin_nontemporal_store(v, &p[i]);
+    } else {
+        using NV = typename V::Native_vec_;
+        __builtin_nontemporal_store(__builtin_bit_cast(NV, v), (gp<NV>)(&p[i]));
+    }
+}
+
 // The user term's inputs at linear index g (rows of N points along x).
 template <int N, typename T>
 __device__ __forceinline__ UserTermIn user_in(const XArgs<T>& a, long long g, int c, double pr, double pi, double n,
@@ -603,11 +635,11 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     double2 po = double2{0.0, 0.0};
                     double2 ps;
                     if (from_p0) {
-                        const double2 r = P0[c][g];
+                        const double2 r = ldn(P0[c], g);
                         po = double2{sig * r.x, sig * r.y};
                         ps = po;
                     } else {
-                        ps = to_d2<T>(PS[c][g]);
+                        ps = to_d2<T>(ldn(PS[c], g));
                     }
                     const double n = ps.x * ps.x + ps.y * ps.y;
                     double no = 0.0;
@@ -615,10 +647,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (c == 0) {
                             double2 o;
                             if (from_p0) {
-                                const double2 r = P0[1][g];
+                                const double2 r = ldn(P0[1], g);
                                 o = double2{sig1 * r.x, sig1 * r.y};
                             } else {
-                                o = to_d2<T>(PS[1][g]);
+                                o = to_d2<T>(ldn(PS[1], g));
                             }
                             no = o.x * o.x + o.y * o.y;
                             nsave[q] = n;
@@ -626,8 +658,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             no = nsave[q];
                         }
                     }
-                    const double vr = (double)Vr[c][g];
-                    const double vi = Vi[c] ? (double)Vi[c][g] : 0.0;
+                    const double vr = (double)ldn(Vr[c], g);
+                    const double vi = Vi[c] ? (double)ldn(Vi[c], g) : 0.0;
                     const double2 ke = to_d2<T>(v[q]);
                     double rX = (vr + g4s * n);
                     if (NC == 2) rX = rX + g4o * no;
@@ -653,12 +685,12 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
                         sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
                     } else if (MODE == MODE_HB) {
-                        const double2 old = to_d2<T>(PS[c][g]);
+                        const double2 old = to_d2<T>(ldn(PS[c], g));
                         const double m = a.mu ? glob(a.mu)[c] : 0.0;
                         const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
                                                    (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
-                        PS[c][g] = from_d2<T>(ps);
-                        P0[c][g] = nw;
+                        stn(PS[c], g, from_d2<T>(ps));
+                        stn(P0[c], g, nw);
                         sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
                         sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
                         sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
@@ -666,7 +698,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
                     } else {
                         if (a.stage != 1) {
-                            const double2 r = P0[c][g];
+                            const double2 r = ldn(P0[c], g);
                             po = double2{sig * r.x, sig * r.y};
                         }
                         if (a.stage < 4) {
@@ -674,21 +706,21 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
                                                        po.y + (a.cr * h.y + a.ci * h.x)};
                             if (a.stage == 1) {
-                                K[c][g] = from_d2<T>(h);
+                                stn(K[c], g, from_d2<T>(h));
                             } else {
-                                const double2 k = to_d2<T>(K[c][g]);
-                                K[c][g] = from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y});
+                                const double2 k = to_d2<T>(ldn(K[c], g));
+                                stn(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
                             }
                             const T2 st = from_d2<T>(nw);
-                            (PSo[c] ? PSo[c] : PS[c])[g] = st;
+                            stn(PSo[c] ? PSo[c] : PS[c], g, st);
                             v[q] = st;
                         } else {
                             // FinalRKStep: psi = old + c*(k + h)/6
-                            const double2 k = to_d2<T>(K[c][g]);
+                            const double2 k = to_d2<T>(ldn(K[c], g));
                             const double aX = k.x + h.x, aY = k.y + h.y;
                             const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
                                                        po.y + (a.cr * aY + a.ci * aX) / 6.0};
-                            P0[c][g] = nw;
+                            stn(P0[c], g, nw);
                             v[q] = from_d2<T>(nw);
                             if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                         }
@@ -706,10 +738,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     const long long g = row0 + elemA<N, V>(t, q);
                     float2 ps;
                     if (a.stage == 1) {
-                        const double2 r = P0[c][g];
+                        const double2 r = ldn(P0[c], g);
                         ps = float2{(float)(sig * r.x), (float)(sig * r.y)};
                     } else {
-                        ps = PS[c][g];
+                        ps = ldn(PS[c], g);
                     }
                     const float n = ps.x * ps.x + ps.y * ps.y;
                     float no = 0.f;
@@ -717,10 +749,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (c == 0) {
                             float2 o;
                             if (a.stage == 1) {
-                                const double2 r = P0[1][g];
+                                const double2 r = ldn(P0[1], g);
                                 o = float2{(float)(sig1 * r.x), (float)(sig1 * r.y)};
                             } else {
-                                o = PS[1][g];
+                                o = ldn(PS[1], g);
                             }
                             no = o.x * o.x + o.y * o.y;
                             nsave[q] = n;
@@ -728,8 +760,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             no = (float)nsave[q];
                         }
                     }
-                    const float vr = (float)Vr[c][g];
-                    const float vi = Vi[c] ? (float)Vi[c][g] : 0.f;
+                    const float vr = (float)ldn(Vr[c], g);
+                    const float vi = Vi[c] ? (float)ldn(Vi[c], g) : 0.f;
                     const float2 ke = v[q];
                     float rX = vr + f4s * n;
                     if (NC == 2) rX = rX + f4o * no;
@@ -742,25 +774,25 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     if (a.stage < 4) {
                         float2 po = ps;
                         if (a.stage != 1) {
-                            const double2 r = P0[c][g];
+                            const double2 r = ldn(P0[c], g);
                             po = float2{(float)(sig * r.x), (float)(sig * r.y)};
                         }
                         const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
                         if (a.stage == 1) {
-                            K[c][g] = h;
+                            stn(K[c], g, h);
                         } else {
-                            const float2 k = K[c][g];
-                            K[c][g] = float2{k.x + 2.f * h.x, k.y + 2.f * h.y};
+                            const float2 k = ldn(K[c], g);
+                            stn(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
                         }
-                        (PSo[c] ? PSo[c] : PS[c])[g] = nw;
+                        stn(PSo[c] ? PSo[c] : PS[c], g, nw);
                         v[q] = nw;
                     } else {
-                        const float2 k = K[c][g];
+                        const float2 k = ldn(K[c], g);
                         const float ax = k.x + h.x, ay = k.y + h.y;
                         const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
-                        const double2 r = P0[c][g];
+                        const double2 r = ldn(P0[c], g);
                         const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
-                        P0[c][g] = nw;
+                        stn(P0[c], g, nw);
                         v[q] = from_d2<T>(nw);
                         if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                     }
