@@ -60,6 +60,7 @@ struct fgpe_ctx {
 
     double* k[3] = {nullptr, nullptr, nullptr};
     float2* k32[3] = {nullptr, nullptr, nullptr};  // the same axes in FP32 as (k[i], k[(n - i) % n])
+    double2* k64[3] = {nullptr, nullptr, nullptr}; // and in FP64 as (k[i], k[(n - i) % n])
     double dv = 0.0;
     bool kset = false;
 
@@ -199,12 +200,12 @@ inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int 
     LastArgs a;
     a.s = strided_args(c, data, axis, scale, nch, ncomp, half, stop);
     const int oaxis = axis == 1 ? 2 : 1;
-    a.kA = c->k[axis];
-    a.kX = c->k[0];
-    a.kO = c->k[oaxis];
     a.kA32 = c->k32[axis];
     a.kX32 = c->k32[0];
     a.kO32 = c->k32[oaxis];
+    a.kA64 = c->k64[axis];
+    a.kX64 = c->k64[0];
+    a.kO64 = c->k64[oaxis];
     const double* d = c->hp.dip;
     a.dA = d[axis];
     a.dX = d[0];

@@ -1002,6 +1002,7 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     for (int a = 0; a < 3 && r == FGPE_OK; ++a) {
         r = dalloc(c, (void**)&c->k[a], sizeof(double) * c->n[a]);
         if (r == FGPE_OK) r = dalloc(c, (void**)&c->k32[a], sizeof(float2) * c->n[a]);
+        if (r == FGPE_OK) r = dalloc(c, (void**)&c->k64[a], sizeof(double2) * c->n[a]);
         if (r == FGPE_OK) r = dalloc(c, &c->tw64[a], sizeof(double2) * c->n[a]);
         if (r == FGPE_OK) r = dalloc(c, &c->tw32[a], sizeof(float2) * c->n[a]);
         if (r == FGPE_OK) {
@@ -1041,7 +1042,7 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
 {
     if (!c) return FGPE_OK;
     (void)hipDeviceSynchronize();
-    void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
+    void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->k64[0], c->k64[1], c->k64[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
                     c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
@@ -1085,8 +1086,13 @@ int fgpe_set_kspace(fgpe_ctx* c, const double* kx, const double* ky, const doubl
         FGPE_HIPCHK(hipMemcpy(c->k[a], src[a], sizeof(double) * c->n[a], hipMemcpyHostToDevice));
         const int n = c->n[a];
         std::vector<float2> f(n);
-        for (int i = 0; i < n; ++i) f[i] = float2{(float)src[a][i], (float)src[a][(n - i) % n]};
+        std::vector<double2> d(n);
+        for (int i = 0; i < n; ++i) {
+            f[i] = float2{(float)src[a][i], (float)src[a][(n - i) % n]};
+            d[i] = double2{src[a][i], src[a][(n - i) % n]};
+        }
         FGPE_HIPCHK(hipMemcpy(c->k32[a], f.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+        FGPE_HIPCHK(hipMemcpy(c->k64[a], d.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     }
     c->dv = dv;
     c->kset = true;

@@ -276,16 +276,16 @@ k_strided(const c2_t<T>* __restrict__ tw, StridedArgs a0, const StridedArgs* __r
 // and Vt(k') = Vt(k) bit for bit, so only Nyquist-plane entries change.
 struct LastArgs {
     StridedArgs s;
-    const double* kA;   // k along this axis
-    const double* kX;   // k along x (column index)
-    const double* kO;   // k along the outer axis
-    double dA, dX, dO;  // dipole direction components along A, X, O
+    double dA, dX, dO;  // dipole direction components along A (this axis), X, O (outer axis)
     double lap_scale;   // 19.739208802 / Ntot
     double inv_ntot;    // 1 / Ntot
     double c_dd;        // 12 pi a_dd (0: no dipolar term)
     const void* wtab;   // T table of Vt (complex-layout spatial index) or null
     double vpx[2];      // moving-frame factor vpx_c / Ntot per component (0: off)
-    const float2* kA32; // the three k axes rounded to FP32 as pairs (k[i], k[(n - i) % n])
+    const double2* kA64;   // k along A, X, O as pairs (k[i], k[(n - i) % n]): k and the
+    const double2* kX64;   // index-negated partner in one load
+    const double2* kO64;
+    const float2* kA32;    // the same pairs rounded to FP32 (the mixed form)
     const float2* kX32;
     const float2* kO32;
 };
@@ -313,39 +313,42 @@ template <int N, int V>
 __device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastArgs& a, const TileAt& at)
 {
     constexpr int E = Plan<N, V>::E;
-    const gp<const double> kA = glob(a.kA), kX = glob(a.kX), kO = glob(a.kO);
-    const double kx = kX[at.ix], ko = kO[at.outer];
+    // k and its index-negated partner k' come from one 16-byte load
+    const gp<const double2> kA = glob(a.kA64);
+    const double2 kxp2 = glob(a.kX64)[at.ix], kop2 = glob(a.kO64)[at.outer];
+    const double kx = kxp2.x, ko = kop2.x;
     const double kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
         const double w = vpx_of(a, at.ch) * kx;   // moving frame: - vpx k_x / N
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const double ka = kA[elemB<N, V>(t, q)];
+            const double ka = kA[elemB<N, V>(t, q)].x;
             const double k2 = kxo2 + ka * ka;
             // w = 0 leaves (lap_scale v) k2 unchanged (a zero product is exact)
             v[q] = double2{(a.lap_scale * v[q].x) * k2 - w * v[q].x, (a.lap_scale * v[q].y) * k2 - w * v[q].y};
         }
         return;
     }
-    const gp<const double> wt = glob((const double*)a.wtab);
-    const int nxf = a.s.nx, nO = a.s.nouter;
-    const double kxp = kX[(nxf - at.ix) % nxf], kop = kO[(nO - (int)at.outer) % nO];
+    // Vt rows are addressed from one base: every in-grid offset fits 32 bits.
+    const gp<const double> wt = (a.wtab && at.valid) ? glob((const double*)a.wtab) + at.sp_off : nullptr;
+    const double kxp = kxp2.y, kop = kop2.y;
     const double num0 = kx * a.dX + ko * a.dO;
     const double num0p = kxp * a.dX + kop * a.dO;
     const double kxo2p = kxp * kxp + kop * kop;
     const bool xo_neg = (kxp == -kx) && (kop == -ko);
+    const int st = (int)a.s.stride;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int ea = elemB<N, V>(t, q);
-        const double ka = kA[ea];
         double vt = 0.0;
         if (a.c_dd != 0.0) {
+            const double2 kk = kA[ea];
+            const double ka = kk.x, kap = kk.y;
             vt = vdd_of(a.c_dd, num0 + ka * a.dA, kxo2 + ka * ka);
-            const double kap = kA[(N - ea) % N];
             if (!(xo_neg && kap == -ka))
                 vt = 0.5 * (vt + vdd_of(a.c_dd, num0p + kap * a.dA, kxo2p + kap * kap));
         }
-        if (wt && at.valid) vt = vt + wt[at.sp_off + (long long)ea * a.s.stride];
+        if (wt) vt = vt + wt[ea * st];
         const double f = vt * a.inv_ntot;
         v[q] = double2{v[q].x * f, v[q].y * f};
     }
