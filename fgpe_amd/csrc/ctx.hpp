@@ -13,6 +13,9 @@
 
 namespace fgpe {
 
+// Capacity of the MI355X memory-side cache (Infinity Cache).
+constexpr size_t kMallBytes = 256ull << 20;
+
 int fail(int code, const std::string& msg);
 
 #define FGPE_HIPCHK(expr)                                                                     \
@@ -112,6 +115,14 @@ struct fgpe_ctx {
 
     bool mixed() const { return prec == FGPE_PREC_MIXED; }
     size_t tsz() const { return mixed() ? 4 : 8; }
+    // Bytes one time step streams through: per component the FP64 state, the
+    // stage state and accumulator (bulk complex) and the trap (bulk real),
+    // plus the spectral channels.
+    size_t working_set() const
+    {
+        const size_t comp = 16 + 2 * (2 * tsz()) + tsz();
+        return (size_t)ntot * (hp.ncomp * comp + (size_t)nch * 2 * tsz());
+    }
 };
 
 namespace fgpe {

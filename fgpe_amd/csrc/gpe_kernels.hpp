@@ -515,28 +515,34 @@ template <typename T, int N, int V, int NC, int MODE> constexpr int xstage_waves
 }
 
 // Accesses of the x stage's pointwise operands (state, stage state, RK
-// accumulator, trap) are non-temporal: these streams are read or written
-// once per stage, and left to allocate in L2 and the memory-side cache they
-// evict the spectral channels that the x stage itself and the passes around
-// it re-read (measured at 256^3: x stage -8%, step +5% FP64 / +6% mixed).
-// Every access of these arrays in the kernel goes through here, so they are
-// always reached through the same (native vector) type.
-template <typename V> __device__ __forceinline__ V ldn(gp<V> p, long long i)
+// accumulator, trap).  With NT they are non-temporal: these streams are read
+// or written once per stage, and when the working set is larger than the
+// 256 MB memory-side cache, letting them allocate there evicts the spectral
+// channels that the x stage itself and the passes around it re-read
+// (measured at 256^3: x stage -8%, step +5% FP64 / +6% mixed).  When it all
+// fits, the hint only sends hits to HBM, so the launcher picks NT from the
+// working-set size.  Every access of these arrays in the kernel goes through
+// here, so they are always reached through the same (native vector) type.
+template <bool NT, typename V> __device__ __forceinline__ V ldn(gp<const V> p, long long i)
 {
-    if constexpr (std::is_arithmetic_v<V>) {
+    if constexpr (!NT) {
+        return p[i];
+    } else if constexpr (std::is_arithmetic_v<V>) {
         return __builtin_nontemporal_load(&p[i]);
     } else {
         using NV = typename V::Native_vec_;
         return __builtin_bit_cast(V, __builtin_nontemporal_load((gp<const NV>)(&p[i])));
     }
 }
-template <typename V> __device__ __forceinline__ V ldn(gp<const V> p, long long i)
+template <bool NT, typename V> __device__ __forceinline__ V ldn(gp<V> p, long long i)
 {
-    return __builtin_nontemporal_load(&p[i]);
+    return ldn<NT, V>((gp<const V>)p, i);
 }
-template <typename V> __device__ __forceinline__ void stn(gp<V> p, long long i, const V& v)
+template <bool NT, typename V> __device__ __forceinline__ void stn(gp<V> p, long long i, const V& v)
 {
-    if constexpr (std::is_arithmetic_v<V>) {
+    if constexpr (!NT) {
+        p[i] = v;
+    } else if constexpr (std::is_arithmetic_v<V>) {
         __builtin_nontemporal_store(v, &p[i]);
     } else {
         using NV = typename V::Native_vec_;
@@ -553,7 +559,7 @@ __device__ __forceinline__ UserTermIn user_in(const XArgs<T>& a, long long g, in
     return UserTermIn{a.t, (int)(g - r * N), (int)(r % a.ny), (int)(r / a.ny), c, pr, pi, n, no, a.user};
 }
 
-template <typename T, int N, int V, int NC, bool DENS, int MODE>
+template <typename T, int N, int V, int NC, bool DENS, int MODE, bool NT = false>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 __attribute__((amdgpu_waves_per_eu(xstage_waves<T, N, V, NC, MODE>())))
 k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict__ tab)
@@ -638,11 +644,11 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     double2 po = double2{0.0, 0.0};
                     double2 ps;
                     if (from_p0) {
-                        const double2 r = ldn(P0[c], g);
+                        const double2 r = ldn<NT>(P0[c], g);
                         po = double2{sig * r.x, sig * r.y};
                         ps = po;
                     } else {
-                        ps = to_d2<T>(ldn(PS[c], g));
+                        ps = to_d2<T>(ldn<NT>(PS[c], g));
                     }
                     const double n = ps.x * ps.x + ps.y * ps.y;
                     double no = 0.0;
@@ -650,10 +656,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (c == 0) {
                             double2 o;
                             if (from_p0) {
-                                const double2 r = ldn(P0[1], g);
+                                const double2 r = ldn<NT>(P0[1], g);
                                 o = double2{sig1 * r.x, sig1 * r.y};
                             } else {
-                                o = to_d2<T>(ldn(PS[1], g));
+                                o = to_d2<T>(ldn<NT>(PS[1], g));
                             }
                             no = o.x * o.x + o.y * o.y;
                             nsave[q] = n;
@@ -661,8 +667,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             no = nsave[q];
                         }
                     }
-                    const double vr = (double)ldn(Vr[c], g);
-                    const double vi = Vi[c] ? (double)ldn(Vi[c], g) : 0.0;
+                    const double vr = (double)ldn<NT>(Vr[c], g);
+                    const double vi = Vi[c] ? (double)ldn<NT>(Vi[c], g) : 0.0;
                     const double2 ke = to_d2<T>(v[q]);
                     double rX = (vr + g4s * n);
                     if (NC == 2) rX = rX + g4o * no;
@@ -688,12 +694,12 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
                         sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
                     } else if (MODE == MODE_HB) {
-                        const double2 old = to_d2<T>(ldn(PS[c], g));
+                        const double2 old = to_d2<T>(ldn<NT>(PS[c], g));
                         const double m = a.mu ? glob(a.mu)[c] : 0.0;
                         const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
                                                    (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
-                        stn(PS[c], g, from_d2<T>(ps));
-                        stn(P0[c], g, nw);
+                        stn<NT>(PS[c], g, from_d2<T>(ps));
+                        stn<NT>(P0[c], g, nw);
                         sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
                         sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
                         sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
@@ -701,7 +707,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
                     } else {
                         if (a.stage != 1) {
-                            const double2 r = ldn(P0[c], g);
+                            const double2 r = ldn<NT>(P0[c], g);
                             po = double2{sig * r.x, sig * r.y};
                         }
                         if (a.stage < 4) {
@@ -709,21 +715,21 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
                                                        po.y + (a.cr * h.y + a.ci * h.x)};
                             if (a.stage == 1) {
-                                stn(K[c], g, from_d2<T>(h));
+                                stn<NT>(K[c], g, from_d2<T>(h));
                             } else {
-                                const double2 k = to_d2<T>(ldn(K[c], g));
-                                stn(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
+                                const double2 k = to_d2<T>(ldn<NT>(K[c], g));
+                                stn<NT>(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
                             }
                             const T2 st = from_d2<T>(nw);
-                            stn(PSo[c] ? PSo[c] : PS[c], g, st);
+                            stn<NT>(PSo[c] ? PSo[c] : PS[c], g, st);
                             v[q] = st;
                         } else {
                             // FinalRKStep: psi = old + c*(k + h)/6
-                            const double2 k = to_d2<T>(ldn(K[c], g));
+                            const double2 k = to_d2<T>(ldn<NT>(K[c], g));
                             const double aX = k.x + h.x, aY = k.y + h.y;
                             const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
                                                        po.y + (a.cr * aY + a.ci * aX) / 6.0};
-                            stn(P0[c], g, nw);
+                            stn<NT>(P0[c], g, nw);
                             v[q] = from_d2<T>(nw);
                             if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                         }
@@ -741,10 +747,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     const long long g = row0 + elemA<N, V>(t, q);
                     float2 ps;
                     if (a.stage == 1) {
-                        const double2 r = ldn(P0[c], g);
+                        const double2 r = ldn<NT>(P0[c], g);
                         ps = float2{(float)(sig * r.x), (float)(sig * r.y)};
                     } else {
-                        ps = ldn(PS[c], g);
+                        ps = ldn<NT>(PS[c], g);
                     }
                     const float n = ps.x * ps.x + ps.y * ps.y;
                     float no = 0.f;
@@ -752,10 +758,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                         if (c == 0) {
                             float2 o;
                             if (a.stage == 1) {
-                                const double2 r = ldn(P0[1], g);
+                                const double2 r = ldn<NT>(P0[1], g);
                                 o = float2{(float)(sig1 * r.x), (float)(sig1 * r.y)};
                             } else {
-                                o = ldn(PS[1], g);
+                                o = ldn<NT>(PS[1], g);
                             }
                             no = o.x * o.x + o.y * o.y;
                             nsave[q] = n;
@@ -763,8 +769,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                             no = (float)nsave[q];
                         }
                     }
-                    const float vr = (float)ldn(Vr[c], g);
-                    const float vi = Vi[c] ? (float)ldn(Vi[c], g) : 0.f;
+                    const float vr = (float)ldn<NT>(Vr[c], g);
+                    const float vi = Vi[c] ? (float)ldn<NT>(Vi[c], g) : 0.f;
                     const float2 ke = v[q];
                     float rX = vr + f4s * n;
                     if (NC == 2) rX = rX + f4o * no;
@@ -777,25 +783,25 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
                     if (a.stage < 4) {
                         float2 po = ps;
                         if (a.stage != 1) {
-                            const double2 r = ldn(P0[c], g);
+                            const double2 r = ldn<NT>(P0[c], g);
                             po = float2{(float)(sig * r.x), (float)(sig * r.y)};
                         }
                         const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
                         if (a.stage == 1) {
-                            stn(K[c], g, h);
+                            stn<NT>(K[c], g, h);
                         } else {
-                            const float2 k = ldn(K[c], g);
-                            stn(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
+                            const float2 k = ldn<NT>(K[c], g);
+                            stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
                         }
-                        stn(PSo[c] ? PSo[c] : PS[c], g, nw);
+                        stn<NT>(PSo[c] ? PSo[c] : PS[c], g, nw);
                         v[q] = nw;
                     } else {
-                        const float2 k = ldn(K[c], g);
+                        const float2 k = ldn<NT>(K[c], g);
                         const float ax = k.x + h.x, ay = k.y + h.y;
                         const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
-                        const double2 r = ldn(P0[c], g);
+                        const double2 r = ldn<NT>(P0[c], g);
                         const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
-                        stn(P0[c], g, nw);
+                        stn<NT>(P0[c], g, nw);
                         v[q] = from_d2<T>(nw);
                         if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                     }

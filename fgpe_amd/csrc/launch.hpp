@@ -77,6 +77,18 @@ int xstage_t(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
     XArgs<T> b = a;
     b.rev = (c->order >> 3) & 1;
     TScope ts(c, FGPE_KCLASS_XSTAGE);
+    // Non-temporal operand streams when the working set exceeds the
+    // memory-side cache: a context's own (time-stepping modes, long rows), or
+    // a batched heavy-ball launch, whose groups share the cache with every
+    // other group of the sweep.
+    if constexpr ((N >= 256 && MODE == MODE_RK) || MODE == MODE_HB) {
+        if (c->working_set() * (size_t)ninst > kMallBytes || (MODE == MODE_HB && tab)) {
+            hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE, true>), grid, block, 0, c->stream, b,
+                               tw_of<T>(c, 0), (const XArgs<T>*)tab);
+            FGPE_LAUNCHCHK("k_xstage");
+            return FGPE_OK;
+        }
+    }
     hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, b, tw_of<T>(c, 0),
                        (const XArgs<T>*)tab);
     FGPE_LAUNCHCHK("k_xstage");
