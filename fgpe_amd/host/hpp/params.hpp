@@ -52,6 +52,9 @@ private:
     void InitializeParams(const Params* par);
 
 public:
+    // the soft-core droplet-lattice box (hamiltonian "soft"): lengths made a
+    // whole number of inter-droplet distances
+    void RedefineLatticeSoft(const Params* par);
     double omega[3] = {0, 0, 0};
     double ax[3] = {0, 0, 0};
     double aho = 0, omho = 0;

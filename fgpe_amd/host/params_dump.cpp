@@ -96,10 +96,10 @@ int main(int argc, const char* argv[])
     }
     InputVariables iv(argv[1]);
     if (argc >= 4) return dump_init(iv, argc, argv);
-    std::printf("{");
     if (!iv.getBool("t2component")) {
-        Params p(iv);
-        std::printf("\"ncomp\": 1, \"aho\": %.17g, \"omho\": %.17g, \"a\": %.17g, \"add\": %.17g, \"edd\": %.17g, "
+        Params p(iv);   // before the JSON: its set-up may print (the soft-core box lines)
+        std::cout.flush();
+        std::printf("{\"ncomp\": 1, \"aho\": %.17g, \"omho\": %.17g, \"a\": %.17g, \"add\": %.17g, \"edd\": %.17g, "
                     "\"npart\": %.17g, \"ax\": [%.17g, %.17g, %.17g], ",
                     p.aho, p.omho, p.a, p.add, p.edd, p.npart, p.ax[0], p.ax[1], p.ax[2]);
         std::printf("\"DIM\": %d, \"NX\": [%d, %d, %d], \"XMAX\": [%.17g, %.17g, %.17g], \"DX\": [%.17g, %.17g, "
@@ -110,7 +110,8 @@ int main(int argc, const char* argv[])
                     p.epsilon_e);
     } else {
         Params2 p(iv);
-        std::printf("\"ncomp\": 2, \"aho\": %.17g, \"omho\": %.17g, \"aho2\": %.17g, \"a11\": %.17g, \"a22\": %.17g, "
+        std::cout.flush();
+        std::printf("{\"ncomp\": 2, \"aho\": %.17g, \"omho\": %.17g, \"aho2\": %.17g, \"a11\": %.17g, \"a22\": %.17g, "
                     "\"a12\": %.17g, \"npart1\": %.17g, \"npart2\": %.17g, \"ax1\": [%.17g, %.17g, %.17g], "
                     "\"ax2\": [%.17g, %.17g, %.17g], ",
                     p.aho1, p.omho1, p.aho2, p.a11, p.a22, p.a12, p.npart1, p.npart2, p.ax1[0], p.ax1[1], p.ax1[2],

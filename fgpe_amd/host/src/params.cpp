@@ -1,5 +1,5 @@
-// params.cpp -- Params::InitializeParams / calculateLengthScales
-// (src/params.cu:134-279) and the two-component Params2 (src/params2.cu:
+// params.cpp -- Params::InitializeParams / calculateLengthScales /
+// RedefineLatticeSoft (src/params.cu:35-279) and the two-component Params2 (src/params2.cu:
 // 28-186), including their unit conversion: lengths in a_ho of the geometric
 // mean trap frequency, a in a_ho, dt in ms converted at the call sites.
 #include "../hpp/params.hpp"
@@ -12,6 +12,34 @@ Params& Params::operator=(const Params& other)
     iv = other.getIV();
     InitializeParams(&other);
     return *this;
+}
+
+// src/params.cu:35-131.  The inter-droplet distance a_soft / (akrot / 2 pi)
+// is stretched per axis for the expected lattice, and each box length is
+// rounded to a whole number of those distances along the active axes (BCC's
+// half-cell offset is overwritten at once in the reference, and so here).
+void Params::RedefineLatticeSoft(const Params* par)
+{
+    std::cout << "Old numerical domain lengths: \nLX: " << 2 * XMAX[0] << "\nLY: " << 2 * XMAX[1]
+              << "\nLZ: " << 2 * XMAX[2] << std::endl;
+    const InputVariables& v = par->iv;
+    const double akrot = v.getDouble("akrot") / (2 * pi);
+    const double as = v.getDouble("a_soft");
+    const double idd = as / akrot;
+    const double triangular = 2 * std::sqrt(3.) / 2.;
+    double d[3] = {idd, idd, idd};
+    const std::string lt = v.getString("Latticetype");
+    if (lt == "triangular" || lt == "HEX") d[1] = idd * triangular;
+    if (lt == "BCC") d[0] = d[1] = d[2] = idd * std::sqrt(2);
+    if (lt == "SC") d[0] = d[1] = d[2] = idd / std::sqrt(2);
+    if (lt == "FCC") d[0] = d[1] = d[2] = idd * std::sqrt(3);
+    const int active = DIM < 1 ? 0 : (DIM > 3 ? 3 : DIM);
+    for (int i = 0; i < active; ++i) {
+        const int n = (int)std::round((2 * XMAX[i]) / d[i]);
+        XMAX[i] = 0.5 * n * d[i];
+    }
+    std::cout << "New numerical domain lengths: \nLX: " << 2 * XMAX[0] << "\nLY: " << 2 * XMAX[1]
+              << "\nLZ: " << 2 * XMAX[2] << std::endl;
 }
 
 void Params::InitializeParams(const Params* par)
@@ -33,10 +61,7 @@ void Params::InitializeParams(const Params* par)
     XMAX[0] = v.getDouble("xmax");
     XMAX[1] = v.getDouble("ymax");
     XMAX[2] = v.getDouble("zmax");
-    if (v.getString("hamiltonian") == "soft") {
-        std::cerr << "#WARNING the soft-core lattice box redefinition (RedefineLatticeSoft) is not part "
-                     "of this build; the box is used as given" << std::endl;
-    }
+    if (v.getString("hamiltonian") == "soft") RedefineLatticeSoft(par);
     for (int i = 0; i < 3; ++i) DX[i] = 2 * XMAX[i] / NX[i];
     omega[0] = 2 * pi * v.getDouble("omx");
     omega[1] = 2 * pi * v.getDouble("omy");

@@ -95,3 +95,26 @@ def test_cli_exit_codes(tmp_path):
     r = subprocess.run([GPE, write_input(tmp_path, src.replace('"bec"', '"nosuch"'))], capture_output=True,
                        text=True)
     assert r.returncode == 1 and "No hamiltonian available for: nosuch" in r.stderr
+
+
+@pytest.mark.parametrize("lattice,fx,fy,fz", [("triangular", 1, 3 ** 0.5, 1), ("HEX", 1, 3 ** 0.5, 1),
+                                              ("BCC", 2 ** 0.5, 2 ** 0.5, 2 ** 0.5),
+                                              ("SC", 2 ** -0.5, 2 ** -0.5, 2 ** -0.5),
+                                              ("FCC", 3 ** 0.5, 3 ** 0.5, 3 ** 0.5), ("stripes", 1, 1, 1)])
+def test_soft_lattice_box_redefinition(tmp_path, lattice, fx, fy, fz):
+    """Params::RedefineLatticeSoft (src/params.cu:35-131): with hamiltonian
+    "soft" each box length becomes the whole number of inter-droplet
+    distances d = a_soft / (akrot / 2 pi) (stretched per lattice) nearest to
+    it; the box is then converted to code units like any other."""
+    src = open(os.path.join(INPUTS, "im_test_input.txt")).read().replace('"bec"', '"soft"')
+    src += "\nakrot           3.0\na_soft          1.7\nLatticetype     \"%s\"\n" % lattice
+    d = dump(write_input(tmp_path, src))
+    kv = parse(os.path.join(INPUTS, "im_test_input.txt"))
+    idd = 1.7 / (3.0 / (2 * PI))   # in the input's length unit, like xmax
+    for i, (key, f) in enumerate(zip(("xmax", "ymax", "zmax"), (fx, fy, fz))):
+        step = idd * f
+        L = 2 * float(kv[key])
+        n = round(L / step)
+        assert abs(n * step - L) <= 0.5 * step + 1e-12
+        assert d["XMAX"][i] == pytest.approx(0.5 * n * step / d["aho"], rel=1e-14)
+        assert d["DX"][i] == pytest.approx(2 * d["XMAX"][i] / d["NX"][i], rel=1e-14)
