@@ -478,3 +478,26 @@ def test_rk4_dipolar_softcore_all_plans_f64(shape):
     nrm = orc.norm(c.grid, ref)
     mu, en = orc.obs2(c.grid, nrm, ref, hmu, hen)
     assert abs(obs[2] - en) < 1e-12 * abs(en), (shape, obs[2], en)
+
+
+def test_rk4_nontemporal_x_stage_f64():
+    """256x256x64 FP64 dipolar: the per-step working set (about 370 MB) is
+    larger than the memory-side cache, so the x stage runs its non-temporal
+    instantiation; two RK4 steps against the oracle at the drop-in bar."""
+    from fgpe_amd import configs
+    h = dict(HAMS["dipolar"])
+    c = Case(256, 256, 64, L=(16.0, 16.0, 8.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 2.0))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(configs.harmonic_trap(c.spec, (1, 1, 2)))) +
+                            0.01 * c.rand_psi(9), c.dv)
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        print("working set MB", 256 * 256 * 64 * (56 + 2 * 16) / 2 ** 20, flush=True)
+        ctx.rk4_steps(5e-4, 2)
+        gpu = ctx.download_psi()
+    orc.set_threads(16)
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 2)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
