@@ -45,3 +45,20 @@ def test_error_path_without_device_work():
     assert b"precision" in L.fgpe_last_error()
     assert L.fgpe_ctx_create(63, 64, 64, 0, ctypes.byref(h)) == -2
     assert L.fgpe_ctx_destroy(None) == 0
+
+
+def test_initialiser_and_user_term_argument_checks():
+    """The device-initialiser and user-term entry points reject bad
+    arguments before any device work."""
+    from fgpe_amd import _lib
+    L = _lib.load()
+    g = _lib.Geom.of((1.0, 1.0, 1.0), (0.1, 0.1, 0.1))
+    ax = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
+    assert L.fgpe_psi_init(None, 0, 0, ctypes.byref(g), ax, 0.0, 0.0, 1) == -1
+    assert L.fgpe_psi_normalize(None, 0, 1.0, ctypes.byref(g), None) == -1
+    assert L.fgpe_vext_init(None, 0, 0, ctypes.byref(g), ax) == -1
+    assert L.fgpe_vext_commit(None, 0, None) == -1
+    assert L.fgpe_user_term(None) == -1
+    v = ctypes.c_int(7)
+    assert L.fgpe_user_term(ctypes.byref(v)) == 0
+    assert v.value == 0   # the shipped library compiles no user term
