@@ -101,7 +101,8 @@ def run_adjoint(torch, args):
     the trap V, a and a_dd through the discrete adjoint, for the dipolar GPE
     (omega = (1,1,0.25), g = 200, eps_dd = 1.3, box 16 x 16 x 32) on 128^3 over
     T RK4 steps of dt = 5e-4 with a checkpoint every 45 steps; checked by
-    central differences (two smooth V directions, a and a_dd)."""
+    central differences along --adjoint-dirs seeded smooth V directions
+    (16 by default), a and a_dd."""
     from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, Context, configs
     torch.cuda.set_device(0)
     n, dt, every = args.adjoint_steps, 5e-4, 45
@@ -115,6 +116,9 @@ def run_adjoint(torch, args):
     kx, ky, kz = grid.kaxes()
     dv = grid.dv
 
+    import functools
+
+    @functools.lru_cache(maxsize=None)
     def smooth(seed):
         u = configs.splitmix64_uniform(seed, grid.npoints).reshape(grid.shape)
         F = np.fft.fftn(u)
@@ -158,8 +162,9 @@ def run_adjoint(torch, args):
             # against the FP64 adjoint: finite differences of an FP32-bulk forward are dominated by its
             # rounding noise at these step sizes
             ref = out["f64"]
-            cmp = [("V", float(np.sum(gVn * smooth(4 + j))), ref["directional"][j]) for j in range(2)]
-            cmp += [("a", gs[0], ref["directional"][2]), ("a_dd", gs[1], ref["directional"][3])]
+            nd = args.adjoint_dirs
+            cmp = [("V", float(np.sum(gVn * smooth(4 + j))), ref["directional"][j]) for j in range(nd)]
+            cmp += [("a", gs[0], ref["directional"][nd]), ("a_dd", gs[1], ref["directional"][nd + 1])]
             checks = [{"dir": lab, "mixed": m, "f64": r, "rel": abs(m - r) / abs(r)} for lab, m, r in cmp]
             for x in checks:
                 print(f"#   {x['dir']}: mixed {x['mixed']:.10e} f64 {x['f64']:.10e} rel {x['rel']:.2e}",
@@ -170,8 +175,8 @@ def run_adjoint(torch, args):
         # central differences of the same form's forward
         checks = []
         h = 1e-4
-        dirs = [("V", smooth(4 + j), 0.0, 0.0) for j in range(2)] + [("a", None, h * a, 0.0),
-                                                                      ("a_dd", None, 0.0, h * a_dd)]
+        dirs = [("V", smooth(4 + j), 0.0, 0.0) for j in range(args.adjoint_dirs)] + [
+            ("a", None, h * a, 0.0), ("a_dd", None, 0.0, h * a_dd)]
         for lab, dVx, pa, pd in dirs:
             Js = []
             for sgn in (1.0, -1.0):
@@ -353,6 +358,8 @@ def main():
                     help="time adjoint gradients (128^3 dipolar, 2000 RK4 steps, checkpoint every 45) "
                          "in both forms, with central finite-difference checks")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-dirs", type=int, default=16,
+                    help="seeded smooth V directions checked by central differences")
     ap.add_argument("--init", action="store_true",
                     help="time the device initialisers (state and trap) at --grid instead")
     ap.add_argument("--sweep-n", type=int, default=64)
