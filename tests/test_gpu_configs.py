@@ -1,0 +1,118 @@
+"""The benchmark configurations at their full grid sizes.
+
+Where the oracle finishes in seconds the device is compared with it at the
+full size over a shortened horizon (1e-12); over the full horizon, where the
+oracle would take minutes to hours, the checks are the properties the
+equations fix independently of size: imaginary time lowers the energy at
+fixed norm, real-time RK4 never raises the norm and, on smooth states,
+conserves norm and energy to its truncation error, and the mixed-precision
+form stays within the north-star bound (1e-6 relative energy) of the FP64
+form."""
+import numpy as np
+import pytest
+
+from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, FGPE_IMAG_RK4_RENORM, Context, configs
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def ctx_for(w, prec):
+    g = w.grid
+    kx, ky, kz = g.kaxes()
+    c = Context(g.nx, g.ny, g.nz, prec)
+    c.set_kspace(kx, ky, kz, g.dv)
+    kw = dict(ncomp=w.ncomp, a=w.a)
+    if w.a_dd:
+        kw.update(a_dd=w.a_dd, dip=w.dip)
+    if w.sc_U:
+        kw.update(sc_U=w.sc_U, sc_R=w.sc_R, sc_dim=3)
+    c.set_hamiltonian(**kw)
+    c.upload_psi(w.initial_state())
+    c.upload_vext(w.trap())
+    return c
+
+
+def oracle_grid(w):
+    g = w.grid
+    kx, ky, kz = g.kaxes()
+    return orc.Grid(g.nx, g.ny, g.nz, kx, ky, kz, g.dv)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_config1_imaginary_rk4_64cubed():
+    """64^3 contact GPE, imaginary-time RK4 renormalised to N = 1, dt = 1e-3:
+    100 steps against the oracle at the full size, then the full 2000 steps
+    lower the energy at every checkpoint with the norm held at 1."""
+    w = configs.contact_imag_64()
+    print(f"config 1: {w.name}, {w.grid.shape}", flush=True)
+    og = oracle_grid(w)
+    orc.set_threads(16)
+    ref = orc.rk4(og, orc.Ham(1, w.a), w.initial_state(), w.trap(), w.dt, 100, imaginary=True, npart=1.0)
+    with ctx_for(w, FGPE_PREC_F64) as c:
+        c.imag_steps(w.dt, 100, scheme=FGPE_IMAG_RK4_RENORM, npart=1.0)
+        assert rel(c.download_psi(), ref) < 1e-12
+        energies = [c.observables()[2]]
+        for _ in range(19):
+            c.imag_steps(w.dt, 100, scheme=FGPE_IMAG_RK4_RENORM, npart=1.0)
+            o = c.observables()
+            assert abs(o[0] - 1.0) < 1e-12
+            energies.append(o[2])
+    assert all(e1 < e0 for e0, e1 in zip(energies, energies[1:])), energies
+    assert energies[0] - energies[-1] > 1e-4
+
+
+def test_config2_dipolar_real_time_full_grid():
+    """128 x 128 x 256 dipolar real-time RK4 (eps_dd = 1.3), dt = 5e-4: two
+    steps against the oracle at the full size; over 400 steps the norm never
+    grows (|R(iz)| <= 1 for RK4 at |z| <= 2 sqrt 2, and the seeded white noise
+    puts a little weight on the damped high-k modes), and the mixed form stays
+    within 1e-6 of FP64 in energy."""
+    w = configs.dipolar_real(128, 128, 256, nsteps=400)
+    print(f"config 2: {w.name}", flush=True)
+    og = oracle_grid(w)
+    orc.set_threads(16)
+    vt = orc.vdd(og, w.a_dd, w.dip)
+    ref = orc.rk4(og, orc.Ham(1, w.a, vtilde=vt), w.initial_state(), w.trap(), w.dt, 2)
+    out = {}
+    for prec in (FGPE_PREC_F64, FGPE_PREC_MIXED):
+        with ctx_for(w, prec) as c:
+            o0 = c.observables(0.0)
+            c.rk4_steps(w.dt, 2)
+            if prec == FGPE_PREC_F64:
+                assert rel(c.download_psi(), ref) < 1e-12
+            c.rk4_steps(w.dt, w.nsteps - 2, 2 * w.dt)
+            o1 = c.observables(w.nsteps * w.dt)
+            out[prec] = (o0, o1, c.download_psi())
+    o0, o1, p64 = out[FGPE_PREC_F64]
+    assert o1[0] <= o0[0] and o0[0] - o1[0] < 1e-4 * o0[0]   # norm: non-increasing, small
+    m0, m1, pm = out[FGPE_PREC_MIXED]
+    assert abs(m1[2] - o1[2]) < 1e-6 * abs(o1[2])
+    assert np.sqrt(np.sum(np.abs(pm - p64) ** 2) / np.sum(np.abs(p64) ** 2)) < 1e-5
+
+
+def test_config3_two_component_softcore_256cubed():
+    """Two coupled components at 256^3 with the soft-core term: 50 RK4 steps
+    conserve each component's norm and the total energy, in FP64 and in the
+    mixed form (the configuration's FP32 bulk), which agree to 1e-6."""
+    w = configs.twocomp_softcore_256(256, nsteps=50)
+    print(f"config 3: {w.name}", flush=True)
+    out = {}
+    for prec in (FGPE_PREC_F64, FGPE_PREC_MIXED):
+        with ctx_for(w, prec) as c:
+            o0 = c.observables(0.0)
+            c.rk4_steps(w.dt, w.nsteps)
+            o1 = c.observables(w.nsteps * w.dt)
+            out[prec] = (o0, o1)
+    for prec, (o0, o1) in out.items():
+        for comp in range(2):
+            assert abs(o1[3 * comp] - o0[3 * comp]) < 1e-8 * o0[3 * comp], (prec, comp)
+        e0 = o0[0] * o0[2] + o0[3] * o0[5]
+        e1 = o1[0] * o1[2] + o1[3] * o1[5]
+        assert abs(e1 - e0) < 1e-6 * abs(e0), prec
+    f, m = out[FGPE_PREC_F64][1], out[FGPE_PREC_MIXED][1]
+    for comp in range(2):
+        assert abs(m[3 * comp + 2] - f[3 * comp + 2]) < 1e-6 * abs(f[3 * comp + 2])
