@@ -100,6 +100,7 @@ struct fgpe_ctx {
     bool adj_ready = false;
     double* xbuf = nullptr;   // x cell centres of fgpe_calc_cos
     double* vstage[2] = {nullptr, nullptr};   // FP64 trap being assembled (fgpe_vext_*)
+    double* axbuf = nullptr;                   // per-axis factor tables of the initialisers
 
     void* tw64[3] = {nullptr, nullptr, nullptr};
     void* tw32[3] = {nullptr, nullptr, nullptr};

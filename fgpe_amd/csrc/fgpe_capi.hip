@@ -1045,7 +1045,7 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->k64[0], c->k64[1], c->k64[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
-                    c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
+                    c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->axbuf, c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
                     c->lam[1], c->lam[2], c->pacc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

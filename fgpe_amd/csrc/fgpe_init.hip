@@ -6,7 +6,13 @@
 // multiply into an add.  The kernels keep its operation order and turn
 // contraction off, so they reproduce its values up to the last-ulp
 // differences of exp, cos, sin and tanh between the device and host math
-// libraries.  The one deliberate difference is the noise: the reference
+// libraries.  Factors that depend on one axis only (the trap's per-axis
+// terms, the vortex phase, the cosine seeds, the box profile, the Gaussian's
+// squared coordinates) are evaluated once per axis index on the host, with
+// the same operations and math library as the reference's loops, and the
+// kernels combine them per point in the reference's order; what is left per
+// point is the Gaussian's exp and the lattice profiles.  The one deliberate
+// difference is the noise: the reference
 // draws it from a sequential std::mt19937 stream, here every point draws its
 // own Philox4x32-10 block (counter-based, Salmon, Moraes, Dror and Shaw,
 // "Parallel random numbers: as easy as 1, 2, 3", SC'11), so the kernels need
@@ -25,7 +31,6 @@ using namespace fgpe;
 
 namespace {
 
-constexpr double kPi = 3.141592653589793;   // hpp/header.hpp:28
 constexpr int IB = 256;                     // threads per block
 constexpr int IPT = 4;                      // points per thread
 constexpr long long IPB = (long long)IB * IPT;
@@ -98,7 +103,8 @@ __device__ __forceinline__ void block_partial(double s, double* partials)
 // the one of std::complex (__muldc3 order): (a + ib)(e + i0) = (a e - b 0,
 // a 0 + b e).
 template <int KIND>
-__global__ void __launch_bounds__(IB) k_psi_init(double2* __restrict__ psi, GeomD g, double ax0, double ax1, double ax2,
+__global__ void __launch_bounds__(IB) k_psi_init(double2* __restrict__ psi, GeomD g, const double* __restrict__ sx,
+                                                 const double* __restrict__ sy, const double* __restrict__ sz,
                                                  double rr, double ci, uint32_t k0, uint32_t k1,
                                                  double* __restrict__ partials)
 {
@@ -114,12 +120,9 @@ __global__ void __launch_bounds__(IB) k_psi_init(double2* __restrict__ psi, Geom
         const double a = 1.0 + rr * u1, b = 0.0 + ci * u2;
         double re = a, im = b;
         if (KIND != FGPE_INIT_VOID) {
-            const double y = centre(g, 1, p.iy) / ax1, z = centre(g, 2, p.iz) / ax2;
-            double q = (y * y) + (z * z);
-            if (KIND == FGPE_INIT_HARMONIC) {
-                const double x = centre(g, 0, p.ix) / ax0;
-                q = q + (x * x);
-            }
+            // (y/ay)^2 + (z/az)^2 [+ (x/ax)^2], each square from the axis table
+            double q = sy[p.iy] + sz[p.iz];
+            if (KIND == FGPE_INIT_HARMONIC) q = q + sx[p.ix];
             const double e = exp(-0.5 * 0.1 * q);
             re = a * e - b * 0.0;
             im = a * 0.0 + b * e;
@@ -152,23 +155,21 @@ __global__ void __launch_bounds__(IB) k_psi_noise(double2* __restrict__ psi, lon
 }
 
 // psi *= exp(i ncirc 2 pi x / (2 xmax_x)) (Psi::imprintVortexTube,
-// src/psi.cu:155-164)
-__global__ void __launch_bounds__(IB) k_psi_vortex(double2* __restrict__ psi, GeomD g, double ncirc,
+// src/psi.cu:155-164); cs[ix] = (cos, sin) of the phase
+__global__ void __launch_bounds__(IB) k_psi_vortex(double2* __restrict__ psi, GeomD g, const double2* __restrict__ cs,
                                                    const double* __restrict__ pend)
 {
     const double sc = *pend;
-    const double xlen = 2 * g.xmax[0];
     const long long b0 = (long long)blockIdx.x * IPB + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const long long i = b0 + (long long)j * IB;
         if (i >= g.n) break;
         const Pt p = point(g, i);
-        const double x = centre(g, 0, p.ix);
-        const double c = cos(ncirc * 2 * kPi / xlen * x), s = sin(ncirc * 2 * kPi / xlen * x);
+        const double2 w = cs[p.ix];
         const double2 v = psi[i];
         const double re = sc * v.x, im = sc * v.y;
-        psi[i] = double2{c * re - s * im, c * im + s * re};
+        psi[i] = double2{w.x * re - w.y * im, w.x * im + w.y * re};
     }
 }
 
@@ -277,9 +278,12 @@ __global__ void __launch_bounds__(IB) k_psi_rescale(double2* __restrict__ psi, l
 
 __global__ void k_set_one(double* p) { *p = 1.0; }
 
-// Vext::initHarmonic / initTube / initVoid (src/vext.cu:60-118)
+// Vext::initHarmonic / initTube / initVoid (src/vext.cu:60-118):
+// 0.5 (tx + ty + tz) with t[i] = (x_i / ax)^2 / ax^2 per axis (no x term for
+// the tube, 0 for void)
 template <int KIND>
-__global__ void __launch_bounds__(IB) k_vext_init(double* __restrict__ v, GeomD g, double ax0, double ax1, double ax2)
+__global__ void __launch_bounds__(IB) k_vext_init(double* __restrict__ v, GeomD g, const double* __restrict__ tx,
+                                                  const double* __restrict__ ty, const double* __restrict__ tz)
 {
     const long long b0 = (long long)blockIdx.x * IPB + threadIdx.x;
 #pragma unroll
@@ -288,22 +292,17 @@ __global__ void __launch_bounds__(IB) k_vext_init(double* __restrict__ v, GeomD 
         if (i >= g.n) break;
         const Pt p = point(g, i);
         double r = 0;
-        if (KIND != FGPE_INIT_VOID) {
-            const double y = centre(g, 1, p.iy) / ax1, z = centre(g, 2, p.iz) / ax2;
-            if (KIND == FGPE_INIT_HARMONIC) {
-                const double x = centre(g, 0, p.ix) / ax0;
-                r = 0.5 * ((x * x) / (ax0 * ax0) + (y * y) / (ax1 * ax1) + (z * z) / (ax2 * ax2));
-            } else {
-                r = 0.5 * ((y * y) / (ax1 * ax1) + (z * z) / (ax2 * ax2));
-            }
-        }
+        if (KIND == FGPE_INIT_HARMONIC) r = 0.5 * (tx[p.ix] + ty[p.iy] + tz[p.iz]);
+        else if (KIND == FGPE_INIT_TUBE) r = 0.5 * (ty[p.iy] + tz[p.iz]);
         v[i] = r;
     }
 }
 
-// Vext::initBox (src/vext.cu:88-109)
-__global__ void __launch_bounds__(IB) k_vext_box(double* __restrict__ v, GeomD g, double V, double B, double w0,
-                                                 double w1, double w2)
+// Vext::initBox (src/vext.cu:88-109): 0.5 V (bx by bz) with
+// b[i] = tanh(B (x_i - w)) - tanh(B (x_i + w)) per axis
+__global__ void __launch_bounds__(IB) k_vext_box(double* __restrict__ v, GeomD g, double V,
+                                                 const double* __restrict__ bx, const double* __restrict__ by,
+                                                 const double* __restrict__ bz)
 {
     const long long b0 = (long long)blockIdx.x * IPB + threadIdx.x;
 #pragma unroll
@@ -311,24 +310,21 @@ __global__ void __launch_bounds__(IB) k_vext_box(double* __restrict__ v, GeomD g
         const long long i = b0 + (long long)j * IB;
         if (i >= g.n) break;
         const Pt p = point(g, i);
-        const double x = centre(g, 0, p.ix), y = centre(g, 1, p.iy), z = centre(g, 2, p.iz);
-        v[i] = 0.5 * V *
-               ((tanh(B * (x - w0)) - tanh(B * (x + w0))) * (tanh(B * (y - w1)) - tanh(B * (y + w1))) *
-                (tanh(B * (z - w2)) - tanh(B * (z + w2))));
+        v[i] = 0.5 * V * (bx[p.ix] * by[p.iy] * bz[p.iz]);
     }
 }
 
-// V += A cos(n 2 pi / L x), L = 2 xmax_x (src/vext.cu:122-179)
-__global__ void __launch_bounds__(IB) k_vext_cos(double* __restrict__ v, GeomD g, double A, double nn)
+// V += A cos(n 2 pi / L x), L = 2 xmax_x (src/vext.cu:122-179); cx[ix] is
+// that whole term
+__global__ void __launch_bounds__(IB) k_vext_cos(double* __restrict__ v, GeomD g, const double* __restrict__ cx)
 {
-    const double L = 2 * g.xmax[0];
     const long long b0 = (long long)blockIdx.x * IPB + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const long long i = b0 + (long long)j * IB;
         if (i >= g.n) break;
         const Pt p = point(g, i);
-        v[i] += A * cos(nn * 2 * kPi / L * centre(g, 0, p.ix));
+        v[i] += cx[p.ix];
     }
 }
 
@@ -388,6 +384,44 @@ int vstage_of(fgpe_ctx* c, int comp, double** v)
     return FGPE_OK;
 }
 
+// -XMAX + i DX + DX/2 on the host (src/psi.cu:134-150)
+double centre_h(const fgpe_geom* g, int a, int i) { return -g->xmax[a] + i * g->dx[a] + 0.5 * g->dx[a]; }
+
+// Per-axis factor tables: f(a, i) for the three axes, laid out x, y, z in
+// the context's axis buffer.  The copy is ordered on the context's stream
+// after the launches that read the previous tables, and the host waits for
+// it, so the next call may rewrite the buffer.
+template <typename F> int axis_tables(fgpe_ctx* c, F f, const double** tab)
+{
+    const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+    const size_t cap = 2 * (size_t)(n0 + n1 + n2);
+    if (!c->axbuf) FGPE_RET(dalloc(c, (void**)&c->axbuf, sizeof(double) * cap));
+    std::vector<double> h((size_t)(n0 + n1 + n2));
+    for (int i = 0; i < n0; ++i) h[i] = f(0, i);
+    for (int i = 0; i < n1; ++i) h[n0 + i] = f(1, i);
+    for (int i = 0; i < n2; ++i) h[n0 + n1 + i] = f(2, i);
+    FGPE_HIPCHK(hipMemcpyAsync(c->axbuf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
+    tab[0] = c->axbuf;
+    tab[1] = c->axbuf + n0;
+    tab[2] = c->axbuf + n0 + n1;
+    return FGPE_OK;
+}
+
+// (cos, sin) pairs per x index, in the same buffer
+template <typename F> int x_pairs(fgpe_ctx* c, F f, const double2** tab)
+{
+    const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+    const size_t cap = 2 * (size_t)(n0 + n1 + n2);
+    if (!c->axbuf) FGPE_RET(dalloc(c, (void**)&c->axbuf, sizeof(double) * cap));
+    std::vector<double2> h((size_t)n0);
+    for (int i = 0; i < n0; ++i) h[i] = f(i);
+    FGPE_HIPCHK(hipMemcpyAsync(c->axbuf, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->stream));
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
+    *tab = (const double2*)c->axbuf;
+    return FGPE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -400,20 +434,27 @@ int fgpe_psi_init(fgpe_ctx* c, int comp, int kind, const fgpe_geom* g, const dou
     if (kind < FGPE_INIT_HARMONIC || kind > FGPE_INIT_VOID) return fail(FGPE_EINVAL, "unknown psi kind");
     if (!c->P0[comp]) FGPE_RET(dalloc(c, (void**)&c->P0[comp], sizeof(double2) * c->ntot));
     const GeomD d = geom_of(c, g);
-    const double a0 = ax ? ax[0] : 1.0, a1 = ax ? ax[1] : 1.0, a2 = ax ? ax[2] : 1.0;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const double* t[3] = {nullptr, nullptr, nullptr};
+    if (kind != FGPE_INIT_VOID) {
+        // (x / ax)^2 per axis, as getX(p, i) / ax[a] squared in the reference
+        FGPE_RET(axis_tables(c, [&](int a, int i) {
+            const double x = centre_h(g, a, i) / ax[a];
+            return x * x;
+        }, t));
+    }
     switch (kind) {
     case FGPE_INIT_HARMONIC:
         hipLaunchKernelGGL(k_psi_init<FGPE_INIT_HARMONIC>, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], d,
-                           a0, a1, a2, rand_re, rand_im, k0, k1, c->partials);
+                           t[0], t[1], t[2], rand_re, rand_im, k0, k1, c->partials);
         break;
     case FGPE_INIT_TUBE:
-        hipLaunchKernelGGL(k_psi_init<FGPE_INIT_TUBE>, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], d, a0,
-                           a1, a2, rand_re, rand_im, k0, k1, c->partials);
+        hipLaunchKernelGGL(k_psi_init<FGPE_INIT_TUBE>, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], d,
+                           t[0], t[1], t[2], rand_re, rand_im, k0, k1, c->partials);
         break;
     default:
-        hipLaunchKernelGGL(k_psi_init<FGPE_INIT_VOID>, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], d, a0,
-                           a1, a2, rand_re, rand_im, k0, k1, c->partials);
+        hipLaunchKernelGGL(k_psi_init<FGPE_INIT_VOID>, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], d,
+                           t[0], t[1], t[2], rand_re, rand_im, k0, k1, c->partials);
     }
     FGPE_LAUNCHCHK("k_psi_init");
     return state_changed(c, comp);
@@ -434,7 +475,13 @@ int fgpe_psi_imprint_vortex_tube(fgpe_ctx* c, int comp, const fgpe_geom* g, doub
 {
     FGPE_RET(check_comp(c, comp, true));
     if (!g) return fail(FGPE_EINVAL, "NULL geometry");
-    hipLaunchKernelGGL(k_psi_vortex, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], geom_of(c, g), ncirc,
+    const double xlen = 2 * g->xmax[0];
+    const double2* cs = nullptr;
+    FGPE_RET(x_pairs(c, [&](int i) {
+        const double x = centre_h(g, 0, i);
+        return double2{std::cos(ncirc * 2 * kPi / xlen * x), std::sin(ncirc * 2 * kPi / xlen * x)};
+    }, &cs));
+    hipLaunchKernelGGL(k_psi_vortex, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->P0[comp], geom_of(c, g), cs,
                        pend_of(c, comp));
     FGPE_LAUNCHCHK("k_psi_vortex");
     return state_changed(c, comp);
@@ -505,13 +552,23 @@ int fgpe_vext_init(fgpe_ctx* c, int comp, int kind, const fgpe_geom* g, const do
     double* v;
     FGPE_RET(vstage_of(c, comp, &v));
     const GeomD d = geom_of(c, g);
-    const double a0 = ax ? ax[0] : 1.0, a1 = ax ? ax[1] : 1.0, a2 = ax ? ax[2] : 1.0;
+    const double* t[3] = {nullptr, nullptr, nullptr};
+    if (kind != FGPE_INIT_VOID) {
+        // pow(x, 2) / pow(ax, 2) with x = getX / ax (src/vext.cu:72-85)
+        FGPE_RET(axis_tables(c, [&](int a, int i) {
+            const double x = centre_h(g, a, i) / ax[a];
+            return (x * x) / (ax[a] * ax[a]);
+        }, t));
+    }
     if (kind == FGPE_INIT_HARMONIC)
-        hipLaunchKernelGGL(k_vext_init<FGPE_INIT_HARMONIC>, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, d, a0, a1, a2);
+        hipLaunchKernelGGL(k_vext_init<FGPE_INIT_HARMONIC>, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, d, t[0], t[1],
+                           t[2]);
     else if (kind == FGPE_INIT_TUBE)
-        hipLaunchKernelGGL(k_vext_init<FGPE_INIT_TUBE>, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, d, a0, a1, a2);
+        hipLaunchKernelGGL(k_vext_init<FGPE_INIT_TUBE>, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, d, t[0], t[1],
+                           t[2]);
     else
-        hipLaunchKernelGGL(k_vext_init<FGPE_INIT_VOID>, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, d, a0, a1, a2);
+        hipLaunchKernelGGL(k_vext_init<FGPE_INIT_VOID>, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, d, t[0], t[1],
+                           t[2]);
     FGPE_LAUNCHCHK("k_vext_init");
     return FGPE_OK;
 }
@@ -522,7 +579,12 @@ int fgpe_vext_init_box(fgpe_ctx* c, int comp, const fgpe_geom* g, double V, doub
     if (!g || !w) return fail(FGPE_EINVAL, "NULL geometry or box half-widths");
     double* v;
     FGPE_RET(vstage_of(c, comp, &v));
-    hipLaunchKernelGGL(k_vext_box, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, geom_of(c, g), V, B, w[0], w[1], w[2]);
+    const double* t[3] = {nullptr, nullptr, nullptr};
+    FGPE_RET(axis_tables(c, [&](int a, int i) {
+        const double x = centre_h(g, a, i);
+        return std::tanh(B * (x - w[a])) - std::tanh(B * (x + w[a]));
+    }, t));
+    hipLaunchKernelGGL(k_vext_box, dim3(nblocks(c)), dim3(IB), 0, c->stream, v, geom_of(c, g), V, t[0], t[1], t[2]);
     FGPE_LAUNCHCHK("k_vext_box");
     return FGPE_OK;
 }
@@ -532,7 +594,12 @@ int fgpe_vext_add_cos(fgpe_ctx* c, int comp, const fgpe_geom* g, double A, doubl
     FGPE_RET(check_comp(c, comp, false));
     if (!g) return fail(FGPE_EINVAL, "NULL geometry");
     if (!c->vstage[comp]) return fail(FGPE_ESTATE, "no trap being assembled: call fgpe_vext_init first");
-    hipLaunchKernelGGL(k_vext_cos, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->vstage[comp], geom_of(c, g), A, n);
+    const double L = 2 * g->xmax[0];
+    const double* t[3] = {nullptr, nullptr, nullptr};
+    FGPE_RET(axis_tables(c, [&](int a, int i) {
+        return a == 0 ? A * std::cos(n * 2 * kPi / L * centre_h(g, 0, i)) : 0.0;
+    }, t));
+    hipLaunchKernelGGL(k_vext_cos, dim3(nblocks(c)), dim3(IB), 0, c->stream, c->vstage[comp], geom_of(c, g), t[0]);
     FGPE_LAUNCHCHK("k_vext_cos");
     return FGPE_OK;
 }
