@@ -21,6 +21,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "radix_consts.hpp"
 
 namespace fgpe {
@@ -220,11 +222,32 @@ template <int N, int V> __device__ __forceinline__ int elemB(int t, int q)
 
 // Twiddle table entry W_N^m = exp(-2 pi i m / N) (forward), conjugated for the
 // inverse.
-template <int DIR, typename T2> __device__ __forceinline__ T2 twid(const T2* __restrict__ tw, int m)
+template <int DIR, typename T2, typename TP> __device__ __forceinline__ T2 twid(TP tw, int m)
 {
     T2 w = tw[m];
     if (DIR > 0) w.y = -w.y;
     return w;
+}
+
+// A twiddle table staged in LDS (the kernel copies it there once per block).
+// The host pass of the compiler only parses the kernels, and HIP's vector
+// types have no copy from an address-space-qualified reference, so there the
+// table is a plain pointer.
+#ifdef __HIP_DEVICE_COMPILE__
+template <typename T2> using lds_table = __attribute__((address_space(3))) const T2*;
+#else
+template <typename T2> using lds_table = const T2*;
+#endif
+
+// Copies the N-entry twiddle table into the block's LDS array twl (every
+// thread of the block must call it) and returns it as an LDS table, so that
+// the stages read twiddles with ds loads rather than through the vector cache.
+template <typename T2, int N>
+__device__ __forceinline__ lds_table<T2> stage_twiddles(T2* twl, const T2* __restrict__ twg)
+{
+    for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = twg[i];
+    __syncthreads();
+    return (lds_table<T2>)twl;
 }
 
 // Ordering of the stage-to-stage exchange.  A pencil owned by consecutive
@@ -272,8 +295,8 @@ __device__ __forceinline__ void exchange(T2* v, int t, T2* lds)
     exchange_sync<WAVE>();
 }
 
-template <typename T2, int N, int V, int DIR, int S, bool WAVE>
-__device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __restrict__ tw)
+template <typename T2, int N, int V, int DIR, int S, bool WAVE, typename TP>
+__device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, TP tw)
 {
     constexpr int L = Plan<N, V>::L, TT = Plan<N, V>::T, E = N / TT;
     constexpr int r = radix_of<N, V, DIR, S>();
@@ -287,13 +310,13 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __re
             const int k = j % Ns;
 #pragma unroll
             for (int i = 1; i < r; ++i)
-                v[m * r + i] = cmul(v[m * r + i], twid<DIR>(tw, i * k * (N / (Ns * r))));
+                v[m * r + i] = cmul(v[m * r + i], twid<DIR, T2>(tw, i * k * (N / (Ns * r))));
         }
         Dft<r, DIR>::run(v + m * r);
     }
     if constexpr (S + 1 < L) {
         exchange<T2, N, V, DIR, S, WAVE>(v, t, lds);
-        fft_stages<T2, N, V, DIR, S + 1, WAVE>(v, t, lds, tw);
+        fft_stages<T2, N, V, DIR, S + 1, WAVE, TP>(v, t, lds, tw);
     }
 }
 
@@ -302,28 +325,30 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, const T2* __re
 // loads the same table entries in both; left visible, those loads are merged
 // and the values stay live in registers across everything between the two
 // transforms, which costs the FP32 kernels occupancy (reloading them is an
-// L1/L2 hit).  The FP64 kernels measured faster with the merged loads.
-template <typename T2> __device__ __forceinline__ const T2* opaque_table(const T2* p)
+// L1/L2 hit).  The FP64 kernels measured faster with the merged loads.  An
+// LDS copy of the table is left alone.
+template <typename T2, typename TP> __device__ __forceinline__ TP opaque_table(TP p)
 {
-    if constexpr (sizeof(T2) == 8) asm volatile("" : "+s"(p));
+    if constexpr (sizeof(T2) == 8 && !std::is_same_v<TP, lds_table<T2>>) asm volatile("" : "+s"(p));
     return p;
 }
 
 // Forward: v in layout A -> layout B.  Inverse: layout B -> layout A.
 // lds: this pencil's lds_stride<N>() elements.  WAVE: the pencil's T threads
 // are consecutive lanes of one wave (then only that wave must call);
-// otherwise every thread of the block must call (block barriers).
-template <typename T2, int N, int V, bool WAVE = false>
-__device__ __forceinline__ void fft_forward(T2* v, int t, T2* lds, const T2* __restrict__ tw)
+// otherwise every thread of the block must call (block barriers).  tw: the
+// twiddle table of length N, in global memory (gp) or staged in LDS.
+template <typename T2, int N, int V, bool WAVE = false, typename TP>
+__device__ __forceinline__ void fft_forward(T2* v, int t, T2* lds, TP tw)
 {
     static_assert(!WAVE || (Plan<N, V>::T <= 64 && 64 % Plan<N, V>::T == 0), "pencil must fit in a wave");
-    fft_stages<T2, N, V, -1, 0, WAVE>(v, t, lds, opaque_table(tw));
+    fft_stages<T2, N, V, -1, 0, WAVE>(v, t, lds, opaque_table<T2>(tw));
 }
-template <typename T2, int N, int V, bool WAVE = false>
-__device__ __forceinline__ void fft_inverse(T2* v, int t, T2* lds, const T2* __restrict__ tw)
+template <typename T2, int N, int V, bool WAVE = false, typename TP>
+__device__ __forceinline__ void fft_inverse(T2* v, int t, T2* lds, TP tw)
 {
     static_assert(!WAVE || (Plan<N, V>::T <= 64 && 64 % Plan<N, V>::T == 0), "pencil must fit in a wave");
-    fft_stages<T2, N, V, +1, 0, WAVE>(v, t, lds, opaque_table(tw));
+    fft_stages<T2, N, V, +1, 0, WAVE>(v, t, lds, opaque_table<T2>(tw));
 }
 
 }  // namespace fgpe

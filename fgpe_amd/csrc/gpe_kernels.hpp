@@ -96,9 +96,10 @@ template <int NV> __device__ __forceinline__ void block_sum_store(double (&s)[NV
 // Standalone x-axis transform of rows of length N (the building-block API).
 template <typename T, int N, int V, int DIR>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
-k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ tw)
+k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ twg)
 {
     using T2 = c2_t<T>;
+    const gp<const T2> tw = glob(twg);
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
@@ -227,13 +228,15 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
 
 template <typename T, int N, int V, int DIR>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
-k_strided(const c2_t<T>* __restrict__ tw, StridedArgs a0, const StridedArgs* __restrict__ tab)
+k_strided(const c2_t<T>* __restrict__ twg, StridedArgs a0, const StridedArgs* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const StridedArgs a = args_of(a0, tab, blockIdx.y);
     if (a.stop && *glob(a.stop)) return;
+    __shared__ T2 twl[N];
+    const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const gp<T2> data = glob((T2*)a.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a, p);
@@ -422,13 +425,15 @@ template <typename T, int N, int V> constexpr int last_waves()
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
 __attribute__((amdgpu_waves_per_eu(last_waves<T, N, V>())))
-k_last(const c2_t<T>* __restrict__ tw, LastArgs a0, const LastArgs* __restrict__ tab)
+k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const LastArgs a = args_of(a0, tab, blockIdx.y);
     if (a.s.stop && *glob(a.s.stop)) return;
+    __shared__ T2 twl[N];
+    const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const gp<T2> data = glob((T2*)a.s.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a.s, p);
@@ -562,7 +567,7 @@ __device__ __forceinline__ UserTermIn user_in(const XArgs<T>& a, long long g, in
 template <typename T, int N, int V, int NC, bool DENS, int MODE, bool NT = false>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 __attribute__((amdgpu_waves_per_eu(xstage_waves<T, N, V, NC, MODE>())))
-k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict__ tab)
+k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
@@ -570,6 +575,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ tw, const XArgs<T>* __restrict
     __shared__ T2 lds[P * lds_stride<N>()];
     const XArgs<T> a = args_of(a0, tab, blockIdx.y);
     if (a.stop && *glob(a.stop)) return;
+    __shared__ T2 twl[N];
+    const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const gp<T2> spec = glob(a.spec);
     const gp<double2> P0[2] = {glob(a.P0[0]), glob(a.P0[1])};
     const gp<T2> PS[2] = {glob(a.PS[0]), glob(a.PS[1])};
@@ -863,12 +870,14 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return double2{a
 
 template <typename T, typename YT, int N, int V, bool DENS>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
-k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
+k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ twg)
 {
     using T2 = c2_t<T>;
     using Y2 = c2_t<YT>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
+    __shared__ T2 twl[N];
+    const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
@@ -912,12 +921,14 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ tw)
 
 template <typename T, typename YT, int N, int V, bool DENS>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
-k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ tw)
+k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
 {
     using T2 = c2_t<T>;
     using Y2 = c2_t<YT>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
+    __shared__ T2 twl[N];
+    const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
