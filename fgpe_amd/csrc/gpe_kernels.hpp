@@ -611,7 +611,16 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
             for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
         }
     } else {
+        // phi, the k-space potential of the density, is held from the start
+        // of the stage to the pointwise update, across the inverse transform
+        // of psi.  Its first QL register slots live in LDS instead: in
+        // registers it costs the FP64 kernel 66 VGPRs (202 -> 136 with all of
+        // it in LDS), but all of it in LDS makes the FP64 block too large for
+        // three blocks per CU, so the FP64 form keeps half of it in registers.
+        constexpr int QL = sizeof(T) == 8 ? E / 2 : E;
+        __shared__ T sphi[DENS ? P * TT * QL : 1];
         T phi[E];
+        auto phi_at = [&](int q) -> T& { return q < QL ? sphi[q * (P * TT) + threadIdx.x] : phi[q]; };
         if (DENS) {
             // kx > N/2 from the conjugate symmetry of a real row's transform
 #pragma unroll
@@ -626,7 +635,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
             }
             fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-            for (int q = 0; q < E; ++q) phi[q] = v[q].x;
+            for (int q = 0; q < E; ++q) phi_at(q) = v[q].x;
         }
         double nsave[E];     // NC == 2: density of component 0's stage input
         double sums[NV];
@@ -679,13 +688,13 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                     const double2 ke = to_d2<T>(v[q]);
                     double rX = (vr + g4s * n);
                     if (NC == 2) rX = rX + g4o * no;
-                    if (DENS) rX += (double)phi[q];
+                    if (DENS) rX += (double)phi_at(q);
                     double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
                     double2 he = double2{0.0, 0.0};
                     if (MODE == MODE_APPLY || MODE == MODE_HB) {
                         double eX = (vr + g2s * n);
                         if (NC == 2) eX = eX + g2o * no;
-                        if (DENS) eX += 0.5 * (double)phi[q];
+                        if (DENS) eX += 0.5 * (double)phi_at(q);
                         he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
                     }
                     if constexpr (UserTerm::enabled) {
@@ -781,7 +790,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                     const float2 ke = v[q];
                     float rX = vr + f4s * n;
                     if (NC == 2) rX = rX + f4o * no;
-                    if (DENS) rX += (float)phi[q];
+                    if (DENS) rX += (float)phi_at(q);
                     float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
                     if constexpr (UserTerm::enabled) {
                         const UserTermOut w = UserTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
