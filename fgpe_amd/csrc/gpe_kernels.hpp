@@ -313,19 +313,19 @@ __device__ __forceinline__ double vpx_of(const LastArgs& a, int ch)
 
 // The k-space multiply of the FP64 form.
 template <int N, int V>
-__device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastArgs& a, const TileAt& at)
+__device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastArgs& a, const TileAt& at,
+                                                  lds_table<double2> kA, double2 kxp2, double2 kop2)
 {
     constexpr int E = Plan<N, V>::E;
-    // k and its index-negated partner k' come from one 16-byte load
-    const gp<const double2> kA = glob(a.kA64);
-    const double2 kxp2 = glob(a.kX64)[at.ix], kop2 = glob(a.kO64)[at.outer];
+    // k and its index-negated partner k' come as one 16-byte pair
     const double kx = kxp2.x, ko = kop2.x;
     const double kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
         const double w = vpx_of(a, at.ch) * kx;   // moving frame: - vpx k_x / N
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const double ka = kA[elemB<N, V>(t, q)].x;
+            const double2 kk = kA[elemB<N, V>(t, q)];
+            const double ka = kk.x;
             const double k2 = kxo2 + ka * ka;
             // w = 0 leaves (lap_scale v) k2 unchanged (a zero product is exact)
             v[q] = double2{(a.lap_scale * v[q].x) * k2 - w * v[q].x, (a.lap_scale * v[q].y) * k2 - w * v[q].y};
@@ -370,11 +370,10 @@ __device__ __forceinline__ float vdd_of_f32(float c_dd, float num, float k2)
 }
 
 template <int N, int V>
-__device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastArgs& a, const TileAt& at)
+__device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastArgs& a, const TileAt& at,
+                                                  lds_table<float2> kA, float2 kxp2, float2 kop2)
 {
     constexpr int E = Plan<N, V>::E;
-    const gp<const float2> kA = glob(a.kA32);
-    const float2 kxp2 = glob(a.kX32)[at.ix], kop2 = glob(a.kO32)[at.outer];
     const float kx = kxp2.x, ko = kop2.x;
     const float kxo2 = kx * kx + ko * ko;
     if (at.ch < a.s.ncomp) {
@@ -382,7 +381,8 @@ __device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastAr
         const float w = (float)vpx_of(a, at.ch) * kx;   // moving frame: - vpx k_x / N
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const float ka = kA[elemB<N, V>(t, q)].x;
+            const float2 kk = kA[elemB<N, V>(t, q)];
+            const float ka = kk.x;
             const float f = ls * (kxo2 + ka * ka) - w;
             v[q] = float2{v[q].x * f, v[q].y * f};
         }
@@ -432,13 +432,25 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
     __shared__ T2 lds[P * lds_stride<N>()];
     const LastArgs a = args_of(a0, tab, blockIdx.y);
     if (a.s.stop && *glob(a.s.stop)) return;
+    // the last axis's k pairs, read once per element by the multiply, are
+    // staged in LDS with the twiddle table (one barrier for both)
     __shared__ T2 twl[N];
+    __shared__ T2 kal[N];
+    {
+        const T2* kag = (const T2*)(sizeof(T) == 8 ? (const void*)a.kA64 : (const void*)a.kA32);
+        for (int i = threadIdx.x; i < N; i += blockDim.x) kal[i] = kag[i];
+    }
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
+    const lds_table<T2> kA = (lds_table<T2>)kal;
     const gp<T2> data = glob((T2*)a.s.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a.s, p);
     const gp<T2> base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
+    // this column's k_x and outer-axis k pairs, loaded before the pencil so
+    // that their latency hides behind it rather than after the transform
+    const T2 kxp2 = glob((const T2*)(sizeof(T) == 8 ? (const void*)a.kX64 : (const void*)a.kX32))[at.ix];
+    const T2 kop2 = glob((const T2*)(sizeof(T) == 8 ? (const void*)a.kO64 : (const void*)a.kO32))[at.outer];
     T2 v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = at.valid ? base[(long long)elemA<N, V>(t, q) * at.st] : T2{0, 0};
@@ -451,8 +463,8 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
     }
     fft_forward<T2, N, V>(v, t, my, tw);
 
-    if constexpr (sizeof(T) == 4) last_multiply_f32<N, V>(v, t, a, at);
-    else last_multiply_f64<N, V>(v, t, a, at);
+    if constexpr (sizeof(T) == 4) last_multiply_f32<N, V>(v, t, a, at, kA, kxp2, kop2);
+    else last_multiply_f64<N, V>(v, t, a, at, kA, kxp2, kop2);
     fft_inverse<T2, N, V>(v, t, my, tw);
     if (at.valid)
 #pragma unroll
