@@ -245,7 +245,8 @@ template <typename T2> using lds_table = const T2*;
 template <typename T2, int N>
 __device__ __forceinline__ lds_table<T2> stage_twiddles(T2* twl, const T2* __restrict__ twg)
 {
-    for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = twg[i];
+    const gp<const T2> g = glob(twg);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = g[i];
     __syncthreads();
     return (lds_table<T2>)twl;
 }

@@ -437,7 +437,7 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
     __shared__ T2 twl[N];
     __shared__ T2 kal[N];
     {
-        const T2* kag = (const T2*)(sizeof(T) == 8 ? (const void*)a.kA64 : (const void*)a.kA32);
+        const gp<const T2> kag = glob((const T2*)(sizeof(T) == 8 ? (const void*)a.kA64 : (const void*)a.kA32));
         for (int i = threadIdx.x; i < N; i += blockDim.x) kal[i] = kag[i];
     }
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);

@@ -378,6 +378,33 @@ def test_variant1_plans_parity(torch_cuda, shape):
     assert err < 1e-12, err
 
 
+@pytest.mark.parametrize("shape", [(128, 64, 64), (256, 32, 32), (256, 64, 1)])
+def test_variant1_plans_mixed(shape):
+    """The mixed form (FP32 bulk, FP64 state) with the second plan variant of
+    every kernel class, dipolar and soft-core terms on (the density potential
+    that the FP32 x stage keeps in LDS): RK4 steps within 1e-6 of the FP64
+    oracle."""
+    from fgpe_amd import FGPE_PREC_MIXED, configs
+    nx, ny, nz = shape
+    c = Case(*shape, L=(10.0, 12.0, 20.0 if nz > 1 else 1.0))
+    h = dict(a=(0.9,), a_dd=0.8, dip=(0.0, 0.6, 0.8) if nz > 1 else (0.6, 0.8, 0.0),
+             sc_U=1.5, sc_R=1.1, sc_dim=3 if nz > 1 else 2)
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(V)) + 0.01 * c.rand_psi(57), c.dv)
+    with c.ctx(FGPE_PREC_MIXED) as ctx:
+        for k in range(5):
+            for axis in range(3):
+                ctx.set_variant(k, axis, 1)
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ctx.rk4_steps(5e-4, 3)
+        gpu = ctx.download_psi()
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 3)
+    err = rel_max(gpu, ref)
+    assert err < 1e-6, (shape, err)
+
+
 def test_autotune_keeps_state():
     from fgpe_amd import configs
     c = Case(64, 64, 128, L=(16.0, 16.0, 32.0))
