@@ -73,6 +73,7 @@ struct fgpe_ctx {
 
     double2* P0[2] = {nullptr, nullptr};
     void* PS[2] = {nullptr, nullptr};
+    void* PSb[2] = {nullptr, nullptr};   // mixed form: RK stage-2 state (stage 3 reads stage 1's in PS)
     void* K[2] = {nullptr, nullptr};
     void* Vr[2] = {nullptr, nullptr};
     void* Vi[2] = {nullptr, nullptr};
@@ -121,7 +122,7 @@ struct fgpe_ctx {
     // plus the spectral channels.
     size_t working_set() const
     {
-        const size_t comp = 16 + 2 * (2 * tsz()) + tsz();
+        const size_t comp = 16 + 2 * (2 * tsz()) + tsz() + (mixed() ? 2 * tsz() : 0);
         return (size_t)ntot * (hp.ncomp * comp + (size_t)nch * 2 * tsz());
     }
 };

@@ -271,6 +271,13 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
     if (!c->spec_valid) FGPE_RET(prologue<T>(c));
     const double* scale = imaginary ? c->scal + SC_SCALE : nullptr;
     const long long nb = Launch<T>::xblocks(c);
+    // The mixed form keeps the stage-1 state until stage 3, which rebuilds
+    // psi_n from it instead of reading the FP64 state: stage 2 writes its
+    // state to a second buffer.
+    constexpr bool keep1 = sizeof(T) == 4;
+    if (keep1)
+        for (int i = 0; i < nc; ++i)
+            if (!c->PSb[i]) FGPE_RET(dalloc(c, &c->PSb[i], 2 * c->tsz() * (size_t)c->ntot));
     for (int it = 0; it < nsteps; ++it) {
         for (int s = 1; s <= 4; ++s) {
             // PARS[T] at each stage (src/oneComponentGPSolver.cu:175,184)
@@ -286,6 +293,17 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
             a.stage = s;
             a.scale = scale;
             a.want_norm = (imaginary && s == 4) ? 1 : 0;
+            if (keep1)
+                for (int i = 0; i < nc; ++i) {
+                    c2_t<T>* const ps1 = (c2_t<T>*)c->PS[i];
+                    c2_t<T>* const ps2 = (c2_t<T>*)c->PSb[i];
+                    if (s == 2) a.PSo[i] = ps2;
+                    if (s == 3) {
+                        a.PS[i] = ps2;
+                        a.PS1[i] = ps1;
+                        a.PSo[i] = ps1;
+                    }
+                }
             FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_RK));
             if (imaginary && s == 4)
                 FGPE_RET(finalize(c, nb, FIN_RENORM, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP,
@@ -586,6 +604,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
                     b.cr = 0.0;
                     b.ci = (s <= 2) ? -0.5 * dt : -1.0 * dt;
                     b.stage = s;
+                    if (sizeof(T) == 4 && s == 3) b.PS1[0] = (c2_t<T>*)c->ystage[0];   // stage 1's state
                     FGPE_RET(Launch<T>::xstage(c, b, 1, dens, MODE_RK));
                 }
                 c->spec_valid = false;
@@ -646,7 +665,8 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
 }
 
 // Times both plan variants of each kernel class on the context's own buffers
-// (stage-2 x passes read P0 and write only PS and K, so the state survives)
+// (stage-2 x passes write only PS, K and the spectral channels, never the
+// state, so it survives; the spectrum is recomputed before the next step)
 // and keeps the faster one.
 template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
 {
@@ -1043,7 +1063,7 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     if (!c) return FGPE_OK;
     (void)hipDeviceSynchronize();
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->k64[0], c->k64[1], c->k64[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
-                    c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
+                    c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->PSb[0], c->PSb[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
                     c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->axbuf, c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
                     c->lam[1], c->lam[2], c->pacc};

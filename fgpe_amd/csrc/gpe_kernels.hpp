@@ -501,6 +501,7 @@ template <typename T> struct XArgs {
     double2* P0[2];         // FP64 state psi_n (RK: psi_old of the step)
     c2_t<T>* PS[2];         // RK: stage state; HB: previous psi
     c2_t<T>* PSo[2];        // RK stages 1-3: where the next stage state goes (null: PS)
+    c2_t<T>* PS1[2];        // mixed RK stage 3: the stage-1 state (null: the FP64 state is read)
     c2_t<T>* K[2];          // RK accumulator
     const T* Vr[2];         // Re V_c
     const T* Vi[2];         // Im V_c or null
@@ -593,6 +594,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
     const gp<double2> P0[2] = {glob(a.P0[0]), glob(a.P0[1])};
     const gp<T2> PS[2] = {glob(a.PS[0]), glob(a.PS[1])};
     const gp<T2> PSo[2] = {glob(a.PSo[0]), glob(a.PSo[1])};
+    const gp<const T2> PS1[2] = {glob((const T2*)a.PS1[0]), glob((const T2*)a.PS1[1])};
     const gp<T2> K[2] = {glob(a.K[0]), glob(a.K[1])};
     const gp<const T> Vr[2] = {glob(a.Vr[0]), glob(a.Vr[1])};
     const gp<const T> Vi[2] = {glob(a.Vi[0]), glob(a.Vi[1])};
@@ -809,18 +811,31 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                         h = float2{h.x + (float)(w.wr * ps.x - w.wi * ps.y), h.y + (float)(w.wr * ps.y + w.wi * ps.x)};
                     }
                     if (a.stage < 4) {
-                        float2 po = ps;
-                        if (a.stage != 1) {
-                            const double2 r = ldn<NT>(P0[c], g);
-                            po = float2{(float)(sig * r.x), (float)(sig * r.y)};
+                        float2 po = ps;   // stage 1: the (rounded) state itself
+                        float2 k = float2{0.f, 0.f};
+                        if (a.stage != 1) k = ldn<NT>(K[c], g);
+                        if (a.stage == 2) {
+                            // The state from the stage-1 output and its increment: PS = psi_n + c h1
+                            // and K = h1, and stage 2 has the same c, so the FP64 state is not read
+                            // (it would only be rounded to FP32 here).
+                            po = float2{ps.x - (fcr * k.x - fci * k.y), ps.y - (fcr * k.y + fci * k.x)};
+                        } else if (a.stage == 3) {
+                            if (PS1[c]) {
+                                // With PS1 = psi_n + c2 h1, PS = psi_n + c2 h2, K = h1 + 2 h2 and
+                                // c2 = c3 / 2: psi_n = (PS1 + 2 PS - c2 K) / 3, again without
+                                // reading the FP64 state.
+                                const float2 p1 = ldn<NT>(PS1[c], g);
+                                const float hr = 0.5f * fcr, hi = 0.5f * fci;
+                                po = float2{(p1.x + 2.f * ps.x - (hr * k.x - hi * k.y)) * (1.f / 3.f),
+                                            (p1.y + 2.f * ps.y - (hr * k.y + hi * k.x)) * (1.f / 3.f)};
+                            } else {
+                                const double2 r = ldn<NT>(P0[c], g);
+                                po = float2{(float)(sig * r.x), (float)(sig * r.y)};
+                            }
                         }
                         const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
-                        if (a.stage == 1) {
-                            stn<NT>(K[c], g, h);
-                        } else {
-                            const float2 k = ldn<NT>(K[c], g);
-                            stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
-                        }
+                        if (a.stage == 1) stn<NT>(K[c], g, h);
+                        else stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
                         stn<NT>(PSo[c] ? PSo[c] : PS[c], g, nw);
                         v[q] = nw;
                     } else {

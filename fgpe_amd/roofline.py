@@ -42,7 +42,9 @@ def xstage_bytes(stage: int, mixed: bool, ncomp: int = 1, dens: bool = True, nx:
     if stage == 1:
         pw = s0 + sv + sc + sc
     elif stage in (2, 3):
-        pw = sc + s0 + sv + 2 * sc + sc
+        # the mixed form rebuilds the state instead of reading the FP64 one: stage 2
+        # from its input and K, stage 3 from its input, K and the kept stage-1 state
+        pw = sc + ((0 if stage == 2 else sc) if mixed else s0) + sv + 2 * sc + sc
     else:
         pw = sc + 2 * s0 + sc + sv
     return spec + ncomp * pw
