@@ -370,7 +370,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is "
-              "single-GPU by design, see DESIGN.md for how it would shard", file=sys.stderr)
+              "single-GPU by design (no sharded form is built)", file=sys.stderr)
         return 2
 
     import torch

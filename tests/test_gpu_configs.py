@@ -6,8 +6,7 @@ oracle would take minutes to hours, the checks are the properties the
 equations fix independently of size: imaginary time lowers the energy at
 fixed norm, real-time RK4 never raises the norm and, on smooth states,
 conserves norm and energy to its truncation error, and the mixed-precision
-form stays within the north-star bound (1e-6 relative energy) of the FP64
-form."""
+form stays within 1e-6 relative energy of the FP64 form."""
 import numpy as np
 import pytest
 
