@@ -115,3 +115,28 @@ def test_config3_two_component_softcore_256cubed():
     f, m = out[FGPE_PREC_F64][1], out[FGPE_PREC_MIXED][1]
     for comp in range(2):
         assert abs(m[3 * comp + 2] - f[3 * comp + 2]) < 1e-6 * abs(f[3 * comp + 2])
+
+
+def test_mixed_long_horizon_dipolar_256cubed():
+    """The benchmark workload itself, 256^3 dipolar real-time RK4, over 2000
+    steps (one time unit) in both forms: the mixed form (FP32 bulk, stages 2
+    and 3 rebuilding psi_n from the stage states) stays within 1e-6 of the
+    FP64 form in energy and within 1e-4 in L2 norm of psi, and within 1e-6 of
+    its norm (RK4 damps the seeded high-k noise in both forms alike)."""
+    w = configs.dipolar_real(256, 256, 256, nsteps=2000)
+    print(f"{w.name}, {w.nsteps} steps", flush=True)
+    out = {}
+    for prec in (FGPE_PREC_F64, FGPE_PREC_MIXED):
+        with ctx_for(w, prec) as c:
+            o0 = c.observables(0.0)
+            c.rk4_steps(w.dt, w.nsteps)
+            out[prec] = (o0, c.observables(w.nsteps * w.dt), c.download_psi())
+    f0, f1, pf = out[FGPE_PREC_F64]
+    m0, m1, pm = out[FGPE_PREC_MIXED]
+    de = abs(m1[2] - f1[2]) / abs(f1[2])
+    l2 = float(np.sqrt(np.sum(np.abs(pm - pf) ** 2) / np.sum(np.abs(pf) ** 2)))
+    print(f"mixed vs FP64 after {w.nsteps} steps: rel energy {de:.3e}, L2 psi {l2:.3e}, "
+          f"norm drift FP64 {abs(f1[0] - f0[0]) / f0[0]:.3e} mixed {abs(m1[0] - m0[0]) / m0[0]:.3e}", flush=True)
+    assert de < 1e-6
+    assert l2 < 1e-4
+    assert abs(m1[0] - f1[0]) < 1e-6 * f1[0]
