@@ -273,11 +273,11 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
     const long long nb = Launch<T>::xblocks(c);
     // The mixed form keeps the stage-1 state until stage 3, which rebuilds
     // psi_n from it instead of reading the FP64 state: stage 2 writes its
-    // state to a second buffer.
+    // state to a second buffer (allocated with the other per-component ones).
     constexpr bool keep1 = sizeof(T) == 4;
     if (keep1)
         for (int i = 0; i < nc; ++i)
-            if (!c->PSb[i]) FGPE_RET(dalloc(c, &c->PSb[i], 2 * c->tsz() * (size_t)c->ntot));
+            if (!c->PSb[i]) return fail(FGPE_ESTATE, "mixed-form stage buffer missing");
     for (int it = 0; it < nsteps; ++it) {
         for (int s = 1; s <= 4; ++s) {
             // PARS[T] at each stage (src/oneComponentGPSolver.cu:175,184)
@@ -1035,6 +1035,7 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     }
     if (r == FGPE_OK) r = dalloc(c, (void**)&c->P0[0], sizeof(double2) * N);
     if (r == FGPE_OK) r = dalloc(c, &c->PS[0], 2 * c->tsz() * N);
+    if (r == FGPE_OK && c->mixed()) r = dalloc(c, &c->PSb[0], 2 * c->tsz() * N);
     if (r == FGPE_OK) r = dalloc(c, &c->K[0], 2 * c->tsz() * N);
     if (r == FGPE_OK) r = dalloc(c, (void**)&c->scal, sizeof(double) * SC_COUNT);
     if (r == FGPE_OK) r = dalloc(c, (void**)&c->flags, sizeof(int) * FL_COUNT);
@@ -1138,6 +1139,7 @@ int fgpe_set_hamiltonian(fgpe_ctx* c, const fgpe_hparams* hp)
     if (hp->ncomp == 2) {
         if (!c->P0[1]) FGPE_RET(dalloc(c, (void**)&c->P0[1], sizeof(double2) * N));
         if (!c->PS[1]) FGPE_RET(dalloc(c, &c->PS[1], 2 * c->tsz() * N));
+        if (c->mixed() && !c->PSb[1]) FGPE_RET(dalloc(c, &c->PSb[1], 2 * c->tsz() * N));
         if (!c->K[1]) FGPE_RET(dalloc(c, &c->K[1], 2 * c->tsz() * N));
     }
     if (hp->sc_U != 0.0) {
