@@ -301,6 +301,19 @@ def test_two_component_rk4_f64(name):
     ref = orc.rk4(c.grid, two_comp_ham(c, h), psi, V, dt, n)
     err = rel_max(gpu, ref)
     assert err < 1e-12, err
+    if name == "softcore":
+        # the mixed form on the same case: both components' stage states
+        # rebuilt at stages 2 and 3, the coupling read from the other
+        # component's stage state
+        from fgpe_amd import FGPE_PREC_MIXED
+        with c.ctx(FGPE_PREC_MIXED) as ctx:
+            ctx.set_hamiltonian(**h)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            ctx.rk4_steps(dt, n)
+            gm = ctx.download_psi()
+        errm = rel_max(gm, ref)
+        assert errm < 1e-6, errm
     hmu, hen = orc.hpsi(c.grid, two_comp_ham(c, h), n * dt, ref, V)
     for k in range(2):
         nrm = orc.norm(c.grid, ref[k])
