@@ -442,13 +442,17 @@ def test_variant1_plans_mixed(shape):
     assert err < 1e-6, (shape, err)
 
 
-def test_autotune_keeps_state():
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_keeps_state(precision):
+    """The autotuner's timing launches (stage-2 x stages on whatever the stage
+    buffers hold, in the mixed form rebuilding the state from them) leave the
+    state alone, and the steps after it match the oracle."""
     from fgpe_amd import configs
     c = Case(64, 64, 128, L=(16.0, 16.0, 32.0))
     h = dict(a=(1.0,), a_dd=1.3, dip=(0.0, 0.0, 1.0))
     V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25))
     psi = configs.normalise(np.exp(-0.5 * np.abs(V)) + 0.01 * c.rand_psi(53), c.dv)
-    with c.ctx() as ctx:
+    with c.ctx(precision) as ctx:
         ctx.set_hamiltonian(**h)
         ctx.upload_psi(psi)
         ctx.upload_vext(V)
@@ -459,7 +463,7 @@ def test_autotune_keeps_state():
         gpu = ctx.download_psi()
     ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 4)
     err = rel_max(gpu, ref)
-    assert err < 1e-12, err
+    assert err < (1e-12 if precision == 0 else 1e-6), err
 
 
 @pytest.mark.parametrize("precision", [0, 1])
