@@ -363,6 +363,16 @@ __device__ __forceinline__ void last_multiply_f64(double2* v, int t, const LastA
 // on the FP32 values.
 // |k| < 1e-6 tested as k^2 < 1e-12; the quotient through the hardware
 // reciprocal (1 ulp), well inside the FP32 bulk's own rounding.
+// s / 3 in FP32 without a bias: q = s (1/3), corrected by the exact residual
+// s - 3 q; error about half an ulp, against the 3e-8 relative bias of the
+// product with the rounded 1/3 alone.
+__device__ __forceinline__ float div3(float s)
+{
+    constexpr float third = 1.f / 3.f;
+    const float q = s * third;
+    return __builtin_fmaf(__builtin_fmaf(-3.f, q, s), third, q);
+}
+
 __device__ __forceinline__ float vdd_of_f32(float c_dd, float num, float k2)
 {
     if (k2 < 1E-12f) return 0.f;
@@ -823,14 +833,17 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                             if (PS1[c]) {
                                 // With PS1 = psi_n + c2 h1, PS = psi_n + c2 h2, K = h1 + 2 h2 and
                                 // c2 = c3 / 2: psi_n = (PS1 + 2 PS - c2 K) / 3, again without
-                                // reading the FP64 state.  A correctly rounded division: the
-                                // product with the FP32 value of 1/3 (3e-8 high) would scale
-                                // every stage-3 state by that bias, which the steps accumulate
-                                // (2e-6 in energy after 2000 steps at 256^3, against 3e-8).
+                                // reading the FP64 state.  The division by 3 must not be biased:
+                                // the product with the FP32 value of 1/3 alone (3e-8 high) would
+                                // scale every stage-3 state by that bias, which the steps
+                                // accumulate (2e-6 in energy after 2000 steps at 256^3, against
+                                // 3e-8), so the product is corrected by its exact residual
+                                // r = s - 3 q (one FMA), which leaves the quotient within about
+                                // half an ulp of s / 3.
                                 const float2 p1 = ldn<NT>(PS1[c], g);
                                 const float hr = 0.5f * fcr, hi = 0.5f * fci;
-                                po = float2{(p1.x + 2.f * ps.x - (hr * k.x - hi * k.y)) / 3.f,
-                                            (p1.y + 2.f * ps.y - (hr * k.y + hi * k.x)) / 3.f};
+                                po = float2{div3(p1.x + 2.f * ps.x - (hr * k.x - hi * k.y)),
+                                            div3(p1.y + 2.f * ps.y - (hr * k.y + hi * k.x))};
                             } else {
                                 const double2 r = ldn<NT>(P0[c], g);
                                 po = float2{(float)(sig * r.x), (float)(sig * r.y)};
