@@ -88,6 +88,10 @@ struct fgpe_ctx {
     size_t hist_n = 0;
     bool spec_valid = false;
     bool hb_active = false;   // a heavy-ball run can be continued
+    // imaginary-time iterations run since the state or the Hamiltonian was
+    // last set (or a heavy-ball run started afresh): the time of the next
+    // iteration is imag_iter * dt, as the reference's PARS[T] = iteration * dt
+    long long imag_iter = 0;
     double2* aux = nullptr;   // FP64 scratch of the diagnostics
     // adjoint: checkpoints, segment states, stage states, VJP buffers
     std::vector<double2*> ckpt;

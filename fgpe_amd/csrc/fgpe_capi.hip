@@ -44,7 +44,11 @@ __global__ void __launch_bounds__(256) k_finalize(FinArgs f0, const FinArgs* __r
             out[3 * c + 1] = sh[3 * c + 1][0] * f.dv / norm;
             out[3 * c + 2] = sh[3 * c + 2][0] * f.dv / norm;
         } else if (f.kind == FIN_RENORM) {
+            // the RK4 scheme evaluates no energy operator: only the norm is
+            // recorded, the mu and E slots say so
             out[3 * c] = norm;
+            out[3 * c + 1] = __builtin_nan("");
+            out[3 * c + 2] = __builtin_nan("");
             scale[c] = sqrt(f.npart[c] / norm);
         } else {
             const double sc = sqrt(f.npart[c] / norm);
@@ -338,6 +342,7 @@ template <typename T> int hb_prepare(fgpe_ctx* c)
     } else {
         hipLaunchKernelGGL(k_hb_init, dim3(1), dim3(1), 0, c->stream, c->scal, c->flags);
         FGPE_LAUNCHCHK("k_hb_init");
+        c->imag_iter = 0;
         for (int i = 0; i < nc; ++i) {
             if (sizeof(T) == 8) {
                 FGPE_HIPCHK(hipMemcpyAsync(c->PS[i], c->P0[i], sizeof(double2) * c->ntot, hipMemcpyDeviceToDevice,
@@ -400,7 +405,8 @@ template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const doub
     const int* stop = c->flags + FL_STOP;
     for (int it = 0; it < nsteps; ++it) {
         FGPE_RET(spectral_passes<T>(c, c->scal + SC_SCALE, stop, true));
-        FGPE_RET(Launch<T>::xstage(c, hb_xargs<T>(c, it * dt, dt, beta), nc, dens, MODE_HB));
+        // PARS[T] = iteration * dt over the whole run (src/oneComponentGPSolver.cu:50)
+        FGPE_RET(Launch<T>::xstage(c, hb_xargs<T>(c, (double)(c->imag_iter + it) * dt, dt, beta), nc, dens, MODE_HB));
         FGPE_RET(finalize(c, nb, FIN_HB, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP, eps));
     }
     return FGPE_OK;
@@ -787,6 +793,25 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
     return rc;
 }
 
+// The soft-core Vt(k) table from the context's k axes and Hamiltonian (built
+// by fgpe_set_hamiltonian, rebuilt by fgpe_set_kspace).
+int build_softcore(fgpe_ctx* c)
+{
+    const size_t N = (size_t)c->ntot;
+    const fgpe_hparams* hp = &c->hp;
+    if (!c->wtab) FGPE_RET(dalloc(c, &c->wtab, c->tsz() * N));
+    const int th = 256;
+    dim3 grid((unsigned)((N + th - 1) / th));
+    if (c->mixed())
+        hipLaunchKernelGGL(k_softcore_table<float>, grid, dim3(th), 0, c->stream, (float*)c->wtab, c->k[0], c->k[1],
+                           c->k[2], c->n[0], c->n[1], c->ntot, hp->sc_dim, hp->sc_R, hp->sc_U);
+    else
+        hipLaunchKernelGGL(k_softcore_table<double>, grid, dim3(th), 0, c->stream, (double*)c->wtab, c->k[0],
+                           c->k[1], c->k[2], c->n[0], c->n[1], c->ntot, hp->sc_dim, hp->sc_R, hp->sc_U);
+    FGPE_LAUNCHCHK("k_softcore_table");
+    return FGPE_OK;
+}
+
 int ensure_hist(fgpe_ctx* c, size_t n)
 {
     if (n > c->hist_n) {
@@ -943,6 +968,13 @@ template <typename T> int group_autotune(fgpe_group* g, int reps, double* out)
 {
     FGPE_RET(group_check(g));
     fgpe_ctx* c0 = g->c[0];
+    // The timing x stages overwrite PS, which holds a heavy-ball run's old
+    // state, so a run in progress ends here: its pending renormalisation is
+    // applied, and the next run starts afresh from the normalised state.
+    for (fgpe_ctx* c : g->c) {
+        FGPE_RET(apply_pending_scale(c));
+        c->hb_active = false;
+    }
     const double one[2] = {1.0, 1.0};
     const GroupTabs t = group_tabs<T>(g, 0.0, 0.0, one, 0.0, true);
     FGPE_RET(group_upload(c0, &g->tab, &g->cap, t));
@@ -1065,7 +1097,8 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
 int fgpe_ctx_destroy(fgpe_ctx* c)
 {
     if (!c) return FGPE_OK;
-    (void)hipDeviceSynchronize();
+    // work queued by this context may still read its buffers
+    (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->k64[0], c->k64[1], c->k64[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->PSb[0], c->PSb[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
@@ -1105,6 +1138,8 @@ int fgpe_ctx_device_bytes(fgpe_ctx* c, size_t* bytes)
 int fgpe_set_kspace(fgpe_ctx* c, const double* kx, const double* ky, const double* kz, double dv)
 {
     if (!c || !kx || !ky || !kz) return fail(FGPE_EINVAL, "NULL argument");
+    // launches still queued on the context's stream read the k tables
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     const double* src[3] = {kx, ky, kz};
     for (int a = 0; a < 3; ++a) {
         FGPE_HIPCHK(hipMemcpy(c->k[a], src[a], sizeof(double) * c->n[a], hipMemcpyHostToDevice));
@@ -1121,6 +1156,7 @@ int fgpe_set_kspace(fgpe_ctx* c, const double* kx, const double* ky, const doubl
     c->dv = dv;
     c->kset = true;
     c->spec_valid = false;
+    if (c->hset && c->hp.sc_U != 0.0) FGPE_RET(build_softcore(c));
     return FGPE_OK;
 }
 
@@ -1146,22 +1182,14 @@ int fgpe_set_hamiltonian(fgpe_ctx* c, const fgpe_hparams* hp)
         if (!c->K[1]) FGPE_RET(dalloc(c, &c->K[1], 2 * c->tsz() * N));
     }
     if (hp->sc_U != 0.0) {
-        if (!c->wtab) FGPE_RET(dalloc(c, &c->wtab, c->tsz() * N));
-        const int th = 256;
-        dim3 grid((unsigned)((N + th - 1) / th));
-        if (c->mixed())
-            hipLaunchKernelGGL(k_softcore_table<float>, grid, dim3(th), 0, c->stream, (float*)c->wtab, c->k[0],
-                               c->k[1], c->k[2], c->n[0], c->n[1], c->ntot, hp->sc_dim, hp->sc_R, hp->sc_U);
-        else
-            hipLaunchKernelGGL(k_softcore_table<double>, grid, dim3(th), 0, c->stream, (double*)c->wtab, c->k[0],
-                               c->k[1], c->k[2], c->n[0], c->n[1], c->ntot, hp->sc_dim, hp->sc_R, hp->sc_U);
-        FGPE_LAUNCHCHK("k_softcore_table");
+        FGPE_RET(build_softcore(c));
     } else {
         dfree(c, &c->wtab, c->tsz() * N);
     }
     c->hset = true;
     c->spec_valid = false;
     c->hb_active = false;
+    c->imag_iter = 0;
     return FGPE_OK;
 }
 
@@ -1176,6 +1204,7 @@ int fgpe_upload_psi(fgpe_ctx* c, int comp, const double* host)
     FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     c->spec_valid = false;
     c->hb_active = false;
+    c->imag_iter = 0;
     return FGPE_OK;
 }
 
@@ -1193,6 +1222,8 @@ int fgpe_upload_vext(fgpe_ctx* c, int comp, const double* host)
     if (!c || !host) return fail(FGPE_EINVAL, "NULL argument");
     if (comp < 0 || comp > 1) return fail(FGPE_EINVAL, "comp must be 0 or 1");
     const size_t N = (size_t)c->ntot;
+    // launches still queued on the context's stream read the trap
+    FGPE_HIPCHK(hipStreamSynchronize(c->stream));
     bool cplx = false;
     for (size_t i = 0; i < N && !cplx; ++i) cplx = host[2 * i + 1] != 0.0;
     if (!c->Vr[comp]) FGPE_RET(dalloc(c, &c->Vr[comp], c->tsz() * N));
@@ -1274,6 +1305,7 @@ int fgpe_rk4_steps(fgpe_ctx* c, double dt, double t0, int nsteps)
 {
     if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
     c->hb_active = false;
+    c->imag_iter = 0;
     if (nsteps < 0) return fail(FGPE_EINVAL, "nsteps < 0");
     const double one[2] = {1.0, 1.0};
     return c->mixed() ? rk4_run<float>(c, dt, t0, nsteps, false, one, nullptr)
@@ -1296,8 +1328,9 @@ int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, const doubl
     }
     if (scheme == FGPE_IMAG_RK4_RENORM) {
         c->hb_active = false;
-        FGPE_RET(c->mixed() ? rk4_run<float>(c, dt, 0.0, nsteps, true, npart, hd)
-                            : rk4_run<double>(c, dt, 0.0, nsteps, true, npart, hd));
+        const double t0 = (double)c->imag_iter * dt;
+        FGPE_RET(c->mixed() ? rk4_run<float>(c, dt, t0, nsteps, true, npart, hd)
+                            : rk4_run<double>(c, dt, t0, nsteps, true, npart, hd));
     } else {
         FGPE_RET(c->mixed() ? hb_run<float>(c, dt, beta, npart, eps, nsteps, hd)
                             : hb_run<double>(c, dt, beta, npart, eps, nsteps, hd));
@@ -1307,6 +1340,7 @@ int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, const doubl
     if (scheme == FGPE_IMAG_HEAVY_BALL)
         FGPE_HIPCHK(hipMemcpyAsync(&done, c->flags + FL_DONE, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     FGPE_HIPCHK(hipStreamSynchronize(c->stream));
+    c->imag_iter += done;
     if (hd && done > 0)
         FGPE_HIPCHK(hipMemcpy(host_hist, hd, sizeof(double) * 3 * nc * done, hipMemcpyDeviceToHost));
     if (steps_done) *steps_done = done;

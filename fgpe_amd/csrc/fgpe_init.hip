@@ -372,6 +372,7 @@ int state_changed(fgpe_ctx* c, int comp)
     FGPE_LAUNCHCHK("k_set_one");
     c->spec_valid = false;
     c->hb_active = false;
+    c->imag_iter = 0;
     return FGPE_OK;
 }
 

@@ -218,9 +218,15 @@ int fgpe_rk4_steps(fgpe_ctx* ctx, double dt, double t0, int nsteps);
    norm of each component (ncomp doubles).  eps > 0 stops heavy-ball at |E - E_prev| / E_prev < eps
    (src/oneComponentGPSolver.cu:84-88; two components: |dE_c| < eps for
    both, src/twoComponentGPSolver.cu:94).  host_hist (3*ncomp*nsteps doubles
-   or NULL) gets norm, mu, E per component per step; *steps_done the number
-   run.  A heavy-ball call continues the previous one (momentum, mu, E_prev,
-   convergence) unless the state or the Hamiltonian was set in between. */
+   or NULL) gets norm, mu, E per component per step (heavy ball); the RK4
+   scheme evaluates no energy operator and fills only the norm, with NaN in
+   the mu and E slots; *steps_done the number run.  A heavy-ball call
+   continues the previous one (momentum, mu, E_prev, convergence) unless the
+   state or the Hamiltonian was set in between.  Time continues across calls
+   as well: iteration j of a call runs at t = (m + j) dt, m the iterations
+   run since the state or Hamiltonian was set or a heavy-ball run started
+   afresh (PARS[T] = iteration * dt, src/oneComponentGPSolver.cu:50), so a
+   run split into several calls matches one call. */
 int fgpe_imag_steps(fgpe_ctx* ctx, int scheme, double dt, double beta, const double* host_npart,
                     double eps, int nsteps, double* host_hist, int* steps_done);
 /* *converged = 1 once a heavy-ball run has met its convergence test. */
@@ -296,7 +302,9 @@ int fgpe_group_create(fgpe_ctx* const* ctxs, int n, fgpe_group** out);
 int fgpe_group_destroy(fgpe_group* group);
 /* Times both plan variants of each kernel class as batched launches and sets
    the faster on every context (host_ms: 9 doubles or NULL, as fgpe_autotune;
-   groups run unslabbed, the last 3 stay 0). */
+   groups run unslabbed, the last 3 stay 0).  A heavy-ball run in progress
+   ends here (its pending renormalisation is applied); the next
+   fgpe_group_imag_steps starts a fresh run from the normalised states. */
 int fgpe_group_autotune(fgpe_group* group, int reps, double* host_ms);
 /* nsteps heavy-ball iterations of every instance (same scheme and stopping
    test as fgpe_imag_steps, per instance), enqueued without a host

@@ -12,7 +12,7 @@ import torch  # before the library: torch's HIP runtime must be the process's fi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 
-from fgpe_amd import FGPE_IMAG_HEAVY_BALL, FGPE_PREC_MIXED, FgpeError, configs  # noqa: E402
+from fgpe_amd import FGPE_IMAG_HEAVY_BALL, FGPE_IMAG_RK4_RENORM, FGPE_PREC_MIXED, FgpeError, configs  # noqa: E402
 from fgpe_amd._lib import user_term_enabled  # noqa: E402
 from oracle import oracle as orc  # noqa: E402
 from tests.cases import Case, rel_max  # noqa: E402
@@ -83,6 +83,43 @@ def main():
     check(done == nref == 60, "heavy ball ran 60 iterations")
     check(rel_max(gpu, ref) < 1e-11, "heavy-ball state with the user term")
     check(np.abs(hist - href).max() <= 1e-11 * np.abs(href).max(), "heavy-ball norm, mu, E history")
+
+    say("heavy ball with a time-dependent user term, one call and three calls")
+    ht = dict(a=(0.05,), a_dd=0.03, dip=(0.0, 0.0, 1.0), user=(0.6, 40.0, 0.0, 0.02))
+    ref, nref, href = orc.imag_heavy_ball(c.grid, orc.Ham(1, ht["a"], vtilde=c.vtilde(0.03), user=ht["user"]),
+                                          psi, V, 2e-3, 0.9, 1.0, 1e-30, 60)
+    runs = {}
+    for split in (1, 3):
+        with c.ctx() as ctx:
+            ctx.set_hamiltonian(**ht)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            hs = []
+            for _ in range(split):
+                d, hh_ = ctx.imag_steps(2e-3, 60 // split, scheme=FGPE_IMAG_HEAVY_BALL, beta=0.9, npart=1.0,
+                                        eps=1e-30, history=True)
+                check(d == 60 // split, f"call of {60 // split} iterations ran them all")
+                hs.append(hh_)
+            runs[split] = (ctx.download_psi(), np.concatenate(hs))
+    for split, (st, hi) in runs.items():
+        check(rel_max(st, ref) < 1e-11, f"{split} call(s): state at t = iteration dt against the oracle")
+        check(np.abs(hi - href).max() <= 1e-11 * np.abs(href).max(), f"{split} call(s): history against the oracle")
+    check(rel_max(runs[3][0], runs[1][0]) < 1e-12, "three calls of 20 equal one call of 60")
+
+    say("RK4 imaginary time with a time-dependent user term, one call and two calls")
+    hr = dict(a=(0.05,), user=(0.6, 40.0, 0.0, 0.0))
+    ref = orc.rk4(c.grid, orc.Ham(1, hr["a"], user=hr["user"]), psi, V, 1e-3, 30, imaginary=True, npart=1.0)
+    for split in (1, 2):
+        with c.ctx() as ctx:
+            ctx.set_hamiltonian(**hr)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            for _ in range(split):
+                d, hh_ = ctx.imag_steps(1e-3, 30 // split, scheme=FGPE_IMAG_RK4_RENORM, npart=1.0, history=True)
+            err = rel_max(ctx.download_psi(), ref)
+        check(err < 1e-12, f"{split} call(s): RK4 imaginary time rel err {err:.2e}")
+        check(np.all(np.isfinite(hh_[:, 0]) & (hh_[:, 0] > 0)) and np.all(np.isnan(hh_[:, 1:])),
+              "RK4 history: norm before renormalisation, NaN in the mu and E slots")
 
     say("two components rk4")
     f = 1.0 / (4 * np.pi)
