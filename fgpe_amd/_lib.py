@@ -34,7 +34,7 @@ EXPORTS = (
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
     "fgpe_psi_init", "fgpe_psi_add_noise", "fgpe_psi_imprint_vortex_tube", "fgpe_psi_force_lattice",
     "fgpe_psi_normalize", "fgpe_vext_init", "fgpe_vext_init_box", "fgpe_vext_add_cos", "fgpe_vext_commit",
-    "fgpe_user_term",
+    "fgpe_user_term", "fgpe_imag_steps_noalloc",
 )
 # initialiser kinds and lattices (fgpe_psi_init, fgpe_vext_init, fgpe_psi_force_lattice)
 INIT_HARMONIC, INIT_TUBE, INIT_VOID = 0, 1, 2
@@ -102,6 +102,8 @@ def load() -> C.CDLL:
     L.fgpe_rk4_steps.argtypes = [_vp, C.c_double, C.c_double, C.c_int]
     L.fgpe_imag_steps.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _dp, C.c_double,
                                   C.c_int, _dp, C.POINTER(C.c_int)]
+    L.fgpe_imag_steps_noalloc.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _dp, C.c_double,
+                                          C.c_int, _dp, C.POINTER(C.c_int)]
     L.fgpe_adjoint_forward.argtypes = [_vp, C.c_double, C.c_double, C.c_int, C.c_int]
     L.fgpe_adjoint_backward.argtypes = [_vp, _vp, _vp, _vp, _dp]
     L.fgpe_calc_px.argtypes = [_vp, C.c_int, _dp]

@@ -57,6 +57,7 @@ enum { FL_STOP = 0, FL_DONE = 1, FL_COUNT = 4 };
 struct fgpe_ctx {
     int n[3] = {1, 1, 1};
     long long ntot = 0;
+    int dev = 0;   // HIP device the context was created on
     int prec = FGPE_PREC_F64;
     hipStream_t own = nullptr, stream = nullptr;
     size_t bytes = 0;

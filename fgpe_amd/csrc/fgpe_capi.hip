@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,38 @@ void dfree(fgpe_ctx* c, void** p, size_t bytes)
 }  // namespace fgpe
 
 namespace {
+
+// ---- the fixed scratch of fgpe_imag_steps_noalloc --------------------------
+// The per-iteration history (norm, mu, E per component) of at most
+// FGPE_NOALLOC_MAX_STEPS iterations.  Each device holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host-side
+// stand-in, not a device address).
+__device__ __align__(256) double scratch[3 * 2 * FGPE_NOALLOC_MAX_STEPS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Looks up this device's scratch address once (with the device current).
+int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(FGPE_EUNSUP, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev]) return FGPE_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(FGPE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = (double*)p;
+    return FGPE_OK;
+}
+
+double* scratch_of(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (dev >= 0 && dev < kMaxDevices) ? g_scratch_dev[dev] : nullptr;
+}
 
 bool supported_n(int n)
 {
@@ -1040,7 +1073,15 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     if (!supported_n(nx) || (ny > 1 && !supported_n(ny)) || (nz > 1 && !supported_n(nz)))
         return fail(FGPE_EUNSUP, "grid extents must be 1 or one of 32, 64, 96, 128, 192, 256, 512");
     if (ny == 1 && nz > 1) return fail(FGPE_EUNSUP, "a grid with ny = 1 must have nz = 1");
+    int dev = 0;
+    {
+        const hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return fail(FGPE_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    }
+    // once per device: the address of the no-allocation entry point's scratch
+    FGPE_RET(scratch_setup(dev));
     fgpe_ctx* c = new fgpe_ctx();
+    c->dev = dev;
     c->n[0] = nx;
     c->n[1] = ny;
     c->n[2] = nz;
@@ -1312,20 +1353,26 @@ int fgpe_rk4_steps(fgpe_ctx* c, double dt, double t0, int nsteps)
                       : rk4_run<double>(c, dt, t0, nsteps, false, one, nullptr);
 }
 
-int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, const double* npart, double eps,
-                    int nsteps, double* host_hist, int* steps_done)
+}  // extern "C"
+
+namespace {
+
+int imag_check(fgpe_ctx* c, int scheme, const double* npart, int nsteps)
 {
     if (!c || !npart) return fail(FGPE_EINVAL, "NULL argument");
     if (nsteps < 0) return fail(FGPE_EINVAL, "nsteps < 0");
     if (scheme != FGPE_IMAG_RK4_RENORM && scheme != FGPE_IMAG_HEAVY_BALL)
         return fail(FGPE_EINVAL, "unknown imaginary-time scheme");
     if (!c->hset) return fail(FGPE_ESTATE, "fgpe_set_hamiltonian has not been called");
+    return FGPE_OK;
+}
+
+// The imaginary-time steps with the history written to the device buffer hd
+// (3 * ncomp * nsteps doubles, or null), then copied to host_hist.
+int imag_steps_on(fgpe_ctx* c, int scheme, double dt, double beta, const double* npart, double eps, int nsteps,
+                  double* hd, double* host_hist, int* steps_done)
+{
     const int nc = c->hp.ncomp;
-    double* hd = nullptr;
-    if (host_hist && nsteps > 0) {
-        FGPE_RET(ensure_hist(c, (size_t)3 * nc * nsteps));
-        hd = c->hist;
-    }
     if (scheme == FGPE_IMAG_RK4_RENORM) {
         c->hb_active = false;
         const double t0 = (double)c->imag_iter * dt;
@@ -1345,6 +1392,38 @@ int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, const doubl
         FGPE_HIPCHK(hipMemcpy(host_hist, hd, sizeof(double) * 3 * nc * done, hipMemcpyDeviceToHost));
     if (steps_done) *steps_done = done;
     return FGPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgpe_imag_steps(fgpe_ctx* c, int scheme, double dt, double beta, const double* npart, double eps,
+                    int nsteps, double* host_hist, int* steps_done)
+{
+    FGPE_RET(imag_check(c, scheme, npart, nsteps));
+    double* hd = nullptr;
+    if (host_hist && nsteps > 0) {
+        FGPE_RET(ensure_hist(c, (size_t)3 * c->hp.ncomp * nsteps));
+        hd = c->hist;
+    }
+    return imag_steps_on(c, scheme, dt, beta, npart, eps, nsteps, hd, host_hist, steps_done);
+}
+
+int fgpe_imag_steps_noalloc(fgpe_ctx* c, int scheme, double dt, double beta, const double* npart, double eps,
+                            int nsteps, double* host_hist, int* steps_done)
+{
+    FGPE_RET(imag_check(c, scheme, npart, nsteps));
+    double* hd = nullptr;
+    if (host_hist && nsteps > 0) {
+        if (nsteps > FGPE_NOALLOC_MAX_STEPS)
+            return fail(FGPE_EINVAL, "fgpe_imag_steps_noalloc: nsteps " + std::to_string(nsteps) +
+                                         " with a history exceeds FGPE_NOALLOC_MAX_STEPS (" +
+                                         std::to_string(FGPE_NOALLOC_MAX_STEPS) + ")");
+        hd = scratch_of(c->dev);
+        if (!hd) return fail(FGPE_ESTATE, "no scratch address for the context's device");
+    }
+    return imag_steps_on(c, scheme, dt, beta, npart, eps, nsteps, hd, host_hist, steps_done);
 }
 
 int fgpe_calc_px(fgpe_ctx* c, int comp, double* out)

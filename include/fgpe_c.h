@@ -229,6 +229,20 @@ int fgpe_rk4_steps(fgpe_ctx* ctx, double dt, double t0, int nsteps);
    run split into several calls matches one call. */
 int fgpe_imag_steps(fgpe_ctx* ctx, int scheme, double dt, double beta, const double* host_npart,
                     double eps, int nsteps, double* host_hist, int* steps_done);
+/* fgpe_imag_steps for callers that cannot allocate device memory (code under
+   HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+   counterpart in the reference.  Same arguments, results and errors, but the
+   per-iteration history goes to a fixed device array of the library instead
+   of a buffer allocated on first use: the call makes no hipMalloc / hipFree.
+   The array holds the history of at most FGPE_NOALLOC_MAX_STEPS iterations
+   (3 * 2 * FGPE_NOALLOC_MAX_STEPS doubles, 768 KiB on every device the
+   library runs on, kept until the process ends); a call asking for a
+   history of more iterations returns FGPE_EINVAL (without a history any
+   nsteps is accepted).  Every context on one device shares that array: calls
+   that overlap on one device (other host threads, other streams) race. */
+#define FGPE_NOALLOC_MAX_STEPS 16384
+int fgpe_imag_steps_noalloc(fgpe_ctx* ctx, int scheme, double dt, double beta, const double* host_npart,
+                            double eps, int nsteps, double* host_hist, int* steps_done);
 /* *converged = 1 once a heavy-ball run has met its convergence test. */
 int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
 
