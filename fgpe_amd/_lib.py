@@ -34,7 +34,7 @@ EXPORTS = (
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
     "fgpe_psi_init", "fgpe_psi_add_noise", "fgpe_psi_imprint_vortex_tube", "fgpe_psi_force_lattice",
     "fgpe_psi_normalize", "fgpe_vext_init", "fgpe_vext_init_box", "fgpe_vext_add_cos", "fgpe_vext_commit",
-    "fgpe_user_term", "fgpe_imag_steps_noalloc",
+    "fgpe_user_term", "fgpe_user_term_ids", "fgpe_imag_steps_noalloc",
 )
 # initialiser kinds and lattices (fgpe_psi_init, fgpe_vext_init, fgpe_psi_force_lattice)
 INIT_HARMONIC, INIT_TUBE, INIT_VOID = 0, 1, 2
@@ -46,7 +46,7 @@ class HParams(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
                 ("quench", C.c_double * 6), ("a_dd", C.c_double), ("dip", C.c_double * 3),
                 ("sc_U", C.c_double), ("sc_R", C.c_double), ("sc_dim", C.c_int), ("vpx", C.c_double * 2),
-                ("user", C.c_double * 8)]
+                ("user", C.c_double * 8), ("user_id", C.c_int)]
 
 
 class Geom(C.Structure):
@@ -67,6 +67,14 @@ def user_term_enabled() -> bool:
     v = C.c_int()
     check(load().fgpe_user_term(C.byref(v)))
     return bool(v.value)
+
+
+def user_term_ids() -> list:
+    """The ids of the user pointwise terms the loaded library compiles."""
+    ids = (C.c_int * 16)()
+    n = C.c_int()
+    check(load().fgpe_user_term_ids(ids, 16, C.byref(n)))
+    return list(ids[:min(n.value, 16)])
 
 
 _lib = None
@@ -140,6 +148,7 @@ def load() -> C.CDLL:
     L.fgpe_vext_add_cos.argtypes = [_vp, C.c_int, _gp, C.c_double, C.c_double]
     L.fgpe_vext_commit.argtypes = [_vp, C.c_int, _dp]
     L.fgpe_user_term.argtypes = [C.POINTER(C.c_int)]
+    L.fgpe_user_term_ids.argtypes = [C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -200,14 +209,14 @@ class Context:
         check(load().fgpe_set_kspace(self.h, _dptr(kx), _dptr(ky), _dptr(kz), float(dv)))
 
     def set_hamiltonian(self, ncomp=1, a=(0.0,), quench=None, a_dd=0.0, dip=(0.0, 0.0, 1.0),
-                        sc_U=0.0, sc_R=0.0, sc_dim=3, vpx=(0.0, 0.0), user=()):
+                        sc_U=0.0, sc_R=0.0, sc_dim=3, vpx=(0.0, 0.0), user=(), user_id=0):
         aa = tuple(a) + (0.0,) * (3 - len(a))
         q = (0.0,) * 6 if quench is None else tuple(quench)
         vv = tuple(vpx) + (0.0,) * (2 - len(vpx))
         uu = tuple(map(float, user)) + (0.0,) * (8 - len(user))
         hp = HParams(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
                      (C.c_double * 6)(*q), a_dd, (C.c_double * 3)(*dip), sc_U, sc_R, sc_dim,
-                     (C.c_double * 2)(*vv), (C.c_double * 8)(*uu))
+                     (C.c_double * 2)(*vv), (C.c_double * 8)(*uu), int(user_id))
         check(load().fgpe_set_hamiltonian(self.h, C.byref(hp)))
         self.ncomp = ncomp
 

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -23,25 +24,50 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("launch_f64.hip", "launch_f32.hip", "fgpe_capi.hip", "fgpe_init.hip")]
+CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 
+# Library variants: name -> extra flags.  "user_example" compiles the two
+# example user pointwise terms of csrc/user_term.hpp (the tests' custom
+# Hamiltonians) for the pencil lengths those tests use (FGPE_TEST_SIZES).
+VARIANTS = {"": [], "user_example": ["-DFGPE_USER_TERM_EXAMPLE", "-DFGPE_TEST_SIZES"]}
 
-# Library variants: name -> extra flags.  "user_example" compiles the example
-# user pointwise term of csrc/user_term.hpp (the tests' custom Hamiltonian).
-VARIANTS = {"": [], "user_example": ["-DFGPE_USER_TERM_EXAMPLE"]}
+
+def lengths(variant: str = ""):
+    """The pencil lengths a variant compiles (FGPE_FOR_EACH_N in csrc/ctx.hpp)."""
+    text = open(os.path.join(CSRC, "ctx.hpp")).read()
+    blocks = re.findall(r"#define FGPE_FOR_EACH_N\(X\)([^\n]*)", text)
+    small, full = (list(map(int, re.findall(r"X\((\d+)\)", b))) for b in blocks)
+    return small if "-DFGPE_TEST_SIZES" in VARIANTS[variant] else full
+
+
+def units(variant: str = ""):
+    """(source, object name, extra flags) of every translation unit of a variant:
+    the C ABI, the initialisers, the two dispatch units and one launcher unit
+    per precision and pencil length."""
+    out = [(os.path.join(CSRC, f), f + ".o", []) for f in
+           ("fgpe_capi.hip", "fgpe_init.hip", "launch_f64.hip", "launch_f32.hip")]
+    for prec, t in (("f64", "double"), ("f32", "float")):
+        for n in lengths(variant):
+            out.append((os.path.join(CSRC, "launch_inst.hip"), f"launch_{prec}_{n}.o",
+                        [f"-DFGPE_INST_T={t}", f"-DFGPE_INST_N={n}"]))
+    return out
 
 
 def lib_path(variant: str = "") -> str:
     return LIB if not variant else os.path.join(HERE, f"libfgpe_{variant}.so")
 
 
-def _compile(src: str, verbose: bool, variant: str = "") -> str:
-    bdir = os.path.join(BUILD_DIR, variant) if variant else BUILD_DIR
+def obj_dir(variant: str) -> str:
+    return os.path.join(BUILD_DIR, variant) if variant else BUILD_DIR
+
+
+def _compile(unit, verbose: bool, variant: str = "") -> str:
+    src, name, extra = unit
+    bdir = obj_dir(variant)
     os.makedirs(bdir, exist_ok=True)
-    obj = os.path.join(bdir, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, *VARIANTS[variant], "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+    obj = os.path.join(bdir, name)
+    cmd = [HIPCC, *HIP_FLAGS, *VARIANTS[variant], *extra, "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -51,21 +77,28 @@ def _compile(src: str, verbose: bool, variant: str = "") -> str:
     return obj
 
 
-def build_lib(verbose: bool = False, variants=("", "user_example")) -> str:
-    """Compile libfgpe.so and its variants (all sources in parallel)."""
+def link(variant: str, objs, verbose: bool = False) -> str:
+    out = lib_path(variant)
+    tmp = out + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
+
+
+def build_lib(verbose: bool = False, variants=("", "user_example"), jobs: int = 0) -> str:
+    """Compile libfgpe.so and its variants (all units in parallel, the largest first)."""
     os.makedirs(BUILD_DIR, exist_ok=True)
-    jobs = [(v, src) for v in variants for src in SOURCES]
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        objs = list(ex.map(lambda j: _compile(j[1], verbose, j[0]), jobs))
+    work = [(v, u) for v in variants for u in units(v)]
+    # the long lengths take longest: start them first
+    work.sort(key=lambda j: -int(re.sub(r"\D", "", j[1][1]) or 0))
+    jobs = jobs or min(len(work), os.cpu_count() or 8, 16)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda j: _compile(j[1], verbose, j[0]), work))
     for v in variants:
-        out = lib_path(v)
-        tmp = out + ".tmp"
-        mine = [o for (vv, _), o in zip(jobs, objs) if vv == v]
-        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *mine, "-o", tmp]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        os.replace(tmp, out)
+        link(v, [o for (vv, _), o in zip(work, objs) if vv == v], verbose)
     return LIB
 
 

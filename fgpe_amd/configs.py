@@ -74,8 +74,12 @@ class GridSpec:
 
     @property
     def dv(self):
-        return (self.lx / self.nx) * (self.ly / self.ny) * (self.lz / self.nz) if self.nz > 1 else \
-            (self.lx / self.nx) * (self.ly / self.ny)
+        """dx dy dz over the axes the grid has (dx dy in 2D, dx in 1D)."""
+        if self.nz > 1:
+            return (self.lx / self.nx) * (self.ly / self.ny) * (self.lz / self.nz)
+        if self.ny > 1:
+            return (self.lx / self.nx) * (self.ly / self.ny)
+        return self.lx / self.nx
 
     def kaxes(self):
         return (kaxis(self.nx, self.lx), kaxis(self.ny, self.ly),

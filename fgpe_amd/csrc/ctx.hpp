@@ -38,8 +38,13 @@ int fail(int code, const std::string& msg);
         if (r_ != FGPE_OK) return r_;     \
     } while (0)
 
-// Pencil lengths with a compiled plan.
-#define FGPE_FOR_EACH_N(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
+// Pencil lengths with a compiled plan.  A build for the user-term tests
+// (FGPE_TEST_SIZES) compiles the lengths those tests use only.
+#ifdef FGPE_TEST_SIZES
+#define FGPE_FOR_EACH_N(X) X(32) X(64)
+#else
+#define FGPE_FOR_EACH_N(X) X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
+#endif
 
 // Scalar slots of fgpe_ctx::scal (doubles) and ::flags (ints) on the device.
 enum {
@@ -149,9 +154,15 @@ struct TScope {
 };
 
 // Row pitch (elements) of a half-spectrum real channel: kx = 0..nx/2 stored,
-// padded to a multiple of 16 elements (128-byte row starts for every
-// supported nx and both precisions).
-inline long long dens_pitch(const fgpe_ctx* c) { return c->n[0] / 2 + 16; }
+// rows padded to nx/2 + 16 elements (128-byte row starts for nx >= 32 in
+// both precisions), but never wider than nx: the channel lives in a slot of
+// ntot = nx ny nz elements of the spectral buffer (at nx = 16 the padded
+// pitch, 24, would overrun it).
+inline long long dens_pitch(const fgpe_ctx* c)
+{
+    const long long padded = c->n[0] / 2 + 16;
+    return padded < c->n[0] ? padded : c->n[0];
+}
 
 // Slab s of nslab of the columns of a strided-pass argument set: complex
 // columns in equal parts (multiples of the tile width P), half-layout columns

@@ -166,14 +166,18 @@ template <int N, int V = 0> struct Plan;
 #define FGPE_PLAN(N_, V_, L_, T_, R0, R1, R2, D_)                 \
     template <> struct Plan<N_, V_> {                            \
         static constexpr int L = L_, T = T_, E = N_ / T_;        \
-        static constexpr int r[3] = {R0, R1, R2};                \
+        static constexpr int r[4] = {R0, R1, R2, 1};             \
         static constexpr bool distinct = D_;                     \
     };
-FGPE_PLAN(2, 0, 1, 1, 2, 1, 1, false)
-FGPE_PLAN(4, 0, 1, 1, 4, 1, 1, false)
-FGPE_PLAN(8, 0, 1, 1, 8, 1, 1, false)
-FGPE_PLAN(16, 0, 1, 1, 16, 1, 1, false)
+#define FGPE_PLAN4(N_, V_, T_, R0, R1, R2, R3, D_)                \
+    template <> struct Plan<N_, V_> {                            \
+        static constexpr int L = 4, T = T_, E = N_ / T_;         \
+        static constexpr int r[4] = {R0, R1, R2, R3};            \
+        static constexpr bool distinct = D_;                     \
+    };
+FGPE_PLAN(16, 0, 2, 4, 4, 4, 1, false)
 FGPE_PLAN(32, 0, 2, 4, 8, 4, 1, false)
+FGPE_PLAN(48, 0, 2, 4, 12, 4, 1, false)
 FGPE_PLAN(64, 0, 2, 8, 8, 8, 1, false)
 FGPE_PLAN(96, 0, 3, 8, 6, 4, 4, false)
 FGPE_PLAN(128, 0, 2, 8, 16, 8, 1, false)
@@ -181,19 +185,20 @@ FGPE_PLAN(192, 0, 3, 16, 12, 4, 4, false)
 FGPE_PLAN(256, 0, 2, 16, 16, 16, 1, false)
 FGPE_PLAN(384, 0, 3, 16, 12, 8, 4, false)
 FGPE_PLAN(512, 0, 3, 64, 8, 8, 8, false)
-FGPE_PLAN(2, 1, 1, 1, 2, 1, 1, false)
-FGPE_PLAN(4, 1, 1, 1, 4, 1, 1, false)
-FGPE_PLAN(8, 1, 1, 1, 8, 1, 1, false)
-FGPE_PLAN(16, 1, 1, 1, 16, 1, 1, false)
+FGPE_PLAN(1024, 0, 3, 64, 16, 8, 8, false)
+FGPE_PLAN(16, 1, 2, 4, 4, 4, 1, false)
 FGPE_PLAN(32, 1, 2, 4, 8, 4, 1, false)
+FGPE_PLAN(48, 1, 2, 4, 12, 4, 1, false)
 FGPE_PLAN(64, 1, 3, 16, 4, 4, 4, true)
 FGPE_PLAN(96, 1, 3, 8, 6, 4, 4, false)
 FGPE_PLAN(128, 1, 3, 16, 8, 4, 4, true)
 FGPE_PLAN(192, 1, 3, 16, 12, 4, 4, false)
 FGPE_PLAN(256, 1, 3, 32, 8, 4, 8, true)
-FGPE_PLAN(384, 1, 3, 16, 12, 8, 4, false)
+FGPE_PLAN4(384, 1, 32, 12, 4, 4, 2, true)
 FGPE_PLAN(512, 1, 3, 64, 8, 8, 8, false)
+FGPE_PLAN(1024, 1, 3, 64, 16, 8, 8, false)
 #undef FGPE_PLAN
+#undef FGPE_PLAN4
 template <int N, int V, int DIR, int S> __host__ __device__ constexpr int radix_of()
 {
     return DIR < 0 ? Plan<N, V>::r[S] : Plan<N, V>::r[Plan<N, V>::L - 1 - S];

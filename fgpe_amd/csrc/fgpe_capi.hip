@@ -178,7 +178,8 @@ template <typename T> std::vector<c2_t<T>> twiddle_table(int n)
     return w;
 }
 
-int last_axis(fgpe_ctx* c) { return c->n[2] > 1 ? 2 : 1; }
+// the last axis of a 3D (2) or 2D (1) grid; 0 for 1D grids, which have only x
+int last_axis(fgpe_ctx* c) { return c->n[2] > 1 ? 2 : (c->n[1] > 1 ? 1 : 0); }
 bool has_filter(const fgpe_hparams& h) { return h.a_dd != 0.0 || h.sc_U != 0.0; }
 
 // d_quench_as, hamiltonians/BEC_oneComponent.cu:167-178
@@ -206,10 +207,27 @@ template <typename T> int fft3d(fgpe_ctx* c, const void* in, void* out, int dir)
 // applied by the first forward pass after a prologue or a final stage.
 // half: the real-field channels are in the half-spectrum layout.
 // vpx: the moving-frame term of the Hamiltonian's full form is on.
+// 1D grids: the k-space multiply on the x stage's spectra
+template <typename T> int kmul_1d(fgpe_ctx* c, void* spec, int nch, int nc, const double* scale, bool filter,
+                                  const int* stop, bool half, bool vpx)
+{
+    LastArgs a = last_args(c, spec, 1, nch, nc, scale, filter, stop, half, vpx);
+    a.s.nx = c->n[0];
+    // the real-field channel's half spectrum: kx = 0..nx/2 (row pitch irrelevant, one row)
+    if (!half || nch == nc) a.s.hcols = 0;
+    if (nch > nc && !half) return fail(FGPE_EUNSUP, "1D grids keep the density as a half spectrum");
+    const int total = nc * c->n[0] + (nch > nc ? c->n[0] / 2 + 1 : 0);
+    TScope ts(c, FGPE_KCLASS_LAST);
+    hipLaunchKernelGGL(k_kmul_1d<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, a);
+    FGPE_LAUNCHCHK("k_kmul_1d");
+    return FGPE_OK;
+}
+
 template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, int nc, const double* scale,
                                              const int* stop, bool half, bool vpx = false)
 {
     const bool filter = has_filter(c->hp);
+    if (last_axis(c) == 0) return kmul_1d<T>(c, spec, nch, nc, scale, filter, stop, half, vpx);
     if (last_axis(c) == 2) {
         // slab by slab of columns: a slab's three passes run while it is still in the memory-side cache
         const int ns = half ? c->nslab : 1;
@@ -273,7 +291,6 @@ int ensure_ready(fgpe_ctx* c)
         if (!c->Vr[i]) return fail(FGPE_ESTATE, "no external potential uploaded for a component");
         if (!c->P0[i]) return fail(FGPE_ESTATE, "no state uploaded for a component");
     }
-    if (c->n[1] == 1) return fail(FGPE_EUNSUP, "1D grids are not supported");
     return FGPE_OK;
 }
 
@@ -543,7 +560,7 @@ int adj_check(fgpe_ctx* c)
     if (c->hp.sc_U != 0.0) return fail(FGPE_EUNSUP, "the adjoint supports the dipolar k-space kernel only");
     if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
     if (c->hp.vpx[0] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
-    if (UserTerm::enabled) return fail(FGPE_EUNSUP, "the adjoint does not know the user term's derivative");
+    if (c->hp.user_id != 0) return fail(FGPE_EUNSUP, "the adjoint does not know the user term's derivative");
     return FGPE_OK;
 }
 
@@ -736,6 +753,7 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
     for (int ci = 0; ci < 3 && rc == FGPE_OK; ++ci) {
         const int cls = classes[ci];
         if (cls == FGPE_KCLASS_STRIDED && la != 2) continue;
+        if (cls == FGPE_KCLASS_LAST && la == 0) continue;
         const int axis = cls == FGPE_KCLASS_STRIDED ? 1 : (cls == FGPE_KCLASS_LAST ? la : 0);
         double best = 1e300;
         int bestv = 0;
@@ -803,6 +821,7 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         };
         for (int o = 0; o < 16 && rc == FGPE_OK; ++o) {
             if (la != 2 && (o & 5)) continue;   // 2D grids have no y passes
+            if (la == 0 && (o & 2)) continue;   // 1D grids no last-axis pass
             c->order = o;
             rc = stage();   // warm-up
             if (rc != FGPE_OK) break;
@@ -965,6 +984,8 @@ int group_check(fgpe_group* g)
         if (c->stream != c0->stream) return fail(FGPE_EINVAL, "group instances must share the stream");
         if (c->nch != c0->nch) return fail(FGPE_EINVAL, "group instances must all have or all lack a k-space kernel");
         if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "group instances need a real trap");
+        if (c->hp.user_id != 0) return fail(FGPE_EUNSUP, "group instances run without a user term");
+        if (last_axis(c) == 0) return fail(FGPE_EUNSUP, "groups of 1D grids are not supported");
     }
     return FGPE_OK;
 }
@@ -1057,7 +1078,20 @@ extern "C" {
 int fgpe_user_term(int* enabled)
 {
     if (!enabled) return fail(FGPE_EINVAL, "NULL argument");
-    *enabled = UserTerm::enabled ? 1 : 0;
+    *enabled = user_term_count() > 0 ? 1 : 0;
+    return FGPE_OK;
+}
+
+int fgpe_user_term_ids(int* ids, int cap, int* count)
+{
+    if (!count || (cap > 0 && !ids)) return fail(FGPE_EINVAL, "NULL argument");
+    int n = 0;
+#define FGPE_USER_TERM_ID(ID, S) \
+    if (n < cap) ids[n] = ID;    \
+    ++n;
+    FGPE_USER_TERMS(FGPE_USER_TERM_ID)
+#undef FGPE_USER_TERM_ID
+    *count = n;
     return FGPE_OK;
 }
 
@@ -1070,8 +1104,14 @@ int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out)
     if (precision != FGPE_PREC_F64 && precision != FGPE_PREC_MIXED)
         return fail(FGPE_EINVAL, "precision must be FGPE_PREC_F64 or FGPE_PREC_MIXED");
     if (nx <= 1 || ny < 1 || nz < 1) return fail(FGPE_EINVAL, "grid extents must be positive (nx > 1)");
-    if (!supported_n(nx) || (ny > 1 && !supported_n(ny)) || (nz > 1 && !supported_n(nz)))
-        return fail(FGPE_EUNSUP, "grid extents must be 1 or one of 32, 64, 96, 128, 192, 256, 512");
+    if (!supported_n(nx) || (ny > 1 && !supported_n(ny)) || (nz > 1 && !supported_n(nz))) {
+        std::string list;
+#define FGPE_LIST_N(N_) list += (list.empty() ? "" : ", ") + std::to_string(N_);
+        FGPE_FOR_EACH_N(FGPE_LIST_N)
+#undef FGPE_LIST_N
+        return fail(FGPE_EUNSUP, "grid extents must be 1 or one of " + list + " (nx, and ny, nz when above 1); got " +
+                                     std::to_string(nx) + " x " + std::to_string(ny) + " x " + std::to_string(nz));
+    }
     if (ny == 1 && nz > 1) return fail(FGPE_EUNSUP, "a grid with ny = 1 must have nz = 1");
     int dev = 0;
     {
@@ -1208,8 +1248,13 @@ int fgpe_set_hamiltonian(fgpe_ctx* c, const fgpe_hparams* hp)
     if (hp->sc_U != 0.0 && (hp->sc_dim < 1 || hp->sc_dim > 3)) return fail(FGPE_EINVAL, "sc_dim must be 1, 2 or 3");
     if (hp->sc_U != 0.0 && !c->kset) return fail(FGPE_ESTATE, "set the k space before a soft-core term");
     if (hp->ncomp == 2 && hp->quench_on) return fail(FGPE_EUNSUP, "the a(t) ramp is one-component only");
-    c->hp = *hp;
+    if (hp->user_id != 0 && !user_term_compiled(hp->user_id))
+        return fail(FGPE_EINVAL, "user term " + std::to_string(hp->user_id) + " is not compiled into this library");
     const int nch = hp->ncomp + (has_filter(*hp) ? 1 : 0);
+    // the half-spectrum density channel must fit its ntot-element slot
+    if (nch > hp->ncomp && dens_pitch(c) * c->n[1] * c->n[2] > c->ntot)
+        return fail(FGPE_EUNSUP, "density channel rows wider than the grid's x extent");
+    c->hp = *hp;
     const size_t N = (size_t)c->ntot;
     if (nch != c->nch) {
         dfree(c, &c->spec, 2 * c->tsz() * N * c->nch);
@@ -1316,11 +1361,11 @@ int fgpe_laplacian_f64(fgpe_ctx* c, const void* psi, void* out)
 {
     if (!c || !psi || !out) return fail(FGPE_EINVAL, "NULL argument");
     if (!c->kset) return fail(FGPE_ESTATE, "fgpe_set_kspace has not been called");
-    if (c->n[1] == 1) return fail(FGPE_EUNSUP, "1D grids are not supported");
     FGPE_RET(Launch<double>::fft_x(c, psi, out, -1));
     const int la = last_axis(c);
+    if (la == 0) FGPE_RET(kmul_1d<double>(c, out, 1, 1, nullptr, false, nullptr, false, false));
     if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, -1, 1, 1, nullptr, nullptr, false));
-    FGPE_RET(Launch<double>::last(c, out, la, 1, 1, nullptr, false, nullptr, false));
+    if (la > 0) FGPE_RET(Launch<double>::last(c, out, la, 1, 1, nullptr, false, nullptr, false));
     if (la == 2) FGPE_RET(Launch<double>::strided(c, out, 1, +1, 1, 1, nullptr, nullptr, false));
     FGPE_RET(Launch<double>::fft_x(c, out, out, +1));
     return FGPE_OK;

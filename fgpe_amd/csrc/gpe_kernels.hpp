@@ -39,7 +39,9 @@ template <typename T, int N, int V = 0> __host__ __device__ constexpr int px_of(
 template <typename T, int N, int V = 0> __host__ __device__ constexpr int ps_of()
 {
     constexpr int base = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
-    return V == 0 ? base : 2 * base;
+    // a pencil of 1024 takes 17 KB (FP64) of exchange space per column: a
+    // wider tile would not fit the last-axis pass's LDS
+    return (V == 0 || N >= 1024) ? base : 2 * base;
 }
 
 // The arguments of this block: the by-value kernel argument, or this
@@ -94,16 +96,20 @@ template <int NV> __device__ __forceinline__ void block_sum_store(double (&s)[NV
 
 // ---------------------------------------------------------------------------
 // Standalone x-axis transform of rows of length N (the building-block API).
+// The last block may hold pencils past the last row: they transform row 0
+// again and store nothing.
 template <typename T, int N, int V, int DIR>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
-k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ twg)
+k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ twg, long long nrows)
 {
     using T2 = c2_t<T>;
     const gp<const T2> tw = glob(twg);
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     __shared__ T2 lds[P * lds_stride<N>()];
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
-    const long long row = (long long)blockIdx.x * P + p;
+    const long long r = (long long)blockIdx.x * P + p;
+    const bool pv = r < nrows;
+    const long long row = pv ? r : 0;
     const T2* src = in + row * N;
     T2* dst = out + row * N;
     T2* my = lds + p * lds_stride<N>();
@@ -112,14 +118,16 @@ k_fft_x(const c2_t<T>* in, c2_t<T>* out, const c2_t<T>* __restrict__ twg)
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = src[elemA<N, V>(t, q)];
         fft_forward<T2, N, V, true>(v, t, my, tw);
+        if (pv)
 #pragma unroll
-        for (int q = 0; q < E; ++q) dst[elemB<N, V>(t, q)] = v[q];
+            for (int q = 0; q < E; ++q) dst[elemB<N, V>(t, q)] = v[q];
     } else {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = src[elemB<N, V>(t, q)];
         fft_inverse<T2, N, V, true>(v, t, my, tw);
+        if (pv)
 #pragma unroll
-        for (int q = 0; q < E; ++q) dst[elemA<N, V>(t, q)] = v[q];
+            for (int q = 0; q < E; ++q) dst[elemA<N, V>(t, q)] = v[q];
     }
 }
 
@@ -169,10 +177,11 @@ struct StridedArgs {
 };
 
 // Blocks of one strided launch: per outer index the complex-layout tiles,
-// then the half-layout ones.
+// then the half-layout ones.  A last partial tile of either layout has spare
+// lanes that compute on zeros and store nothing.
 template <int P> __host__ __device__ inline long long strided_blocks(const StridedArgs& a)
 {
-    const long long ntf = a.ncols / P;
+    const long long ntf = (a.ncols + P - 1) / P;
     if (a.hcols == 0) return (long long)a.nchan * ntf * a.nouter;
     const long long nth = (a.hcols + P - 1) / P;
     return ((long long)a.ncomp * ntf + (long long)(a.nchan - a.ncomp) * nth) * a.nouter;
@@ -195,7 +204,7 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     // starts where it ended finds both in the memory-side cache.  Selects,
     // not branches: every value is block-uniform.
     TileAt r;
-    const int ntf = a.ncols / P;
+    const int ntf = (a.ncols + P - 1) / P;
     const int nth = a.hcols ? (a.hcols + P - 1) / P : 1;
     const int nfc = a.hcols ? a.ncomp : a.nchan;          // complex-layout channels
     const long long nfo = (long long)nfc * ntf;           // their tiles per outer index
@@ -212,7 +221,7 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     const int xt = (int)(wl - (long long)cl * nt);
     r.ch = half ? a.ncomp + cl : cl;
     const int loc = xt * P + p;   // column within this launch
-    r.valid = !half || loc < a.hcols;
+    r.valid = loc < (half ? a.hcols : a.ncols);
     r.ix = (half ? a.hx0 : a.x0) + loc;
     if (!r.valid) r.ix = 0;   // keeps table lookups in range; nothing is stored
     r.sp_off = r.outer * a.outer_stride + r.ix;
@@ -482,6 +491,49 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// 1D grids: the k-space multiply between the x stage's forward and inverse
+// transforms (there is no pass along another axis to carry it), the same
+// factors as the last-axis pass with k = k_x: (19.739208802/N) k^2 - vpx_c k
+// on component c (src/parallel.cu:194-205) and Vs(k)/N on the real-field
+// channel's half spectrum (kx = 0..N/2).  One thread per spectral element,
+// complex channels first (natural order), factors in FP64.  A pending scale
+// multiplies component c by scale[c] and the density by scale[0]^2.
+template <typename T> __global__ void __launch_bounds__(256) k_kmul_1d(LastArgs a)
+{
+    using T2 = c2_t<T>;
+    if (a.s.stop && *glob(a.s.stop)) return;
+    const int nx = a.s.nx, nc = a.s.ncomp;
+    const int nh = a.s.nchan > nc ? a.s.hcols : 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc * nx + nh) return;
+    const gp<T2> data = glob((T2*)a.s.data);
+    const gp<const double2> kx = glob(a.kX64);
+    const gp<const double> scale = glob(a.s.scale);
+    if (i < nc * nx) {
+        const int c = i / nx, e = i - c * nx;
+        const double k = kx[e].x;
+        const double w = vpx_of(a, c) * k;
+        const long long at = (long long)c * a.s.chan_stride + e;
+        double2 z = to_d2<T>(data[at]);
+        if (a.s.scale) z = double2{scale[c] * z.x, scale[c] * z.y};
+        data[at] = from_d2<T>(double2{(a.lap_scale * z.x) * (k * k) - w * z.x, (a.lap_scale * z.y) * (k * k) - w * z.y});
+        return;
+    }
+    const int e = i - nc * nx;
+    const double k = kx[e].x, kp = kx[e].y;
+    double vt = 0.0;
+    if (a.c_dd != 0.0) {
+        vt = vdd_of(a.c_dd, k * a.dX, k * k);
+        if (kp != -k) vt = 0.5 * (vt + vdd_of(a.c_dd, kp * a.dX, kp * kp));
+    }
+    if (a.wtab) vt = vt + (sizeof(T) == 8 ? (double)glob((const double*)a.wtab)[e] : (double)glob((const float*)a.wtab)[e]);
+    double f = vt * a.inv_ntot;
+    if (a.s.scale) f = f * (scale[0] * scale[0]);
+    const T2 v = data[a.s.hbase + e];
+    data[a.s.hbase + e] = T2{(T)((double)v.x * f), (T)((double)v.y * f)};
+}
+
+// ---------------------------------------------------------------------------
 // x-axis stage kernel, NC = 1 or 2 components.
 //   MODE_PROLOGUE: spec <- Fx(psi_c) [, Fx(sum_c |psi_c|^2)] of the stored
 //                  (raw) state; a pending scale is applied by the next forward
@@ -587,11 +639,13 @@ __device__ __forceinline__ UserTermIn user_in(const XArgs<T>& a, long long g, in
     return UserTermIn{a.t, (int)(g - r * N), (int)(r % a.ny), (int)(r / a.ny), c, pr, pi, n, no, a.user};
 }
 
-template <typename T, int N, int V, int NC, bool DENS, int MODE, bool NT = false>
+// UT: the user term compiled in (UserTerm<UT>; 0: none).
+template <typename T, int N, int V, int NC, bool DENS, int MODE, bool NT = false, int UT = 0>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 __attribute__((amdgpu_waves_per_eu(xstage_waves<T, N, V, NC, MODE>())))
 k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restrict__ tab)
 {
+    using UTerm = UserTerm<UT>;
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
     constexpr int NV = 3 * NC;
@@ -610,11 +664,17 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
     const gp<const T> Vi[2] = {glob(a.Vi[0]), glob(a.Vi[1])};
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
     // row group of this block (reversed: the kernel starts on what the passes
-    // before it wrote last); partial sums stay indexed by the row group
+    // before it wrote last); partial sums stay indexed by the row group.  The
+    // last group may hold pencils past the last row (grids with fewer rows
+    // than a block's pencils, e.g. 1D): they work on row 0 again, store
+    // nothing and add nothing to the sums.
     const unsigned bx = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
-    const long long row0 = ((long long)bx * P + p) * N;
+    const long long rowi = (long long)bx * P + p;
+    const bool pv = rowi < a.ntot / N;
+    const long long row = pv ? rowi : 0;
+    const long long row0 = row * N;
     T2* my = lds + p * lds_stride<N>();
-    const gp<T2> specd = glob(a.specd) + ((long long)bx * P + p) * a.dpitch;   // this row's half spectrum
+    const gp<T2> specd = glob(a.specd) + row * a.dpitch;   // this row's half spectrum
 
     T2 v[E];
     T nd[E];             // density of the new state (DENS)
@@ -631,8 +691,9 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
             }
             fft_forward<T2, N, V, true>(v, t, my, tw);
             const gp<T2> sc = spec + c * a.ntot + row0;
+            if (pv)
 #pragma unroll
-            for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
+                for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
         }
     } else {
         // phi, the k-space potential of the density, is held from the start
@@ -721,15 +782,15 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                         if (DENS) eX += 0.5 * (double)phi_at(q);
                         he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
                     }
-                    if constexpr (UserTerm::enabled) {
-                        const UserTermOut w = UserTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
+                    if constexpr (UTerm::enabled) {
+                        const UserTermOut w = UTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
                         h = double2{h.x + (w.wr * ps.x - w.wi * ps.y), h.y + (w.wr * ps.y + w.wi * ps.x)};
                         if (MODE == MODE_APPLY || MODE == MODE_HB)
                             he = double2{he.x + (w.er * ps.x - w.ei * ps.y), he.y + (w.er * ps.y + w.ei * ps.x)};
                     }
                     if (MODE == MODE_APPLY) {
-                        if (a.hmu) glob(a.hmu)[c * a.ntot + g] = h;
-                        if (a.hen) glob(a.hen)[c * a.ntot + g] = he;
+                        if (a.hmu && pv) glob(a.hmu)[c * a.ntot + g] = h;
+                        if (a.hen && pv) glob(a.hen)[c * a.ntot + g] = he;
                         sums[3 * c + 0] += n;
                         sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
                         sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
@@ -738,8 +799,10 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                         const double m = a.mu ? glob(a.mu)[c] : 0.0;
                         const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
                                                    (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
-                        stn<NT>(PS[c], g, from_d2<T>(ps));
-                        stn<NT>(P0[c], g, nw);
+                        if (pv) {
+                            stn<NT>(PS[c], g, from_d2<T>(ps));
+                            stn<NT>(P0[c], g, nw);
+                        }
                         sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
                         sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
                         sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
@@ -755,13 +818,13 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                             const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
                                                        po.y + (a.cr * h.y + a.ci * h.x)};
                             if (a.stage == 1) {
-                                stn<NT>(K[c], g, from_d2<T>(h));
+                                if (pv) stn<NT>(K[c], g, from_d2<T>(h));
                             } else {
                                 const double2 k = to_d2<T>(ldn<NT>(K[c], g));
-                                stn<NT>(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
+                                if (pv) stn<NT>(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
                             }
                             const T2 st = from_d2<T>(nw);
-                            stn<NT>(PSo[c] ? PSo[c] : PS[c], g, st);
+                            if (pv) stn<NT>(PSo[c] ? PSo[c] : PS[c], g, st);
                             v[q] = st;
                         } else {
                             // FinalRKStep: psi = old + c*(k + h)/6
@@ -769,7 +832,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                             const double aX = k.x + h.x, aY = k.y + h.y;
                             const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
                                                        po.y + (a.cr * aY + a.ci * aX) / 6.0};
-                            stn<NT>(P0[c], g, nw);
+                            if (pv) stn<NT>(P0[c], g, nw);
                             v[q] = from_d2<T>(nw);
                             if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                         }
@@ -816,8 +879,8 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                     if (NC == 2) rX = rX + f4o * no;
                     if (DENS) rX += (float)phi_at(q);
                     float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
-                    if constexpr (UserTerm::enabled) {
-                        const UserTermOut w = UserTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
+                    if constexpr (UTerm::enabled) {
+                        const UserTermOut w = UTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
                         h = float2{h.x + (float)(w.wr * ps.x - w.wi * ps.y), h.y + (float)(w.wr * ps.y + w.wi * ps.x)};
                     }
                     if (a.stage < 4) {
@@ -850,9 +913,11 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                             }
                         }
                         const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
-                        if (a.stage == 1) stn<NT>(K[c], g, h);
-                        else stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
-                        stn<NT>(PSo[c] ? PSo[c] : PS[c], g, nw);
+                        if (pv) {
+                            if (a.stage == 1) stn<NT>(K[c], g, h);
+                            else stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
+                            stn<NT>(PSo[c] ? PSo[c] : PS[c], g, nw);
+                        }
                         v[q] = nw;
                     } else {
                         const float2 k = ldn<NT>(K[c], g);
@@ -860,7 +925,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                         const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
                         const double2 r = ldn<NT>(P0[c], g);
                         const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
-                        stn<NT>(P0[c], g, nw);
+                        if (pv) stn<NT>(P0[c], g, nw);
                         v[q] = from_d2<T>(nw);
                         if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                     }
@@ -869,12 +934,17 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
             }
             if (MODE != MODE_APPLY) {
                 fft_forward<T2, N, V, true>(v, t, my, tw);
+                if (pv)
 #pragma unroll
-                for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
+                    for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
             }
         }
-        if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm))
+        if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm)) {
+            if (!pv)
+#pragma unroll
+                for (int i = 0; i < NV; ++i) sums[i] = 0.0;
             block_sum_store<NV>(sums, glob(a.partials) + (long long)NV * bx);
+        }
         if (MODE == MODE_APPLY) return;
     }
     if (DENS) {
@@ -884,7 +954,7 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const int e = elemB<N, V>(t, q);
-            if (e <= N / 2) specd[e] = v[q];
+            if (e <= N / 2 && pv) specd[e] = v[q];
         }
     }
 }

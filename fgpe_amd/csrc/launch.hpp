@@ -1,8 +1,12 @@
 // launch.hpp -- template launchers of the pass kernels, dispatched over the
 // compiled pencil lengths and the two plan variants (the variant of each
 // kernel class and axis is a per-context choice, set by fgpe_autotune or
-// fgpe_set_variant).  Included once per precision (launch_f64.hip,
-// launch_f32.hip) so the two sets of kernels compile in parallel.
+// fgpe_set_variant).
+//
+// The per-length launchers (detail::*_n<T, N, V>) are instantiated one length
+// and precision per translation unit (launch_inst.hip, compiled once per
+// (T, N) by the build recipe) so that the kernels compile in parallel; the
+// dispatch units (launch_f64.hip, launch_f32.hip) only declare them.
 #pragma once
 #include "ctx.hpp"
 
@@ -14,20 +18,23 @@ template <typename T> const c2_t<T>* tw_of(fgpe_ctx* c, int axis)
     return (const c2_t<T>*)(sizeof(T) == 8 ? c->tw64[axis] : c->tw32[axis]);
 }
 
+// blocks of an x-pencil launch: rows in groups of P pencils, the last group
+// possibly partial
+template <int P> inline long long x_groups(const fgpe_ctx* c, int n) { return (c->ntot / n + P - 1) / P; }
+
 template <typename T, int N, int V> int fft_x_n(fgpe_ctx* c, const void* in, void* out, int dir)
 {
     constexpr int P = px_of<T, N, V>();
-    const long long rows = c->ntot / N;
-    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
+    dim3 grid((unsigned)x_groups<P>(c, N)), block(P * Plan<N, V>::T);
     using T2 = c2_t<T>;
     TScope ts(c, FGPE_KCLASS_FFTX);
+    const long long rows = c->ntot / N;
     if (dir < 0)
         hipLaunchKernelGGL((k_fft_x<T, N, V, -1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out,
-                           tw_of<T>(c, 0));
+                           tw_of<T>(c, 0), rows);
     else
         hipLaunchKernelGGL((k_fft_x<T, N, V, +1>), grid, block, 0, c->stream, (const T2*)in, (T2*)out,
-                           tw_of<T>(c, 0));
+                           tw_of<T>(c, 0), rows);
     FGPE_LAUNCHCHK("k_fft_x");
     return FGPE_OK;
 }
@@ -39,7 +46,6 @@ template <typename T, int N, int V> int strided_n(fgpe_ctx* c, void* data, int a
     constexpr int P = ps_of<T, N, V>();
     StridedArgs a = slab_of(strided_args(c, data, axis, scale, nch, ncomp, half, stop), slab, nslab);
     a.rev = (c->order >> (dir < 0 ? 0 : 2)) & 1;
-    if (a.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a), (unsigned)ninst), block(P * Plan<N, V>::T);
     const StridedArgs* t = (const StridedArgs*)tab;
     TScope ts(c, FGPE_KCLASS_STRIDED);
@@ -59,7 +65,6 @@ template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis
     LastArgs a = last_args(c, data, axis, nch, ncomp, scale, filter, stop, half, vpx);
     a.s = slab_of(a.s, slab, nslab);
     a.s.rev = (c->order >> 1) & 1;
-    if (a.s.ncols % P) return fail(FGPE_EUNSUP, "slab width not a multiple of the strided tile width");
     dim3 grid((unsigned)strided_blocks<P>(a.s), (unsigned)ninst), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_LAST);
     hipLaunchKernelGGL((k_last<T, N, V>), grid, block, 0, c->stream, tw_of<T>(c, axis), a, (const LastArgs*)tab);
@@ -67,13 +72,11 @@ template <typename T, int N, int V> int last_n(fgpe_ctx* c, void* data, int axis
     return FGPE_OK;
 }
 
-template <typename T, int N, int V, int NC, bool DENS, int MODE>
-int xstage_t(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
+template <typename T, int N, int V, int NC, bool DENS, int MODE, int UT>
+int xstage_u(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
 {
     constexpr int P = px_of<T, N, V>();
-    const long long rows = c->ntot / N;
-    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
-    dim3 grid((unsigned)(rows / P), (unsigned)ninst), block(P * Plan<N, V>::T);
+    dim3 grid((unsigned)x_groups<P>(c, N), (unsigned)ninst), block(P * Plan<N, V>::T);
     XArgs<T> b = a;
     b.rev = (c->order >> 3) & 1;
     TScope ts(c, FGPE_KCLASS_XSTAGE);
@@ -83,16 +86,28 @@ int xstage_t(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
     // other group of the sweep.
     if constexpr ((N >= 256 && MODE == MODE_RK) || MODE == MODE_HB) {
         if (c->working_set() * (size_t)ninst > kMallBytes || (MODE == MODE_HB && tab)) {
-            hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE, true>), grid, block, 0, c->stream, b,
+            hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE, true, UT>), grid, block, 0, c->stream, b,
                                tw_of<T>(c, 0), (const XArgs<T>*)tab);
             FGPE_LAUNCHCHK("k_xstage");
             return FGPE_OK;
         }
     }
-    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE>), grid, block, 0, c->stream, b, tw_of<T>(c, 0),
+    hipLaunchKernelGGL((k_xstage<T, N, V, NC, DENS, MODE, false, UT>), grid, block, 0, c->stream, b, tw_of<T>(c, 0),
                        (const XArgs<T>*)tab);
     FGPE_LAUNCHCHK("k_xstage");
     return FGPE_OK;
+}
+
+// the context's user term (fgpe_hparams::user_id, checked against the
+// build's list when the Hamiltonian is set)
+template <typename T, int N, int V, int NC, bool DENS, int MODE>
+int xstage_t(fgpe_ctx* c, const XArgs<T>& a, const void* tab, int ninst)
+{
+#define FGPE_USER_CASE(ID, S) \
+    if (c->hp.user_id == ID) return xstage_u<T, N, V, NC, DENS, MODE, ID>(c, a, tab, ninst);
+    FGPE_USER_TERMS(FGPE_USER_CASE)
+#undef FGPE_USER_CASE
+    return xstage_u<T, N, V, NC, DENS, MODE, 0>(c, a, tab, ninst);
 }
 
 template <typename T, int N, int V, int NC, bool DENS>
@@ -121,7 +136,7 @@ template <typename T, typename YT, int N, int V, bool DENS> int adj_t(fgpe_ctx* 
 {
     constexpr int P = px_of<T, N, V>();
     const long long rows = c->ntot / N;
-    if (rows % P) return fail(FGPE_EUNSUP, "x rows not a multiple of the pencils per block");
+    if (rows % P) return fail(FGPE_EUNSUP, "the adjoint needs whole blocks of x rows");
     dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_XSTAGE);
     if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, V, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
@@ -138,29 +153,40 @@ template <typename T, int N, int V> int adj_n(fgpe_ctx* c, const AdjArgs& a, boo
 
 }  // namespace detail
 
-#define FGPE_DISPATCH(len, var, CALL)                                        \
-    switch (len) {                                                           \
-    case 32: return (var) ? CALL(32, 1) : CALL(32, 0);                       \
-    case 64: return (var) ? CALL(64, 1) : CALL(64, 0);                       \
-    case 96: return (var) ? CALL(96, 1) : CALL(96, 0);                       \
-    case 128: return (var) ? CALL(128, 1) : CALL(128, 0);                    \
-    case 192: return (var) ? CALL(192, 1) : CALL(192, 0);                    \
-    case 256: return (var) ? CALL(256, 1) : CALL(256, 0);                    \
-    case 512: return (var) ? CALL(512, 1) : CALL(512, 0);                    \
+// Declarations (EXT = extern) or instantiations (EXT empty) of the
+// per-length launchers of one precision, length and variant.
+#define FGPE_LAUNCH_INST(EXT, T, N, V)                                                                            \
+    EXT template int detail::fft_x_n<T, N, V>(fgpe_ctx*, const void*, void*, int);                                \
+    EXT template int detail::strided_n<T, N, V>(fgpe_ctx*, void*, int, int, int, int, const double*, const int*,  \
+                                                bool, const void*, int, int, int);                                \
+    EXT template int detail::last_n<T, N, V>(fgpe_ctx*, void*, int, int, int, const double*, bool, const int*,    \
+                                             bool, const void*, int, int, int, bool);                             \
+    EXT template int detail::xstage_n<T, N, V>(fgpe_ctx*, const XArgs<T>&, int, bool, int, const void*, int);     \
+    EXT template int detail::adj_n<T, N, V>(fgpe_ctx*, const AdjArgs&, bool, bool, bool);
+
+// Dispatch on the runtime length to the compiled one.
+#define FGPE_DISPATCH_CASE(N_) \
+    case N_: return var ? CALL(N_, 1) : CALL(N_, 0);
+#define FGPE_DISPATCH(len, variant)          \
+    {                                        \
+        const int var = (variant);           \
+        switch (len) {                       \
+            FGPE_FOR_EACH_N(FGPE_DISPATCH_CASE) \
+        }                                    \
     }
 
 template <typename T> int Launch<T>::adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
 {
 #define CALL(N_, V_) detail::adj_n<T, N_, V_>(c, a, y64, dens, epi)
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
 
 template <typename T> long long Launch<T>::xblocks(fgpe_ctx* c)
 {
-#define CALL(N_, V_) (c->ntot / N_ / px_of<T, N_, V_>())
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
+#define CALL(N_, V_) detail::x_groups<px_of<T, N_, V_>()>(c, N_)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
 #undef CALL
     return 0;
 }
@@ -168,7 +194,7 @@ template <typename T> long long Launch<T>::xblocks(fgpe_ctx* c)
 template <typename T> int Launch<T>::fft_x(fgpe_ctx* c, const void* in, void* out, int dir)
 {
 #define CALL(N_, V_) detail::fft_x_n<T, N_, V_>(c, in, out, dir)
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_FFTX][0], CALL)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_FFTX][0])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
@@ -179,7 +205,7 @@ template <typename T> int Launch<T>::strided(fgpe_ctx* c, void* data, int axis, 
 {
 #define CALL(N_, V_) \
     detail::strided_n<T, N_, V_>(c, data, axis, dir, nch, ncomp, scale, stop, half, tab, ninst, slab, nslab)
-    FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_STRIDED][axis], CALL)
+    FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_STRIDED][axis])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported strided axis length");
 }
@@ -190,7 +216,7 @@ template <typename T> int Launch<T>::last(fgpe_ctx* c, void* data, int axis, int
 {
 #define CALL(N_, V_) \
     detail::last_n<T, N_, V_>(c, data, axis, nch, ncomp, scale, filter, stop, half, tab, ninst, slab, nslab, vpx)
-    FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis], CALL)
+    FGPE_DISPATCH(c->n[axis], c->var[FGPE_KCLASS_LAST][axis])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported last axis length");
 }
@@ -199,11 +225,9 @@ template <typename T> int Launch<T>::xstage(fgpe_ctx* c, const XArgs<T>& a, int 
                                             const void* tab, int ninst)
 {
 #define CALL(N_, V_) detail::xstage_n<T, N_, V_>(c, a, nc, dens, mode, tab, ninst)
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0], CALL)
+    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
-
-#undef FGPE_DISPATCH
 
 }  // namespace fgpe

@@ -1,3 +1,11 @@
-// FP32 (mixed-precision bulk) instantiation of the pass-kernel launchers.
+// FP32 dispatch of the pass-kernel launchers (the per-length launchers they
+// call are instantiated in launch_inst.hip, one length per unit).
 #include "launch.hpp"
+
+namespace fgpe {
+#define FGPE_DECL(N_) FGPE_LAUNCH_INST(extern, float, N_, 0) FGPE_LAUNCH_INST(extern, float, N_, 1)
+FGPE_FOR_EACH_N(FGPE_DECL)
+#undef FGPE_DECL
+}  // namespace fgpe
+
 template struct fgpe::Launch<float>;

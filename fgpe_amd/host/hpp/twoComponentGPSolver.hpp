@@ -28,8 +28,15 @@ public:
     void runImagTimeEvol();
     void runRealTimeEvol();
 
-    virtual void alg_calcHpsi() {}
-    virtual void alg_calcHpsiMU() {}
+    // H is the library's fused kernel, configured by hparams (and a user term
+    // of the library, hparams.user_id), and every observable applies it
+    // itself, so calling these is a no-op.  A reference plugin that defines
+    // its own alg_calcHpsi / alg_calcHpsiMU (hamiltonians/BEC_twoComponent.cu
+    // :117-139) would have those kernels silently skipped: as final members
+    // they make such a definition a compile error instead, so its terms move
+    // into hparams or a user term.
+    virtual void alg_calcHpsi() final {}
+    virtual void alg_calcHpsiMU() final {}
 
     virtual void call_IM_init() = 0;
     virtual void call_IM_loop_before_step() = 0;

@@ -72,22 +72,33 @@ typedef struct fgpe_hparams {
                            the one-component RK4 step (stages 1-3 use
                            alg_calcHpsiMU, src/oneComponentGPSolver.cu:170-187;
                            the two-component RK4 uses only alg_calcHpsiMU). */
-    double user[8];     /* parameters of the library's user pointwise term
-                           (fgpe_amd/csrc/user_term.hpp), read by its functor */
+    double user[8];     /* parameters of the user pointwise term, read by its
+                           functor (fgpe_amd/csrc/user_term.hpp)             */
+    int user_id;        /* user pointwise term of this context: 0 none, else
+                           one of the ids fgpe_user_term_ids lists         */
 } fgpe_hparams;
 
 /* Custom Hamiltonians (the plugin kernels SingleTemplate_template /
    SingleTemplate_MU_template, hamiltonians/single_template.cu:333-373, that
-   a user edits): the library compiles one user functor into the fused
-   pointwise stage of every H application, which adds w psi_c to H psi_c
-   and w_en psi_c to the energy operator, w from (t, grid index, component,
-   psi_c, densities, user[8]).  Editing fgpe_amd/csrc/user_term.hpp and
-   rebuilding the library replaces it; the shipped library has none.
-   *enabled = 1 when this build has one.  Adjoint gradients reject a build
-   with a user term (its derivative is not known to the library). */
+   a user edits; main.cu:41-69 picks one of several plugins by name): the
+   library compiles a list of user functors, each with an id, into the fused
+   pointwise stage of every H application; fgpe_hparams::user_id selects one
+   per context.  It adds w psi_c to H psi_c and w_en psi_c to the energy
+   operator, w from (t, grid index, component, psi_c, densities, user[8]).
+   Editing fgpe_amd/csrc/user_term.hpp and rebuilding the library changes
+   the list; the shipped library has none.  *enabled = 1 when this build has
+   at least one; fgpe_user_term_ids writes up to cap ids and *count, the
+   number compiled.  Adjoint gradients and batched groups reject a context
+   with a user term (its derivative is not known to the library; group
+   launches run at a fixed time). */
 int fgpe_user_term(int* enabled);
+int fgpe_user_term_ids(int* ids, int cap, int* count);
 
 /* ---- context (replaces Cuda::Cuda, src/parallel.cu:14-113) ------------- */
+/* Grid extents: nx and, when above 1, ny and nz are each one of 16, 32, 48,
+   64, 96, 128, 192, 256, 384, 512, 1024 (other lengths return FGPE_EUNSUP
+   with the list); a 2D grid has nz = 1 and a 1D grid ny = nz = 1
+   (src/parallel.cu:67-110, src/params.cu:145-154). */
 int fgpe_ctx_create(int nx, int ny, int nz, int precision, fgpe_ctx** out);
 int fgpe_ctx_destroy(fgpe_ctx* ctx);
 const char* fgpe_last_error(void);

@@ -346,19 +346,30 @@ void orc_hpsi(const orc_grid *g, const orc_ham *h, double t,
     orc_hpsi_vpx(g, h, t, psi, V, hmu, hen, hen != NULL);
 }
 
-/* The example user pointwise term of the library's user_term.hpp (a
-   modulated contact coupling, three-body loss and a ramp along z):
-   w = u0 sin(u1 t) n + u3 iz - i u2 n^2 / 2, w_en with half the coupling. */
-static void user_example(const orc_ham *h, double t, int iz, double n, double *wr, double *wi, double *er,
-                         double *ei)
+/* The test user pointwise terms of the library's user_term.hpp (not part of
+   the reference: they stand for a plugin author's own kernels).
+   Term 1: a modulated contact coupling, three-body loss and a ramp along z,
+     w = u0 sin(u1 t) n + u3 iz - i u2 n^2 / 2, w_en with half the coupling.
+   Term 2: a decaying lattice felt with opposite signs by the components,
+     w = w_en = sgn_c u4 exp(-u5 t) cos(u6 ix + u7 iy). */
+static void user_term(const orc_ham *h, double t, int ix, int iy, int iz, int c, double n, double *wr, double *wi,
+                      double *er, double *ei)
 {
-    const double dg = h->user[0] * sin(h->user[1] * t);
-    const double loss = -0.5 * h->user[2] * n * n;
-    const double ramp = h->user[3] * iz;
-    *wr = dg * n + ramp;
-    *wi = loss;
-    *er = 0.5 * dg * n + ramp;
-    *ei = loss;
+    *wr = *wi = *er = *ei = 0.0;
+    if (h->user_id == 1) {
+        const double dg = h->user[0] * sin(h->user[1] * t);
+        const double loss = -0.5 * h->user[2] * n * n;
+        const double ramp = h->user[3] * iz;
+        *wr = dg * n + ramp;
+        *wi = loss;
+        *er = 0.5 * dg * n + ramp;
+        *ei = loss;
+    } else if (h->user_id == 2) {
+        const double sgn = c == 0 ? 1.0 : -1.0;
+        const double w = sgn * h->user[4] * exp(-h->user[5] * t) * cos(h->user[6] * ix + h->user[7] * iy);
+        *wr = w;
+        *er = w;
+    }
 }
 
 static void add_user(const orc_grid *g, const orc_ham *h, double t, int nc, const double *psi, double *hmu,
@@ -368,10 +379,12 @@ static void add_user(const orc_grid *g, const orc_ham *h, double t, int nc, cons
     for (int c = 0; c < nc; ++c) {
         const double *p = psi + 2 * N * c;
         for (long long i = 0; i < N; ++i) {
+            const int ix = (int)(i % g->nx);
+            const int iy = (int)((i / g->nx) % g->ny);
             const int iz = (int)(i / ((long long)g->nx * g->ny));
             const double pr = p[2 * i], pi_ = p[2 * i + 1];
             double wr, wi, er, ei;
-            user_example(h, t, iz, pr * pr + pi_ * pi_, &wr, &wi, &er, &ei);
+            user_term(h, t, ix, iy, iz, c, pr * pr + pi_ * pi_, &wr, &wi, &er, &ei);
             double *o = hmu + 2 * N * c;
             o[2 * i] += wr * pr - wi * pi_;
             o[2 * i + 1] += wr * pi_ + wi * pr;
@@ -462,7 +475,7 @@ void orc_hpsi_vpx(const orc_grid *g, const orc_ham *h, double t, const double *p
             }
         }
     }
-    if (h->user_example) add_user(g, h, t, nc, psi, hmu, hen);
+    if (h->user_id) add_user(g, h, t, nc, psi, hmu, hen);
     free(ke);
     free(phi);
 }

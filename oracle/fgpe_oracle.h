@@ -29,8 +29,9 @@ typedef struct orc_ham {
     double quench[6];        /* a0, a1, a2, t1, t2, t3                          */
     const double *vtilde;    /* real k-space kernel Vt(k) (N doubles) or NULL   */
     double vpx[2];           /* moving-frame factor per component (0: off)      */
-    int    user_example;     /* 1: the example user pointwise term              */
-    double user[8];          /* its parameters                                  */
+    int    user_id;          /* test user pointwise term: 0 none, 1 or 2 (the
+                                library's example terms, user_term.hpp)      */
+    double user[8];          /* their parameters                                */
 } orc_ham;
 
 /* ---- building blocks ---------------------------------------------------- */

@@ -35,7 +35,7 @@ class OrcGeom(C.Structure):
 class OrcHam(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("a", C.c_double * 3), ("quench_on", C.c_int),
                 ("quench", C.c_double * 6), ("vtilde", _dp), ("vpx", C.c_double * 2),
-                ("user_example", C.c_int), ("user", C.c_double * 8)]
+                ("user_id", C.c_int), ("user", C.c_double * 8)]
 
 
 def build():
@@ -128,8 +128,9 @@ class Grid:
 
 
 class Ham:
-    def __init__(self, ncomp=1, a=(0.0, 0.0, 0.0), quench=None, vtilde=None, vpx=(0.0, 0.0), user=None):
-        """user: parameters of the example user pointwise term (None: off)."""
+    def __init__(self, ncomp=1, a=(0.0, 0.0, 0.0), quench=None, vtilde=None, vpx=(0.0, 0.0), user=None,
+                 user_id=1):
+        """user: parameters of the test user pointwise term user_id (None: no term)."""
         self.ncomp = ncomp
         self.vtilde = None if vtilde is None else np.ascontiguousarray(vtilde, dtype=np.float64)
         q = (0.0,) * 6 if quench is None else tuple(quench)
@@ -138,7 +139,7 @@ class Ham:
         self._s = OrcHam(ncomp, (C.c_double * 3)(*aa), 0 if quench is None else 1,
                          (C.c_double * 6)(*q),
                          _p(self.vtilde) if self.vtilde is not None else _dp(),
-                         (C.c_double * 2)(*vv), 0 if user is None else 1,
+                         (C.c_double * 2)(*vv), 0 if user is None else int(user_id),
                          (C.c_double * 8)(*(tuple(user or ()) + (0.0,) * (8 - len(user or ())))))
 
 

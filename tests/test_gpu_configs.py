@@ -7,6 +7,8 @@ equations fix independently of size: imaginary time lowers the energy at
 fixed norm, real-time RK4 never raises the norm and, on smooth states,
 conserves norm and energy to its truncation error, and the mixed-precision
 form stays within 1e-6 relative energy of the FP64 form."""
+import os
+
 import numpy as np
 import pytest
 
@@ -14,6 +16,8 @@ from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, FGPE_IMAG_RK4_RENORM, Conte
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def ctx_for(w, prec):
@@ -42,26 +46,39 @@ def rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
+def fixture(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def compare_to_fixture(fx, k, psi, obs, dv, tol=1e-12):
+    """Checkpoint k of an oracle fixture (tests/golden/make_horizon.py):
+    psi on its sampled points against max |psi|, sum |psi| dv, norm, mu, E."""
+    sub = int(fx["sub"])
+    e_psi = float(np.abs(psi[::sub, ::sub, ::sub] - fx["psi_s"][k]).max() / fx["psimax"][k])
+    e_abs = abs(float(np.abs(psi).sum() * dv) - fx["absum"][k]) / fx["absum"][k]
+    e_obs = np.abs(np.asarray(obs[:3]) - fx["obs"][k]) / np.abs(fx["obs"][k])
+    print(f"  step {int(fx['steps'][k])}: psi {e_psi:.2e}, sum|psi| {e_abs:.2e}, norm/mu/E {e_obs}", flush=True)
+    assert e_psi < tol and e_abs < tol and np.all(e_obs < tol), (k, e_psi, e_abs, e_obs)
+
+
 def test_config1_imaginary_rk4_64cubed():
-    """64^3 contact GPE, imaginary-time RK4 renormalised to N = 1, dt = 1e-3:
-    100 steps against the oracle at the full size, then the full 2000 steps
-    lower the energy at every checkpoint with the norm held at 1."""
+    """64^3 contact GPE, imaginary-time RK4 renormalised to N = 1, dt = 1e-3,
+    over the full 2000 steps against the oracle (a fixture the oracle wrote,
+    tests/golden/make_horizon.py): psi, sum |psi|, norm, mu and E within
+    1e-12 at steps 500, 1000, 1500 and 2000, the energy falling between
+    checkpoints at a norm of 1."""
     w = configs.contact_imag_64()
     print(f"config 1: {w.name}, {w.grid.shape}", flush=True)
-    og = oracle_grid(w)
-    orc.set_threads(16)
-    ref = orc.rk4(og, orc.Ham(1, w.a), w.initial_state(), w.trap(), w.dt, 100, imaginary=True, npart=1.0)
+    fx = fixture("config1_64cubed_2000.npz")
+    energies = []
     with ctx_for(w, FGPE_PREC_F64) as c:
-        c.imag_steps(w.dt, 100, scheme=FGPE_IMAG_RK4_RENORM, npart=1.0)
-        assert rel(c.download_psi(), ref) < 1e-12
-        energies = [c.observables()[2]]
-        for _ in range(19):
-            c.imag_steps(w.dt, 100, scheme=FGPE_IMAG_RK4_RENORM, npart=1.0)
-            o = c.observables()
+        for k, step in enumerate(fx["steps"]):
+            c.imag_steps(w.dt, 500, scheme=FGPE_IMAG_RK4_RENORM, npart=1.0)
+            o = c.observables(step * w.dt)
+            compare_to_fixture(fx, k, c.download_psi(), o, w.grid.dv)
             assert abs(o[0] - 1.0) < 1e-12
             energies.append(o[2])
     assert all(e1 < e0 for e0, e1 in zip(energies, energies[1:])), energies
-    assert energies[0] - energies[-1] > 1e-4
 
 
 def test_config2_dipolar_real_time_full_grid():
@@ -91,6 +108,37 @@ def test_config2_dipolar_real_time_full_grid():
     m0, m1, pm = out[FGPE_PREC_MIXED]
     assert abs(m1[2] - o1[2]) < 1e-6 * abs(o1[2])
     assert np.sqrt(np.sum(np.abs(pm - p64) ** 2) / np.sum(np.abs(p64) ** 2)) < 1e-5
+
+
+def test_config3_oracle_steps_256cubed():
+    """Two coupled components at the full 256^3 with the soft-core term on
+    n1 + n2: two RK4 steps against the oracle, FP64 within 1e-12 and the
+    mixed form (the configuration's FP32 bulk) within 1e-6, on both
+    components' psi and on each component's norm and energy."""
+    w = configs.twocomp_softcore_256(256, nsteps=2)
+    print(f"config 3: {w.name}, oracle {w.nsteps} steps", flush=True)
+    og = oracle_grid(w)
+    orc.set_threads(16)
+    vt = orc.vsoftcore(og, 3, w.sc_R, w.sc_U)
+    ham = orc.Ham(2, w.a, vtilde=vt)
+    psi0, V = w.initial_state(), w.trap()
+    ref = orc.rk4(og, ham, psi0, V, w.dt, w.nsteps)
+    ref_obs = []
+    hmu, hen = orc.hpsi(og, ham, w.nsteps * w.dt, ref, V)
+    for comp in range(2):
+        nrm = orc.norm(og, ref[comp])
+        ref_obs.append((nrm,) + orc.obs2(og, nrm, ref[comp], hmu[comp], hen[comp]))
+    del hmu, hen
+    for prec, tol in ((FGPE_PREC_F64, 1e-12), (FGPE_PREC_MIXED, 1e-6)):
+        with ctx_for(w, prec) as c:
+            c.rk4_steps(w.dt, w.nsteps)
+            got = c.download_psi()
+            o = c.observables(w.nsteps * w.dt)
+        for comp in range(2):
+            e_psi = rel(got[comp], ref[comp])
+            e_obs = [abs(o[3 * comp + j] - ref_obs[comp][j]) / abs(ref_obs[comp][j]) for j in (0, 2)]
+            print(f"  precision {prec} component {comp}: psi {e_psi:.2e}, norm / E {e_obs}", flush=True)
+            assert e_psi < tol and max(e_obs) < tol, (prec, comp, e_psi, e_obs)
 
 
 def test_config3_two_component_softcore_256cubed():

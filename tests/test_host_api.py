@@ -118,3 +118,20 @@ def test_soft_lattice_box_redefinition(tmp_path, lattice, fx, fy, fz):
         assert abs(n * step - L) <= 0.5 * step + 1e-12
         assert d["XMAX"][i] == pytest.approx(0.5 * n * step / d["aho"], rel=1e-14)
         assert d["DX"][i] == pytest.approx(2 * d["XMAX"][i] / d["NX"][i], rel=1e-14)
+
+
+def test_plugin_overriding_hamiltonian_does_not_compile(tmp_path):
+    """H is the library's kernel configured by hparams, so a reference plugin
+    that defines its own alg_calcHpsi / alg_calcHpsiMU would lose its terms:
+    the solver classes make such a definition a compile error."""
+    host = os.path.join(ROOT, "fgpe_amd", "host")
+    for solver, base in (("oneComponentGPSolver", "OneComponentGPSolver"),
+                         ("twoComponentGPSolver", "TwoComponentGPSolver")):
+        for member in ("alg_calcHpsi", "alg_calcHpsiMU"):
+            src = tmp_path / f"{member}.cpp"
+            src.write_text(f'#include "hpp/{solver}.hpp"\n'
+                           f"class Mine : public {base} {{\n"
+                           f"  public:\n    void {member}();\n}};\n")
+            r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", host, str(src)],
+                               capture_output=True, text=True)
+            assert r.returncode != 0 and "final" in r.stderr, (solver, member, r.stderr[-500:])

@@ -430,3 +430,26 @@ def test_user_term_ramp_and_energy_halves():
     dg = u[0] * np.sin(u[1] * t) * abs(A0) ** 2
     assert np.abs((h1 - h0) - (dg + u[3] * iz) * psi).max() < 1e-14
     assert np.abs((e1 - e0) - (0.5 * dg + u[3] * iz) * psi).max() < 1e-14
+
+
+def test_user_term_drive_pointwise_and_component_sign():
+    """The second test term on a plane wave (so the kinetic part is not
+    uniform) and two components: H and H_en each gain
+    sgn_c A exp(-gamma t) cos(kx ix + ky iy) psi_c, + for component 0 and -
+    for component 1, with ix, iy the grid indices (x fastest)."""
+    g = grid(6, 4, 3)
+    x = np.arange(g.nx)[None, None, :] * np.ones(g.shape)
+    y = np.arange(g.ny)[None, :, None] * np.ones(g.shape)
+    psi = np.stack([np.exp(2j * np.pi * x / g.nx) * (0.5 + 0.1j), np.exp(-2j * np.pi * y / g.ny) * (0.3 - 0.2j)])
+    V = np.zeros(psi.shape, complex)
+    u = (0.0, 0.0, 0.0, 0.0, 1.4, 0.7, 0.35, 0.9)
+    t = 0.3
+    h0, e0 = orc.hpsi(g, orc.Ham(2, (0.01, 0.02, 0.005)), t, psi, V)
+    h1, e1 = orc.hpsi(g, orc.Ham(2, (0.01, 0.02, 0.005), user=u, user_id=2), t, psi, V)
+    w = u[4] * np.exp(-u[5] * t) * np.cos(u[6] * x + u[7] * y)
+    for c, sgn in ((0, 1.0), (1, -1.0)):
+        assert np.abs((h1[c] - h0[c]) - sgn * w * psi[c]).max() < 1e-14
+        assert np.abs((e1[c] - e0[c]) - sgn * w * psi[c]).max() < 1e-14
+    # a transposed index (iy for ix) would fail: the lattice is not symmetric
+    wt = u[4] * np.exp(-u[5] * t) * np.cos(u[6] * y + u[7] * x)
+    assert np.abs((h1[0] - h0[0]) - wt * psi[0]).max() > 1e-3
