@@ -10,6 +10,12 @@ mixed-precision form is timed the same way and reported beside it with its
 error against the FP64 form.  The roofline object covers the kernel class
 with the largest share of the FP64 step time, from HIP events recorded on
 the stream the kernels run on, over a second timed region of the same length.
+
+The same line carries the metric's second part, adjoint gradients/s
+(`adjoint`: 128^3 dipolar, 2000 RK4 steps, checkpoint every 45, both forms,
+the FP64 gradient checked by central differences along 4 seeded V
+directions, a and a_dd), and the oracle's steps/s on the host
+(`cpu_baseline`: many threads after a warm-up, and one thread).
 """
 from __future__ import annotations
 
@@ -83,29 +89,61 @@ def run_form(torch, w, psi, V, precision, steps, warmup):
                                       "order": order}
 
 
-def cpu_baseline(w, psi, V, nsteps, threads):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(w, psi, V, steps, warmup, threads, steps_1t):
+    """The oracle (oracle/fgpe_oracle.c, its own FFT: no FFTW on these
+    machines) on the host: `warmup` untimed steps, then `steps` timed steps
+    on `threads` OpenMP threads, then `steps_1t` timed steps on one thread,
+    each continuing from the state the previous ones reached."""
     from oracle import oracle as orc
-    orc.set_threads(threads)
     g = w.grid
     kx, ky, kz = g.kaxes()
     og = orc.Grid(g.nx, g.ny, g.nz, kx, ky, kz, g.dv)
     h = orc.Ham(1, w.a, vtilde=orc.vdd(og, w.a_dd, w.dip))
+    orc.set_threads(threads)
+    used = orc.num_threads()
+    t_start = time.perf_counter()
+    t = 0.0
+    if warmup:
+        psi = orc.rk4(og, h, psi, V, w.dt, warmup, t0=t)
+        t += warmup * w.dt
     t0 = time.perf_counter()
-    orc.rk4(og, h, psi, V, w.dt, nsteps)
-    dt = time.perf_counter() - t0
-    return nsteps / dt, orc.num_threads()
+    psi = orc.rk4(og, h, psi, V, w.dt, steps, t0=t)
+    many = steps / (time.perf_counter() - t0)
+    t += steps * w.dt
+    one = None
+    if steps_1t > 0:
+        orc.set_threads(1)
+        t0 = time.perf_counter()
+        psi = orc.rk4(og, h, psi, V, w.dt, steps_1t, t0=t)
+        one = steps_1t / (time.perf_counter() - t0)
+        orc.set_threads(threads)
+    return many, used, one, time.perf_counter() - t_start
 
 
-def run_adjoint(torch, args):
+def adjoint_leg(torch, nsteps, every, ndirs):
     """Gradient of the final-state fidelity |<target|psi(T)>|^2 with respect to
     the trap V, a and a_dd through the discrete adjoint, for the dipolar GPE
     (omega = (1,1,0.25), g = 200, eps_dd = 1.3, box 16 x 16 x 32) on 128^3 over
-    T RK4 steps of dt = 5e-4 with a checkpoint every 45 steps; checked by
-    central differences along --adjoint-dirs seeded smooth V directions
-    (16 by default), a and a_dd."""
-    from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, Context, configs
+    nsteps RK4 steps of dt = 5e-4 with a checkpoint every `every` steps.  One
+    gradient = the checkpointed forward run plus the backward sweep; each
+    form is timed twice and the second (warm) time is kept.  The FP64
+    gradient is checked by central differences of the FP64 forward along
+    `ndirs` of the seeded smooth V directions, a and a_dd; the mixed gradient
+    against the FP64 one (finite differences of an FP32-bulk forward are
+    dominated by its rounding at these step sizes)."""
+    from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, Context, configs, roofline
     torch.cuda.set_device(0)
-    n, dt, every = args.adjoint_steps, 5e-4, 45
+    n, dt = nsteps, 5e-4
     grid = configs.GridSpec(128, 128, 128, 16.0, 16.0, 32.0)
     g = 200.0
     a, a_dd = g / (4 * configs.PI), 1.3 * g / (4 * configs.PI)
@@ -157,25 +195,27 @@ def run_adjoint(torch, args):
             times.append(time.perf_counter() - t0)
             c.close()
         gVn = gV.cpu().numpy()
-        print(f"# adjoint {name}: J = {Jv:.10f}, {times[-1]:.2f} s per gradient", file=sys.stderr, flush=True)
+        bpp = roofline.adjoint_bytes_per_point(prec == FGPE_PREC_MIXED, n, every, 128)
+        gbs = bpp * grid.npoints / times[-1] / 1e9
+        res = {"grads_per_s": round(1.0 / times[-1], 4), "s_per_gradient": round(times[-1], 3), "J": Jv,
+               "bytes_per_point": round(bpp, 1), "GBs": round(gbs, 1),
+               "frac_of_peak": round(gbs / roofline.HBM_PEAK_GBS, 4)}
+        print(f"# adjoint {name}: J = {Jv:.10f}, {times[-1]:.2f} s per gradient, {gbs:.0f} GB/s", file=sys.stderr,
+              flush=True)
         if name == "mixed":
-            # against the FP64 adjoint: finite differences of an FP32-bulk forward are dominated by its
-            # rounding noise at these step sizes
             ref = out["f64"]
-            nd = args.adjoint_dirs
-            cmp = [("V", float(np.sum(gVn * smooth(4 + j))), ref["directional"][j]) for j in range(nd)]
-            cmp += [("a", gs[0], ref["directional"][nd]), ("a_dd", gs[1], ref["directional"][nd + 1])]
+            cmp = [("V", float(np.sum(gVn * smooth(4 + j))), ref["directional"][j]) for j in range(ndirs)]
+            cmp += [("a", gs[0], ref["directional"][ndirs]), ("a_dd", gs[1], ref["directional"][ndirs + 1])]
             checks = [{"dir": lab, "mixed": m, "f64": r, "rel": abs(m - r) / abs(r)} for lab, m, r in cmp]
             for x in checks:
                 print(f"#   {x['dir']}: mixed {x['mixed']:.10e} f64 {x['f64']:.10e} rel {x['rel']:.2e}",
                       file=sys.stderr, flush=True)
-            out[name] = {"grads_per_s": round(1.0 / times[-1], 4), "s_per_gradient": round(times[-1], 3),
-                         "J": Jv, "vs_f64_adjoint": checks, "max_rel_vs_f64": max(x["rel"] for x in checks)}
+            res.update(vs_f64_adjoint=checks, max_rel_vs_f64=max(x["rel"] for x in checks))
+            out[name] = res
             continue
-        # central differences of the same form's forward
         checks = []
         h = 1e-4
-        dirs = [("V", smooth(4 + j), 0.0, 0.0) for j in range(args.adjoint_dirs)] + [
+        dirs = [("V", smooth(4 + j), 0.0, 0.0) for j in range(ndirs)] + [
             ("a", None, h * a, 0.0), ("a_dd", None, 0.0, h * a_dd)]
         for lab, dVx, pa, pd in dirs:
             Js = []
@@ -194,18 +234,26 @@ def run_adjoint(torch, args):
             checks.append({"dir": lab, "adjoint": ad, "fd": fd, "rel": abs(ad - fd) / abs(fd)})
             print(f"#   {lab}: adjoint {ad:.10e} fd {fd:.10e} rel {abs(ad - fd) / abs(fd):.2e}",
                   file=sys.stderr, flush=True)
-        out[name] = {"grads_per_s": round(1.0 / times[-1], 4), "s_per_gradient": round(times[-1], 3), "J": Jv,
-                     "fd_checks": checks, "max_rel_vs_fd": max(x["rel"] for x in checks),
-                     "directional": [x["adjoint"] for x in checks]}
+        res.update(fd_checks=checks, max_rel_vs_fd=max(x["rel"] for x in checks),
+                   directional=[x["adjoint"] for x in checks])
+        out[name] = res
+    return {"workload": f"adjoint 128x128x128 dipolar, T = {n} RK4 steps of dt = {dt}, checkpoint every {every}",
+            "box": [16.0, 16.0, 32.0], "g": g, "eps_dd": 1.3, "omega": list(omega),
+            "objective": "|<target|psi(T)>|^2 wrt V (2M values), a, a_dd",
+            "f64": out["f64"], "mixed": out["mixed"]}
+
+
+def run_adjoint(torch, args):
+    """The adjoint leg alone, as its own JSON line (16 directions by default)."""
+    adj = adjoint_leg(torch, args.adjoint_steps, 45, args.adjoint_dirs)
+    f64 = adj["f64"]
     line = {
         "metric": "adjoint gradients/s (fidelity wrt V, a, a_dd; 128^3 dipolar, 2000 RK4 steps, checkpoint 45)",
-        "value": out["f64"]["grads_per_s"], "unit": "gradients/s", "n_gpus": 1, "steps": n, "warmup": 1,
-        "ms_per_step": round(1e3 * out["f64"]["s_per_gradient"] / n, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "value": f64["grads_per_s"], "unit": "gradients/s", "n_gpus": 1, "steps": args.adjoint_steps,
+        "warmup": 1, "ms_per_step": round(1e3 * f64["s_per_gradient"] / args.adjoint_steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Thomas-Fermi psi0 and target, harmonic trap)",
-        "config": {"workload": f"adjoint 128x128x128 dipolar, T = {n} steps", "dt": dt, "ckpt_every": every,
-                   "box": [16.0, 16.0, 32.0], "g": g, "eps_dd": 1.3, "omega": list(omega)},
-        "f64": out["f64"], "mixed": out["mixed"],
+        "config": {"workload": adj["workload"]}, "f64": f64, "mixed": adj["mixed"],
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -348,8 +396,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", default="256,256,256")
-    ap.add_argument("--cpu-steps", type=int, default=4)   # about 12 s of oracle work on 16 threads
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="timed oracle steps on --cpu-threads threads")
+    ap.add_argument("--cpu-warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps-1t", type=int, default=2, help="timed oracle steps on one thread")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="oracle threads (16 is a GPU box's CPU share; capped at the host's count)")
+    ap.add_argument("--no-adjoint", action="store_true", help="leave the adjoint leg out of the default line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
                     help="run the batched ground-state sweep (64 mixed-shape dipolar / soft-core "
@@ -358,8 +410,9 @@ def main():
                     help="time adjoint gradients (128^3 dipolar, 2000 RK4 steps, checkpoint every 45) "
                          "in both forms, with central finite-difference checks")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
-    ap.add_argument("--adjoint-dirs", type=int, default=16,
-                    help="seeded smooth V directions checked by central differences")
+    ap.add_argument("--adjoint-dirs", type=int, default=0,
+                    help="seeded smooth V directions checked by central differences (default: 4 in the "
+                         "default line, 16 with --adjoint)")
     ap.add_argument("--init", action="store_true",
                     help="time the device initialisers (state and trap) at --grid instead")
     ap.add_argument("--sweep-n", type=int, default=64)
@@ -387,6 +440,7 @@ def main():
     if args.sweep:
         return run_sweep(torch, args)
     if args.adjoint:
+        args.adjoint_dirs = args.adjoint_dirs or 16
         return run_adjoint(torch, args)
     if args.init:
         return run_init(torch, args)
@@ -463,14 +517,22 @@ def main():
                      "bytes_per_point": per_pt, "avg_launch_ms": round(avg_ms, 4)},
         "cpu_baseline": None,
     }
+    if not args.no_adjoint:
+        t0 = time.perf_counter()
+        line["adjoint"] = adjoint_leg(torch, args.adjoint_steps, 45, args.adjoint_dirs or 4)
+        line["adjoint"]["leg_wall_s"] = round(time.perf_counter() - t0, 1)
     if not args.no_cpu and args.cpu_steps > 0:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
-        sps, used = cpu_baseline(w, psi, V, args.cpu_steps, threads)
-        line["cpu_baseline"] = {"value": round(sps, 5), "unit": "steps/s", "cores": used,
-                                "kind": "port",
-                                "sample": f"{args.cpu_steps} FP64 RK4 steps of the same {nx}x{ny}x{nz} "
-                                          f"dipolar workload, oracle/fgpe_oracle.c (own FFT), "
-                                          f"{used} OpenMP threads"}
+        sps, used, sps1, wall = cpu_baseline(w, psi, V, args.cpu_steps, args.cpu_warmup, threads, args.cpu_steps_1t)
+        line["cpu_baseline"] = {
+            "value": round(sps, 5), "unit": "steps/s", "cores": used, "kind": "port",
+            "sample": f"{args.cpu_steps} FP64 RK4 steps (after {args.cpu_warmup} untimed) of the same "
+                      f"{nx}x{ny}x{nz} dipolar workload, oracle/fgpe_oracle.c (its own FFT; no FFTW here), "
+                      f"{used} OpenMP threads",
+            "one_thread_steps_per_s": round(sps1, 5) if sps1 else None,
+            "one_thread_steps": args.cpu_steps_1t, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "wall_s": round(wall, 1),
+            "gpu_f64_speedup": round(steps_s / sps, 1)}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -34,7 +34,8 @@ EXPORTS = (
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
     "fgpe_psi_init", "fgpe_psi_add_noise", "fgpe_psi_imprint_vortex_tube", "fgpe_psi_force_lattice",
     "fgpe_psi_normalize", "fgpe_vext_init", "fgpe_vext_init_box", "fgpe_vext_add_cos", "fgpe_vext_commit",
-    "fgpe_user_term", "fgpe_user_term_ids", "fgpe_imag_steps_noalloc",
+    "fgpe_user_term", "fgpe_user_term_ids", "fgpe_imag_steps_noalloc", "fgpe_upload_psi_dev",
+    "fgpe_download_psi_dev", "fgpe_upload_vext_real_dev",
 )
 # initialiser kinds and lattices (fgpe_psi_init, fgpe_vext_init, fgpe_psi_force_lattice)
 INIT_HARMONIC, INIT_TUBE, INIT_VOID = 0, 1, 2
@@ -102,6 +103,9 @@ def load() -> C.CDLL:
     L.fgpe_download_psi.argtypes = [_vp, C.c_int, _dp]
     L.fgpe_upload_vext.argtypes = [_vp, C.c_int, _dp]
     L.fgpe_psi_device_ptr.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
+    L.fgpe_upload_psi_dev.argtypes = [_vp, C.c_int, _vp]
+    L.fgpe_download_psi_dev.argtypes = [_vp, C.c_int, _vp]
+    L.fgpe_upload_vext_real_dev.argtypes = [_vp, C.c_int, _vp]
     L.fgpe_fft3d_c2c_f64.argtypes = [_vp, _vp, _vp, C.c_int]
     L.fgpe_fft3d_c2c_f32.argtypes = [_vp, _vp, _vp, C.c_int]
     L.fgpe_laplacian_f64.argtypes = [_vp, _vp, _vp]
@@ -289,6 +293,17 @@ class Context:
         out = np.empty(self.N, np.complex128) if download else None
         check(load().fgpe_vext_commit(self.h, comp, _dptr(out) if download else None))
         return out
+
+    def upload_psi_dev(self, dev_psi: int, comp=0):
+        """psi from device memory (N complex doubles), on the context's stream."""
+        check(load().fgpe_upload_psi_dev(self.h, comp, dev_psi))
+
+    def download_psi_dev(self, dev_out: int, comp=0):
+        check(load().fgpe_download_psi_dev(self.h, comp, dev_out))
+
+    def upload_vext_real_dev(self, dev_v: int, comp=0):
+        """A real FP64 trap from device memory (N doubles)."""
+        check(load().fgpe_upload_vext_real_dev(self.h, comp, dev_v))
 
     def psi_device_ptr(self, comp=0) -> int:
         p = _vp()

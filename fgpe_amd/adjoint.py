@@ -25,27 +25,51 @@ from ._lib import FGPE_PREC_F64, Context
 
 
 class AdjointSolver:
-    """One device context reused by every forward/backward of GPEEvolve."""
+    """One device context reused by every forward/backward of GPEEvolve.
 
-    def __init__(self, grid, dt, nsteps, ckpt_every=None, dip=(0.0, 0.0, 1.0), precision=FGPE_PREC_F64):
+    With a torch module given, the context runs on torch's current stream, so
+    device tensors pass in and out without host copies or extra
+    synchronisation."""
+
+    def __init__(self, grid, dt, nsteps, ckpt_every=None, dip=(0.0, 0.0, 1.0), precision=FGPE_PREC_F64,
+                 torch=None):
         self.grid = grid
         self.dt = float(dt)
         self.nsteps = int(nsteps)
         self.every = int(ckpt_every or max(1, round(math.sqrt(nsteps))))
         self.dip = tuple(dip)
         self.ctx = Context(grid.nx, grid.ny, grid.nz, precision)
+        if torch is not None:
+            self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
         kx, ky, kz = grid.kaxes()
         self.ctx.set_kspace(kx, ky, kz, grid.dv)
 
     def configure(self, V, a, a_dd, psi0):
+        """V, psi0 as host arrays."""
         self.ctx.set_hamiltonian(a=(float(a),), a_dd=float(a_dd), dip=self.dip)
         self.ctx.upload_vext(np.asarray(V, dtype=np.float64).astype(np.complex128))
         self.ctx.upload_psi(np.asarray(psi0, dtype=np.complex128))
 
     def forward(self, V, a, a_dd, psi0):
+        """Host arrays in, psi(T) as a host array."""
         self.configure(V, a, a_dd, psi0)
         self.ctx.adjoint_forward(self.dt, self.nsteps, self.every)
         return self.ctx.download_psi()
+
+    def forward_dev(self, torch, V, a, a_dd, psi0):
+        """Device tensors in (V float64, psi0 complex128, contiguous, on the
+        context's device), psi(T) as a new device tensor."""
+        V = V.detach().to(torch.float64).contiguous()
+        psi0 = psi0.detach().to(torch.complex128).contiguous()
+        torch.cuda.synchronize()   # the tensors' producers may run on another stream
+        self.ctx.set_hamiltonian(a=(float(a),), a_dd=float(a_dd), dip=self.dip)
+        self.ctx.upload_vext_real_dev(V.data_ptr())
+        self.ctx.upload_psi_dev(psi0.data_ptr())
+        self.ctx.adjoint_forward(self.dt, self.nsteps, self.every)
+        out = torch.empty_like(psi0)
+        self.ctx.download_psi_dev(out.data_ptr())
+        self.ctx.sync()   # V and psi0 may be freed once this returns
+        return out
 
     def backward(self, torch, lamT):
         """lamT: torch complex128 tensor on the device (dL/dpsi(T))."""
@@ -64,19 +88,15 @@ def _autograd():
     class GPEEvolve(torch.autograd.Function):
         @staticmethod
         def forward(fctx, V, a, a_dd, psi0, solver):
-            psiT = solver.forward(V.detach().cpu().numpy(), float(a), float(a_dd),
-                                  psi0.detach().cpu().numpy())
             fctx.solver = solver
-            fctx.dev = V.device
-            return torch.from_numpy(psiT).to(psi0.device)
+            fctx.vdev, fctx.pdev = V.device, psi0.device
+            return solver.forward_dev(torch, V.to("cuda"), float(a), float(a_dd), psi0.to("cuda")).to(psi0.device)
 
         @staticmethod
         def backward(fctx, grad_psiT):
-            solver = fctx.solver
-            lam = grad_psiT.to("cuda")
-            gV, ga, gadd, lam0 = solver.backward(torch, lam)
-            return (gV.to(fctx.dev), torch.tensor(ga, dtype=torch.float64),
-                    torch.tensor(gadd, dtype=torch.float64), lam0.to(fctx.dev), None)
+            gV, ga, gadd, lam0 = fctx.solver.backward(torch, grad_psiT.to("cuda"))
+            return (gV.to(fctx.vdev), torch.tensor(ga, dtype=torch.float64), torch.tensor(gadd, dtype=torch.float64),
+                    lam0.to(fctx.pdev), None)
 
     return GPEEvolve
 

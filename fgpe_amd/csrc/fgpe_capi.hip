@@ -1335,6 +1335,46 @@ int fgpe_upload_vext(fgpe_ctx* c, int comp, const double* host)
     return FGPE_OK;
 }
 
+int fgpe_upload_psi_dev(fgpe_ctx* c, int comp, const void* dev)
+{
+    if (!c || !dev) return fail(FGPE_EINVAL, "NULL argument");
+    if (comp < 0 || comp > 1) return fail(FGPE_EINVAL, "comp must be 0 or 1");
+    if (!c->P0[comp]) FGPE_RET(dalloc(c, (void**)&c->P0[comp], sizeof(double2) * c->ntot));
+    FGPE_HIPCHK(hipMemcpyAsync(c->P0[comp], dev, sizeof(double2) * c->ntot, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_reset_scales, dim3(1), dim3(1), 0, c->stream, c->scal + SC_SCALE);
+    FGPE_LAUNCHCHK("k_reset_scales");
+    c->spec_valid = false;
+    c->hb_active = false;
+    c->imag_iter = 0;
+    return FGPE_OK;
+}
+
+int fgpe_download_psi_dev(fgpe_ctx* c, int comp, void* dev)
+{
+    if (!c || !dev) return fail(FGPE_EINVAL, "NULL argument");
+    if (comp < 0 || comp > 1 || !c->P0[comp]) return fail(FGPE_EINVAL, "no such component");
+    FGPE_HIPCHK(hipMemcpyAsync(dev, c->P0[comp], sizeof(double2) * c->ntot, hipMemcpyDeviceToDevice, c->stream));
+    return FGPE_OK;
+}
+
+int fgpe_upload_vext_real_dev(fgpe_ctx* c, int comp, const double* dev)
+{
+    if (!c || !dev) return fail(FGPE_EINVAL, "NULL argument");
+    if (comp < 0 || comp > 1) return fail(FGPE_EINVAL, "comp must be 0 or 1");
+    const size_t N = (size_t)c->ntot;
+    if (!c->Vr[comp]) FGPE_RET(dalloc(c, &c->Vr[comp], c->tsz() * N));
+    if (c->mixed()) {
+        const long long n = c->ntot;
+        hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           (float*)c->Vr[comp], dev, n);
+        FGPE_LAUNCHCHK("k_convert");
+    } else {
+        FGPE_HIPCHK(hipMemcpyAsync(c->Vr[comp], dev, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->vcomplex[comp] = false;
+    return FGPE_OK;
+}
+
 int fgpe_psi_device_ptr(fgpe_ctx* c, int comp, void** p)
 {
     if (!c || !p) return fail(FGPE_EINVAL, "NULL argument");

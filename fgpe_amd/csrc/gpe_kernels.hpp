@@ -666,8 +666,9 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
     // row group of this block (reversed: the kernel starts on what the passes
     // before it wrote last); partial sums stay indexed by the row group.  The
     // last group may hold pencils past the last row (grids with fewer rows
-    // than a block's pencils, e.g. 1D): they work on row 0 again, store
-    // nothing and add nothing to the sums.
+    // than a block's pencils, e.g. 1D): their threads skip the pencil work
+    // (the x exchanges are wave-scoped, so idle lanes are harmless) and only
+    // take part in the block's sum with zeros.
     const unsigned bx = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
     const long long rowi = (long long)bx * P + p;
     const bool pv = rowi < a.ntot / N;
@@ -676,287 +677,279 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
     T2* my = lds + p * lds_stride<N>();
     const gp<T2> specd = glob(a.specd) + row * a.dpitch;   // this row's half spectrum
 
-    T2 v[E];
-    T nd[E];             // density of the new state (DENS)
+    double sums[NV];
 #pragma unroll
-    for (int q = 0; q < E; ++q) nd[q] = (T)0;
+    for (int i = 0; i < NV; ++i) sums[i] = 0.0;
 
-    if (MODE == MODE_PROLOGUE) {
+    if (pv) {
+        T2 v[E];
+        T nd[E];             // density of the new state (DENS)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
+        for (int q = 0; q < E; ++q) nd[q] = (T)0;
+
+        if (MODE == MODE_PROLOGUE) {
 #pragma unroll
-            for (int q = 0; q < E; ++q) {
-                v[q] = from_d2<T>(P0[c][row0 + elemA<N, V>(t, q)]);
-                if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
-            }
-            fft_forward<T2, N, V, true>(v, t, my, tw);
-            const gp<T2> sc = spec + c * a.ntot + row0;
-            if (pv)
+            for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    v[q] = from_d2<T>(P0[c][row0 + elemA<N, V>(t, q)]);
+                    if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
+                }
+                fft_forward<T2, N, V, true>(v, t, my, tw);
+                const gp<T2> sc = spec + c * a.ntot + row0;
 #pragma unroll
                 for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
-        }
-    } else {
-        // phi, the k-space potential of the density, is held from the start
-        // of the stage to the pointwise update, across the inverse transform
-        // of psi.  Its first QL register slots live in LDS instead: in
-        // registers it costs the FP64 kernel 66 VGPRs (202 -> 136 with all of
-        // it in LDS), but all of it in LDS makes the FP64 block too large for
-        // three blocks per CU, so the FP64 form keeps half of it in registers.
-        constexpr int QL = sizeof(T) == 8 ? E / 2 : E;
-        __shared__ T sphi[DENS ? P * TT * QL : 1];
-        T phi[E];
-        auto phi_at = [&](int q) -> T& { return q < QL ? sphi[q * (P * TT) + threadIdx.x] : phi[q]; };
-        if (DENS) {
-            // kx > N/2 from the conjugate symmetry of a real row's transform
-#pragma unroll
-            for (int q = 0; q < E; ++q) {
-                const int e = elemB<N, V>(t, q);
-                if (e <= N / 2) {
-                    v[q] = specd[e];
-                } else {
-                    const T2 w = specd[N - e];
-                    v[q] = T2{w.x, -w.y};
-                }
             }
-            fft_inverse<T2, N, V, true>(v, t, my, tw);
+        } else {
+            // phi, the k-space potential of the density, is held from the start
+            // of the stage to the pointwise update, across the inverse transform
+            // of psi.  Its first QL register slots live in LDS instead: in
+            // registers it costs the FP64 kernel 66 VGPRs (202 -> 136 with all of
+            // it in LDS), but all of it in LDS makes the FP64 block too large for
+            // three blocks per CU, so the FP64 form keeps half of it in registers.
+            constexpr int QL = sizeof(T) == 8 ? E / 2 : E;
+            __shared__ T sphi[DENS ? P * TT * QL : 1];
+            T phi[E];
+            auto phi_at = [&](int q) -> T& { return q < QL ? sphi[q * (P * TT) + threadIdx.x] : phi[q]; };
+            if (DENS) {
+                // kx > N/2 from the conjugate symmetry of a real row's transform
 #pragma unroll
-            for (int q = 0; q < E; ++q) phi_at(q) = v[q].x;
-        }
-        double nsave[E];     // NC == 2: density of component 0's stage input
-        double sums[NV];
+                for (int q = 0; q < E; ++q) {
+                    const int e = elemB<N, V>(t, q);
+                    if (e <= N / 2) {
+                        v[q] = specd[e];
+                    } else {
+                        const T2 w = specd[N - e];
+                        v[q] = T2{w.x, -w.y};
+                    }
+                }
+                fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-        for (int i = 0; i < NV; ++i) sums[i] = 0.0;
-        const bool from_p0 = (MODE == MODE_APPLY || MODE == MODE_HB || a.stage == 1);
+                for (int q = 0; q < E; ++q) phi_at(q) = v[q].x;
+            }
+            double nsave[E];     // NC == 2: density of component 0's stage input
+            const bool from_p0 = (MODE == MODE_APPLY || MODE == MODE_HB || a.stage == 1);
 
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const gp<T2> sc = spec + c * a.ntot + row0;
+            for (int c = 0; c < NC; ++c) {
+                const gp<T2> sc = spec + c * a.ntot + row0;
 #pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = sc[elemB<N, V>(t, q)];
-            fft_inverse<T2, N, V, true>(v, t, my, tw);   // v = KE psi_c (layout A)
-            const double sig = a.scale ? glob(a.scale)[c] : 1.0;
-            const double sig1 = (NC == 2 && a.scale) ? glob(a.scale)[1] : 1.0;
-            const double g4s = a.g4[c][c], g2s = a.g2[c][c];
-            const double g4o = a.g4[c][1 - c], g2o = a.g2[c][1 - c];
-            if constexpr (!(MODE == MODE_RK && sizeof(T) == 4)) {
-    #pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const long long g = row0 + elemA<N, V>(t, q);
-                    double2 po = double2{0.0, 0.0};
-                    double2 ps;
-                    if (from_p0) {
-                        const double2 r = ldn<NT>(P0[c], g);
-                        po = double2{sig * r.x, sig * r.y};
-                        ps = po;
-                    } else {
-                        ps = to_d2<T>(ldn<NT>(PS[c], g));
-                    }
-                    const double n = ps.x * ps.x + ps.y * ps.y;
-                    double no = 0.0;
-                    if (NC == 2) {
-                        if (c == 0) {
-                            double2 o;
-                            if (from_p0) {
-                                const double2 r = ldn<NT>(P0[1], g);
-                                o = double2{sig1 * r.x, sig1 * r.y};
-                            } else {
-                                o = to_d2<T>(ldn<NT>(PS[1], g));
-                            }
-                            no = o.x * o.x + o.y * o.y;
-                            nsave[q] = n;
-                        } else {
-                            no = nsave[q];
-                        }
-                    }
-                    const double vr = (double)ldn<NT>(Vr[c], g);
-                    const double vi = Vi[c] ? (double)ldn<NT>(Vi[c], g) : 0.0;
-                    const double2 ke = to_d2<T>(v[q]);
-                    double rX = (vr + g4s * n);
-                    if (NC == 2) rX = rX + g4o * no;
-                    if (DENS) rX += (double)phi_at(q);
-                    double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
-                    double2 he = double2{0.0, 0.0};
-                    if (MODE == MODE_APPLY || MODE == MODE_HB) {
-                        double eX = (vr + g2s * n);
-                        if (NC == 2) eX = eX + g2o * no;
-                        if (DENS) eX += 0.5 * (double)phi_at(q);
-                        he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
-                    }
-                    if constexpr (UTerm::enabled) {
-                        const UserTermOut w = UTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
-                        h = double2{h.x + (w.wr * ps.x - w.wi * ps.y), h.y + (w.wr * ps.y + w.wi * ps.x)};
-                        if (MODE == MODE_APPLY || MODE == MODE_HB)
-                            he = double2{he.x + (w.er * ps.x - w.ei * ps.y), he.y + (w.er * ps.y + w.ei * ps.x)};
-                    }
-                    if (MODE == MODE_APPLY) {
-                        if (a.hmu && pv) glob(a.hmu)[c * a.ntot + g] = h;
-                        if (a.hen && pv) glob(a.hen)[c * a.ntot + g] = he;
-                        sums[3 * c + 0] += n;
-                        sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
-                        sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
-                    } else if (MODE == MODE_HB) {
-                        const double2 old = to_d2<T>(ldn<NT>(PS[c], g));
-                        const double m = a.mu ? glob(a.mu)[c] : 0.0;
-                        const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
-                                                   (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
-                        if (pv) {
-                            stn<NT>(PS[c], g, from_d2<T>(ps));
-                            stn<NT>(P0[c], g, nw);
-                        }
-                        sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
-                        sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
-                        sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
-                        v[q] = from_d2<T>(nw);
-                        if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
-                    } else {
-                        if (a.stage != 1) {
+                for (int q = 0; q < E; ++q) v[q] = sc[elemB<N, V>(t, q)];
+                fft_inverse<T2, N, V, true>(v, t, my, tw);   // v = KE psi_c (layout A)
+                const double sig = a.scale ? glob(a.scale)[c] : 1.0;
+                const double sig1 = (NC == 2 && a.scale) ? glob(a.scale)[1] : 1.0;
+                const double g4s = a.g4[c][c], g2s = a.g2[c][c];
+                const double g4o = a.g4[c][1 - c], g2o = a.g2[c][1 - c];
+                if constexpr (!(MODE == MODE_RK && sizeof(T) == 4)) {
+        #pragma unroll
+                    for (int q = 0; q < E; ++q) {
+                        const long long g = row0 + elemA<N, V>(t, q);
+                        double2 po = double2{0.0, 0.0};
+                        double2 ps;
+                        if (from_p0) {
                             const double2 r = ldn<NT>(P0[c], g);
                             po = double2{sig * r.x, sig * r.y};
+                            ps = po;
+                        } else {
+                            ps = to_d2<T>(ldn<NT>(PS[c], g));
+                        }
+                        const double n = ps.x * ps.x + ps.y * ps.y;
+                        double no = 0.0;
+                        if (NC == 2) {
+                            if (c == 0) {
+                                double2 o;
+                                if (from_p0) {
+                                    const double2 r = ldn<NT>(P0[1], g);
+                                    o = double2{sig1 * r.x, sig1 * r.y};
+                                } else {
+                                    o = to_d2<T>(ldn<NT>(PS[1], g));
+                                }
+                                no = o.x * o.x + o.y * o.y;
+                                nsave[q] = n;
+                            } else {
+                                no = nsave[q];
+                            }
+                        }
+                        const double vr = (double)ldn<NT>(Vr[c], g);
+                        const double vi = Vi[c] ? (double)ldn<NT>(Vi[c], g) : 0.0;
+                        const double2 ke = to_d2<T>(v[q]);
+                        double rX = (vr + g4s * n);
+                        if (NC == 2) rX = rX + g4o * no;
+                        if (DENS) rX += (double)phi_at(q);
+                        double2 h = double2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                        double2 he = double2{0.0, 0.0};
+                        if (MODE == MODE_APPLY || MODE == MODE_HB) {
+                            double eX = (vr + g2s * n);
+                            if (NC == 2) eX = eX + g2o * no;
+                            if (DENS) eX += 0.5 * (double)phi_at(q);
+                            he = double2{ke.x + eX * ps.x - vi * ps.y, ke.y + eX * ps.y + vi * ps.x};
+                        }
+                        if constexpr (UTerm::enabled) {
+                            const UserTermOut w = UTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
+                            h = double2{h.x + (w.wr * ps.x - w.wi * ps.y), h.y + (w.wr * ps.y + w.wi * ps.x)};
+                            if (MODE == MODE_APPLY || MODE == MODE_HB)
+                                he = double2{he.x + (w.er * ps.x - w.ei * ps.y), he.y + (w.er * ps.y + w.ei * ps.x)};
+                        }
+                        if (MODE == MODE_APPLY) {
+                            if (a.hmu) glob(a.hmu)[c * a.ntot + g] = h;
+                            if (a.hen) glob(a.hen)[c * a.ntot + g] = he;
+                            sums[3 * c + 0] += n;
+                            sums[3 * c + 1] += ps.x * h.x + ps.y * h.y;
+                            sums[3 * c + 2] += ps.x * he.x + ps.y * he.y;
+                        } else if (MODE == MODE_HB) {
+                            const double2 old = to_d2<T>(ldn<NT>(PS[c], g));
+                            const double m = a.mu ? glob(a.mu)[c] : 0.0;
+                            const double2 nw = double2{(1.0 + m * a.dt) * ps.x - a.dt * h.x + a.beta * (ps.x - old.x),
+                                                       (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
+                            stn<NT>(PS[c], g, from_d2<T>(ps));
+                            stn<NT>(P0[c], g, nw);
+                            sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
+                            sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
+                            sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
+                            v[q] = from_d2<T>(nw);
+                            if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
+                        } else {
+                            if (a.stage != 1) {
+                                const double2 r = ldn<NT>(P0[c], g);
+                                po = double2{sig * r.x, sig * r.y};
+                            }
+                            if (a.stage < 4) {
+                                // UpdateRKStep: psi = old + c*h ; k = h (stage 1) or k + 2h
+                                const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
+                                                           po.y + (a.cr * h.y + a.ci * h.x)};
+                                if (a.stage == 1) {
+                                    stn<NT>(K[c], g, from_d2<T>(h));
+                                } else {
+                                    const double2 k = to_d2<T>(ldn<NT>(K[c], g));
+                                    stn<NT>(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
+                                }
+                                const T2 st = from_d2<T>(nw);
+                                stn<NT>(PSo[c] ? PSo[c] : PS[c], g, st);
+                                v[q] = st;
+                            } else {
+                                // FinalRKStep: psi = old + c*(k + h)/6
+                                const double2 k = to_d2<T>(ldn<NT>(K[c], g));
+                                const double aX = k.x + h.x, aY = k.y + h.y;
+                                const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
+                                                           po.y + (a.cr * aY + a.ci * aX) / 6.0};
+                                stn<NT>(P0[c], g, nw);
+                                v[q] = from_d2<T>(nw);
+                                if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
+                            }
+                            if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
+                        }
+                    }
+                } else {
+                    // Mixed precision RK stage: H, the stage state and the K
+                    // accumulator in FP32; only stage 4 adds its FP32 increment
+                    // c (K + h) / 6 to the FP64 state.
+                    const float f4s = (float)g4s, f4o = (float)g4o;
+                    const float fcr = (float)a.cr, fci = (float)a.ci;
+#pragma unroll
+                    for (int q = 0; q < E; ++q) {
+                        const long long g = row0 + elemA<N, V>(t, q);
+                        float2 ps;
+                        if (a.stage == 1) {
+                            const double2 r = ldn<NT>(P0[c], g);
+                            ps = float2{(float)(sig * r.x), (float)(sig * r.y)};
+                        } else {
+                            ps = ldn<NT>(PS[c], g);
+                        }
+                        const float n = ps.x * ps.x + ps.y * ps.y;
+                        float no = 0.f;
+                        if (NC == 2) {
+                            if (c == 0) {
+                                float2 o;
+                                if (a.stage == 1) {
+                                    const double2 r = ldn<NT>(P0[1], g);
+                                    o = float2{(float)(sig1 * r.x), (float)(sig1 * r.y)};
+                                } else {
+                                    o = ldn<NT>(PS[1], g);
+                                }
+                                no = o.x * o.x + o.y * o.y;
+                                nsave[q] = n;
+                            } else {
+                                no = (float)nsave[q];
+                            }
+                        }
+                        const float vr = (float)ldn<NT>(Vr[c], g);
+                        const float vi = Vi[c] ? (float)ldn<NT>(Vi[c], g) : 0.f;
+                        const float2 ke = v[q];
+                        float rX = vr + f4s * n;
+                        if (NC == 2) rX = rX + f4o * no;
+                        if (DENS) rX += (float)phi_at(q);
+                        float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
+                        if constexpr (UTerm::enabled) {
+                            const UserTermOut w = UTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
+                            h = float2{h.x + (float)(w.wr * ps.x - w.wi * ps.y), h.y + (float)(w.wr * ps.y + w.wi * ps.x)};
                         }
                         if (a.stage < 4) {
-                            // UpdateRKStep: psi = old + c*h ; k = h (stage 1) or k + 2h
-                            const double2 nw = double2{po.x + (a.cr * h.x - a.ci * h.y),
-                                                       po.y + (a.cr * h.y + a.ci * h.x)};
-                            if (a.stage == 1) {
-                                if (pv) stn<NT>(K[c], g, from_d2<T>(h));
-                            } else {
-                                const double2 k = to_d2<T>(ldn<NT>(K[c], g));
-                                if (pv) stn<NT>(K[c], g, from_d2<T>(double2{k.x + 2.0 * h.x, k.y + 2.0 * h.y}));
+                            float2 po = ps;   // stage 1: the (rounded) state itself
+                            float2 k = float2{0.f, 0.f};
+                            if (a.stage != 1) k = ldn<NT>(K[c], g);
+                            if (a.stage == 2) {
+                                // The state from the stage-1 output and its increment: PS = psi_n + c h1
+                                // and K = h1, and stage 2 has the same c, so the FP64 state is not read
+                                // (it would only be rounded to FP32 here).
+                                po = float2{ps.x - (fcr * k.x - fci * k.y), ps.y - (fcr * k.y + fci * k.x)};
+                            } else if (a.stage == 3) {
+                                if (PS1[c]) {
+                                    // With PS1 = psi_n + c2 h1, PS = psi_n + c2 h2, K = h1 + 2 h2 and
+                                    // c2 = c3 / 2: psi_n = (PS1 + 2 PS - c2 K) / 3, again without
+                                    // reading the FP64 state.  The division by 3 must not be biased:
+                                    // the product with the FP32 value of 1/3 alone (3e-8 high) would
+                                    // scale every stage-3 state by that bias, which the steps
+                                    // accumulate (2e-6 in energy after 2000 steps at 256^3, against
+                                    // 3e-8), so the product is corrected by its exact residual
+                                    // r = s - 3 q (one FMA), which leaves the quotient within about
+                                    // half an ulp of s / 3.
+                                    const float2 p1 = ldn<NT>(PS1[c], g);
+                                    const float hr = 0.5f * fcr, hi = 0.5f * fci;
+                                    po = float2{div3(p1.x + 2.f * ps.x - (hr * k.x - hi * k.y)),
+                                                div3(p1.y + 2.f * ps.y - (hr * k.y + hi * k.x))};
+                                } else {
+                                    const double2 r = ldn<NT>(P0[c], g);
+                                    po = float2{(float)(sig * r.x), (float)(sig * r.y)};
+                                }
                             }
-                            const T2 st = from_d2<T>(nw);
-                            if (pv) stn<NT>(PSo[c] ? PSo[c] : PS[c], g, st);
-                            v[q] = st;
+                            const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
+                            if (a.stage == 1) stn<NT>(K[c], g, h);
+                            else stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
+                            stn<NT>(PSo[c] ? PSo[c] : PS[c], g, nw);
+                            v[q] = nw;
                         } else {
-                            // FinalRKStep: psi = old + c*(k + h)/6
-                            const double2 k = to_d2<T>(ldn<NT>(K[c], g));
-                            const double aX = k.x + h.x, aY = k.y + h.y;
-                            const double2 nw = double2{po.x + (a.cr * aX - a.ci * aY) / 6.0,
-                                                       po.y + (a.cr * aY + a.ci * aX) / 6.0};
-                            if (pv) stn<NT>(P0[c], g, nw);
+                            const float2 k = ldn<NT>(K[c], g);
+                            const float ax = k.x + h.x, ay = k.y + h.y;
+                            const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
+                            const double2 r = ldn<NT>(P0[c], g);
+                            const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
+                            stn<NT>(P0[c], g, nw);
                             v[q] = from_d2<T>(nw);
                             if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
                         }
                         if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
                     }
                 }
-            } else {
-                // Mixed precision RK stage: H, the stage state and the K
-                // accumulator in FP32; only stage 4 adds its FP32 increment
-                // c (K + h) / 6 to the FP64 state.
-                const float f4s = (float)g4s, f4o = (float)g4o;
-                const float fcr = (float)a.cr, fci = (float)a.ci;
-#pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const long long g = row0 + elemA<N, V>(t, q);
-                    float2 ps;
-                    if (a.stage == 1) {
-                        const double2 r = ldn<NT>(P0[c], g);
-                        ps = float2{(float)(sig * r.x), (float)(sig * r.y)};
-                    } else {
-                        ps = ldn<NT>(PS[c], g);
-                    }
-                    const float n = ps.x * ps.x + ps.y * ps.y;
-                    float no = 0.f;
-                    if (NC == 2) {
-                        if (c == 0) {
-                            float2 o;
-                            if (a.stage == 1) {
-                                const double2 r = ldn<NT>(P0[1], g);
-                                o = float2{(float)(sig1 * r.x), (float)(sig1 * r.y)};
-                            } else {
-                                o = ldn<NT>(PS[1], g);
-                            }
-                            no = o.x * o.x + o.y * o.y;
-                            nsave[q] = n;
-                        } else {
-                            no = (float)nsave[q];
-                        }
-                    }
-                    const float vr = (float)ldn<NT>(Vr[c], g);
-                    const float vi = Vi[c] ? (float)ldn<NT>(Vi[c], g) : 0.f;
-                    const float2 ke = v[q];
-                    float rX = vr + f4s * n;
-                    if (NC == 2) rX = rX + f4o * no;
-                    if (DENS) rX += (float)phi_at(q);
-                    float2 h = float2{ke.x + rX * ps.x - vi * ps.y, ke.y + rX * ps.y + vi * ps.x};
-                    if constexpr (UTerm::enabled) {
-                        const UserTermOut w = UTerm::eval(user_in<N>(a, g, c, ps.x, ps.y, n, no));
-                        h = float2{h.x + (float)(w.wr * ps.x - w.wi * ps.y), h.y + (float)(w.wr * ps.y + w.wi * ps.x)};
-                    }
-                    if (a.stage < 4) {
-                        float2 po = ps;   // stage 1: the (rounded) state itself
-                        float2 k = float2{0.f, 0.f};
-                        if (a.stage != 1) k = ldn<NT>(K[c], g);
-                        if (a.stage == 2) {
-                            // The state from the stage-1 output and its increment: PS = psi_n + c h1
-                            // and K = h1, and stage 2 has the same c, so the FP64 state is not read
-                            // (it would only be rounded to FP32 here).
-                            po = float2{ps.x - (fcr * k.x - fci * k.y), ps.y - (fcr * k.y + fci * k.x)};
-                        } else if (a.stage == 3) {
-                            if (PS1[c]) {
-                                // With PS1 = psi_n + c2 h1, PS = psi_n + c2 h2, K = h1 + 2 h2 and
-                                // c2 = c3 / 2: psi_n = (PS1 + 2 PS - c2 K) / 3, again without
-                                // reading the FP64 state.  The division by 3 must not be biased:
-                                // the product with the FP32 value of 1/3 alone (3e-8 high) would
-                                // scale every stage-3 state by that bias, which the steps
-                                // accumulate (2e-6 in energy after 2000 steps at 256^3, against
-                                // 3e-8), so the product is corrected by its exact residual
-                                // r = s - 3 q (one FMA), which leaves the quotient within about
-                                // half an ulp of s / 3.
-                                const float2 p1 = ldn<NT>(PS1[c], g);
-                                const float hr = 0.5f * fcr, hi = 0.5f * fci;
-                                po = float2{div3(p1.x + 2.f * ps.x - (hr * k.x - hi * k.y)),
-                                            div3(p1.y + 2.f * ps.y - (hr * k.y + hi * k.x))};
-                            } else {
-                                const double2 r = ldn<NT>(P0[c], g);
-                                po = float2{(float)(sig * r.x), (float)(sig * r.y)};
-                            }
-                        }
-                        const float2 nw = float2{po.x + (fcr * h.x - fci * h.y), po.y + (fcr * h.y + fci * h.x)};
-                        if (pv) {
-                            if (a.stage == 1) stn<NT>(K[c], g, h);
-                            else stn<NT>(K[c], g, float2{k.x + 2.f * h.x, k.y + 2.f * h.y});
-                            stn<NT>(PSo[c] ? PSo[c] : PS[c], g, nw);
-                        }
-                        v[q] = nw;
-                    } else {
-                        const float2 k = ldn<NT>(K[c], g);
-                        const float ax = k.x + h.x, ay = k.y + h.y;
-                        const float dx = (fcr * ax - fci * ay) / 6.f, dy = (fcr * ay + fci * ax) / 6.f;
-                        const double2 r = ldn<NT>(P0[c], g);
-                        const double2 nw = double2{sig * r.x + (double)dx, sig * r.y + (double)dy};
-                        if (pv) stn<NT>(P0[c], g, nw);
-                        v[q] = from_d2<T>(nw);
-                        if (a.want_norm) sums[3 * c] += nw.x * nw.x + nw.y * nw.y;
-                    }
-                    if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
-                }
-            }
-            if (MODE != MODE_APPLY) {
-                fft_forward<T2, N, V, true>(v, t, my, tw);
-                if (pv)
+                if (MODE != MODE_APPLY) {
+                    fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
                     for (int q = 0; q < E; ++q) sc[elemB<N, V>(t, q)] = v[q];
+                }
             }
         }
-        if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm)) {
-            if (!pv)
+        if (DENS && MODE != MODE_APPLY) {
 #pragma unroll
-                for (int i = 0; i < NV; ++i) sums[i] = 0.0;
-            block_sum_store<NV>(sums, glob(a.partials) + (long long)NV * bx);
-        }
-        if (MODE == MODE_APPLY) return;
-    }
-    if (DENS) {
+            for (int q = 0; q < E; ++q) v[q] = T2{nd[q], (T)0};
+            fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = T2{nd[q], (T)0};
-        fft_forward<T2, N, V, true>(v, t, my, tw);
-#pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = elemB<N, V>(t, q);
-            if (e <= N / 2 && pv) specd[e] = v[q];
+            for (int q = 0; q < E; ++q) {
+                const int e = elemB<N, V>(t, q);
+                if (e <= N / 2) specd[e] = v[q];
+            }
         }
     }
+    if (MODE == MODE_APPLY || MODE == MODE_HB || (MODE == MODE_RK && a.want_norm))
+        block_sum_store<NV>(sums, glob(a.partials) + (long long)NV * bx);
 }
 
 // ---------------------------------------------------------------------------

@@ -61,3 +61,37 @@ def step_bytes_per_point(mixed: bool, ncomp: int = 1, dens: bool = True, nx: int
     plain = pass_bytes(mixed, ncomp, dens, nx)
     per_stage_passes = (2 * plain if three_d else 0) + pass_bytes(mixed, ncomp, dens, nx, table)
     return 4 * per_stage_passes + sum(xstage_bytes(s, mixed, ncomp, dens, nx) for s in (1, 2, 3, 4))
+
+
+def adjoint_bytes_per_point(mixed: bool, nsteps: int, every: int, nx: int = 128) -> float:
+    """Algorithmic bytes per grid point of one adjoint gradient of the
+    one-component dipolar GPE (fgpe_adjoint_forward + fgpe_adjoint_backward):
+
+      forward     nsteps RK4 steps, a checkpoint copy (32 B) per segment
+      recompute   per segment: the checkpoint copied back, every step's state
+                  copied to the segment buffer (32 B), every step but the
+                  last of the segment stepped again
+      per step    the stage states y1..y3 again (an x prologue of the state,
+                  then three RK stages with their spectral passes), and four
+                  vector-Jacobian products: k_adj_pro (adjoint seed in, state
+                  in, three spectral channels out), the y / last / y passes on
+                  three complex channels, k_adj_epi (channels, state, seed,
+                  trap, lambda in and out, next seed out, dV accumulated).
+    The adjoint vectors stay FP64 (16 B) in both forms; the state the
+    products are taken at is the bulk stage state, or the FP64 step input."""
+    sc, sv, s0 = sizes(mixed)
+    h = half_fraction(nx)
+    step = step_bytes_per_point(mixed, 1, True, nx)
+    nseg = (nsteps + every - 1) // every
+    fwd = nsteps * step + nseg * 32
+    recompute = nseg * 32 + nsteps * 32 + (nsteps - nseg) * step
+    stages = (s0 + sc * (1 + h)) + sum(3 * pass_bytes(mixed, 1, True, nx) + xstage_bytes(s, mixed, 1, True, nx)
+                                       for s in (1, 2, 3))
+    passes3 = 3 * 2 * 3 * sc
+    vjp = 0.0
+    for s in (3, 2, 1, 0):
+        y = s0 if s == 0 else sc
+        pro = (32 if s == 3 else 16) + y + 3 * sc
+        epi = 3 * sc + y + 16 + sv + 16 + (0 if s == 3 else 16) + 16 + (16 if s > 0 else 0) + 16
+        vjp += pro + passes3 + epi
+    return fwd + recompute + nsteps * (stages + vjp)

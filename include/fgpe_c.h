@@ -124,6 +124,13 @@ int fgpe_set_hamiltonian(fgpe_ctx* ctx, const fgpe_hparams* hp);
 int fgpe_upload_psi(fgpe_ctx* ctx, int comp, const double* host_psi);
 int fgpe_download_psi(fgpe_ctx* ctx, int comp, double* host_psi);
 int fgpe_upload_vext(fgpe_ctx* ctx, int comp, const double* host_vext);
+/* The same from and to device memory, ordered on the context's stream (no
+   host round trip; for callers whose arrays already live on the device,
+   such as the autograd function of the adjoint).  The trap form takes a real
+   FP64 potential (N doubles). */
+int fgpe_upload_psi_dev(fgpe_ctx* ctx, int comp, const void* dev_psi);
+int fgpe_download_psi_dev(fgpe_ctx* ctx, int comp, void* dev_psi);
+int fgpe_upload_vext_real_dev(fgpe_ctx* ctx, int comp, const double* dev_vext);
 /* Device address of component comp's FP64 state (N complex doubles). */
 int fgpe_psi_device_ptr(fgpe_ctx* ctx, int comp, void** dev_psi);
 

@@ -110,7 +110,7 @@ def test_adjoint_through_autograd(torch_cuda):
     from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
     c, V, psi0, target, a, a_dd = setup_case()
     dt, n = 5e-4, 30
-    solver = AdjointSolver(c.spec, dt, n)
+    solver = AdjointSolver(c.spec, dt, n, torch=torch)   # on torch's stream: device tensors pass straight in
     Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
     at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
     addt = torch.tensor(a_dd, dtype=torch.float64, requires_grad=True)
