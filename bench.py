@@ -277,6 +277,7 @@ def run_sweep(torch, args):
     tune = sw.autotune(3)
     dt, beta, eps = 2e-3, 0.9, 1e-9
     iters, conv, wall, launches = sw.run(dt, beta, eps, iters_per_graph=50, max_iters=args.sweep_max_iters,
+                                         recapture=args.sweep_recapture,
                                          sync=side.synchronize)
     obs = sw.energies()
     pts = np.array([i.grid.npoints for i in insts], dtype=np.float64)
@@ -305,7 +306,8 @@ def run_sweep(torch, args):
                    "iters_per_graph": 50},
         "converged": int(conv.sum()), "instances": len(insts),
         "iterations": {"min": int(iters.min()), "max": int(iters.max()), "total": int(iters.sum())},
-        "wall_s": round(wall, 3), "graph_launches": launches,
+        "wall_s": round(wall, 3), "graph_launches": launches, "graph_captures": sw.captures,
+        "recapture": args.sweep_recapture,
         "point_iterations_per_s": round(work_pts / wall, 1),
         "hbm_GBs_model": round(gbs, 1),
         "autotune": {"x".join(map(str, k)): {c: [round(x, 4) for x in v] for c, v in t.items()}
@@ -417,6 +419,9 @@ def main():
                     help="time the device initialisers (state and trap) at --grid instead")
     ap.add_argument("--sweep-n", type=int, default=64)
     ap.add_argument("--sweep-max-iters", type=int, default=20000)
+    ap.add_argument("--sweep-recapture", type=float, default=0.5,
+                    help="re-capture the graph over the unconverged instances once this share of the "
+                         "instances in it has converged")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches), the known-size "
                          "reference that scales FETCH_SIZE / WRITE_SIZE counter readings")

@@ -313,6 +313,21 @@ int finalize(fgpe_ctx* c, long long nblocks, int kind, const double* npart, doub
     return FGPE_OK;
 }
 
+int apply_pending_scale(fgpe_ctx* c);
+
+// The imaginary-time renormalisation is normally left pending and applied
+// by the next forward pass: a component by its own scale, the density by
+// scale[0]^2.  With two components and a k-space kernel the density is
+// n1 + n2 under two different scales, which no single factor rescales, so
+// there the states are rescaled at once and the spectra rebuilt.
+bool eager_renorm(const fgpe_ctx* c) { return c->hp.ncomp == 2 && c->nch > c->hp.ncomp; }
+
+template <typename T> int renorm_now(fgpe_ctx* c)
+{
+    FGPE_RET(apply_pending_scale(c));
+    return prologue<T>(c);
+}
+
 // RK4 steps; imaginary: c = -dt and renormalisation to npart after each step
 template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps, bool imaginary,
                                   const double* npart, double* hist_dev)
@@ -320,8 +335,6 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
     FGPE_RET(ensure_ready(c));
     const int nc = c->hp.ncomp;
     const bool dens = c->nch > nc;
-    if (imaginary && nc == 2 && dens)
-        return fail(FGPE_EUNSUP, "imaginary time with two components and a k-space kernel is not available");
     if (!c->spec_valid) FGPE_RET(prologue<T>(c));
     const double* scale = imaginary ? c->scal + SC_SCALE : nullptr;
     const long long nb = Launch<T>::xblocks(c);
@@ -359,9 +372,11 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
                     }
                 }
             FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_RK));
-            if (imaginary && s == 4)
+            if (imaginary && s == 4) {
                 FGPE_RET(finalize(c, nb, FIN_RENORM, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP,
                                   0.0));
+                if (eager_renorm(c)) FGPE_RET(renorm_now<T>(c));
+            }
         }
     }
     return FGPE_OK;
@@ -448,8 +463,6 @@ template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const doub
     FGPE_RET(ensure_ready(c));
     const int nc = c->hp.ncomp;
     const bool dens = c->nch > nc;
-    if (nc == 2 && dens)
-        return fail(FGPE_EUNSUP, "imaginary time with two components and a k-space kernel is not available");
     FGPE_RET(hb_prepare<T>(c));
     const long long nb = Launch<T>::xblocks(c);
     const int* stop = c->flags + FL_STOP;
@@ -458,6 +471,7 @@ template <typename T> int hb_run(fgpe_ctx* c, double dt, double beta, const doub
         // PARS[T] = iteration * dt over the whole run (src/oneComponentGPSolver.cu:50)
         FGPE_RET(Launch<T>::xstage(c, hb_xargs<T>(c, (double)(c->imag_iter + it) * dt, dt, beta), nc, dens, MODE_HB));
         FGPE_RET(finalize(c, nb, FIN_HB, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP, eps));
+        if (eager_renorm(c)) FGPE_RET(renorm_now<T>(c));
     }
     return FGPE_OK;
 }
@@ -892,6 +906,10 @@ struct fgpe_graph {
     hipGraphExec_t exec = nullptr;
     hipStream_t stream = nullptr;
     void* tab = nullptr;   // the argument tables the captured launches read
+    // one capture stream per group: the groups are independent, so each is a
+    // branch of the graph and their kernels may run side by side
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;   // the fork event, then one join event per group
 };
 
 namespace {
@@ -1744,7 +1762,22 @@ int fgpe_graph_capture_imag(fgpe_group* const* groups, int ngroups, double dt, d
     fgpe_graph* gr = new fgpe_graph();
     gr->stream = s;
     int rc = FGPE_OK;
-    if (hipMalloc(&gr->tab, total) != hipSuccess) rc = fail(FGPE_EHIP, "hipMalloc of the graph tables failed");
+    for (int k = 0; k < ngroups && rc == FGPE_OK; ++k) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            rc = fail(FGPE_EHIP, "hipStreamCreate for a graph branch failed");
+        else
+            gr->side.push_back(st);
+    }
+    for (int k = 0; k <= ngroups && rc == FGPE_OK; ++k) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            rc = fail(FGPE_EHIP, "hipEventCreate for a graph branch failed");
+        else
+            gr->ev.push_back(e);
+    }
+    if (rc == FGPE_OK && hipMalloc(&gr->tab, total) != hipSuccess)
+        rc = fail(FGPE_EHIP, "hipMalloc of the graph tables failed");
     for (int k = 0; k < ngroups && rc == FGPE_OK; ++k)
         if (hipMemcpy((unsigned char*)gr->tab + base[k], tabs[k].host.data(), tabs[k].bytes,
                       hipMemcpyHostToDevice) != hipSuccess)
@@ -1753,12 +1786,24 @@ int fgpe_graph_capture_imag(fgpe_group* const* groups, int ngroups, double dt, d
     if (rc == FGPE_OK && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
         rc = fail(FGPE_EHIP, "hipStreamBeginCapture failed");
     if (rc == FGPE_OK) {
-        for (int it = 0; it < iters && rc == FGPE_OK; ++it)
-            for (int k = 0; k < ngroups && rc == FGPE_OK; ++k) {
-                const unsigned char* d = (const unsigned char*)gr->tab + base[k];
-                rc = groups[k]->c[0]->mixed() ? group_launch<float>(groups[k], tabs[k], d, -1, MODE_HB)
-                                              : group_launch<double>(groups[k], tabs[k], d, -1, MODE_HB);
+        // fork: every group's iterations on a branch of its own, joined at the end
+        if (hipEventRecord(gr->ev[0], s) != hipSuccess) rc = fail(FGPE_EHIP, "fork event failed");
+        for (int k = 0; k < ngroups && rc == FGPE_OK; ++k) {
+            fgpe_group* g = groups[k];
+            if (hipStreamWaitEvent(gr->side[k], gr->ev[0], 0) != hipSuccess) {
+                rc = fail(FGPE_EHIP, "branch wait failed");
+                break;
             }
+            for (fgpe_ctx* c : g->c) c->stream = gr->side[k];
+            const unsigned char* d = (const unsigned char*)gr->tab + base[k];
+            for (int it = 0; it < iters && rc == FGPE_OK; ++it)
+                rc = g->c[0]->mixed() ? group_launch<float>(g, tabs[k], d, -1, MODE_HB)
+                                      : group_launch<double>(g, tabs[k], d, -1, MODE_HB);
+            for (fgpe_ctx* c : g->c) c->stream = s;
+            if (rc == FGPE_OK && (hipEventRecord(gr->ev[k + 1], gr->side[k]) != hipSuccess ||
+                                  hipStreamWaitEvent(s, gr->ev[k + 1], 0) != hipSuccess))
+                rc = fail(FGPE_EHIP, "branch join failed");
+        }
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &graph);
         if (rc == FGPE_OK && e != hipSuccess)
@@ -1790,6 +1835,8 @@ int fgpe_graph_destroy(fgpe_graph* g)
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->tab) (void)hipFree(g->tab);
+    for (hipEvent_t e : g->ev) (void)hipEventDestroy(e);
+    for (hipStream_t st : g->side) (void)hipStreamDestroy(st);
     delete g;
     return FGPE_OK;
 }

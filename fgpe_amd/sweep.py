@@ -40,33 +40,54 @@ class Sweep:
     def autotune(self, reps=3):
         return {s: g.autotune(reps) for s, g in zip(self.shapes, self.groups)}
 
-    def run(self, dt, beta, eps, iters_per_graph=50, max_iters=20000, sync=None):
-        """Replays the captured iterations until all instances converge.
-        Returns per-instance iteration counts, convergence flags, the wall
-        time of the replays and the number of replays."""
-        self.graph = Graph(self.groups, dt, iters_per_graph, beta=beta, npart=1.0, eps=eps)
+    def run(self, dt, beta, eps, iters_per_graph=50, max_iters=20000, sync=None, recapture=0.5):
+        """Replays captured iterations until all instances converge (or
+        max_iters iterations have been replayed).  Converged instances only
+        cost an early exit per block, but their blocks are still launched:
+        once a `recapture` share of the instances in the current graph has
+        converged, the graph is captured again over the unconverged ones only
+        (each heavy-ball run continues where it stopped).  Returns
+        per-instance iteration counts, convergence flags, the wall time of the
+        replays (captures included) and the number of replays."""
         n = len(self.instances)
         iters = np.zeros(n, dtype=np.int64)
         conv = np.zeros(n, dtype=bool)
-        launches = 0
+        launches = replayed = 0
+        self.captures = 0
+        active = [list(m) for m in self.members]
         t0 = time.perf_counter()
-        while launches * iters_per_graph < max_iters:
-            self.graph.launch()
-            launches += 1
-            conv[:] = True
-            for m, g in zip(self.members, self.groups):
-                it, cv = g.imag_status()
-                iters[m] = it
-                conv[m] = cv
-            if conv.all():
-                break
+        done = False
+        while not done and replayed < max_iters:
+            members = [m for m in active if m]
+            groups = [Group([self.ctxs[i] for i in m]) for m in members]
+            graph = Graph(groups, dt, iters_per_graph, beta=beta, npart=1.0, eps=eps)
+            self.captures += 1
+            base = iters.copy()   # the capture restarts every context's iteration counter
+            n_active = sum(len(m) for m in members)
+            while replayed < max_iters:
+                graph.launch()
+                launches += 1
+                replayed += iters_per_graph
+                n_conv = 0
+                for m, g in zip(members, groups):
+                    it, cv = g.imag_status()
+                    iters[m] = base[m] + it
+                    conv[m] = cv
+                    n_conv += int(cv.sum())
+                if n_conv == n_active:
+                    done = True
+                    break
+                if n_conv >= max(1.0, recapture * n_active):
+                    break
+            graph.close()
+            for g in groups:
+                g.close()
+            active = [[i for i in m if not conv[i]] for m in active]
         if sync is not None:
             sync()
         wall = time.perf_counter() - t0
-        for m, g in zip(self.members, self.groups):
-            it, cv = g.imag_finish()
-            iters[m] = it
-            conv[m] = cv
+        for g in self.groups:
+            g.imag_finish()   # applies every instance's pending renormalisation
         return iters, conv, wall, launches
 
     def energies(self):

@@ -355,7 +355,9 @@ int fgpe_group_imag_finish(fgpe_group* group, int* iters, int* converged);
 /* ---- captured launch sequences -------------------------------------------
    fgpe_graph_capture_imag prepares the groups (as fgpe_group_imag_steps does
    on its first call), then captures `iters` heavy-ball iterations of all of
-   them as one hipGraph on their common stream (not the null stream).  Each
+   them as one hipGraph on their common stream (not the null stream), each
+   group on a branch of its own (the groups are independent, so their
+   kernels may run side by side).  Each
    fgpe_graph_launch replays those iterations; the host polls with
    fgpe_group_imag_status between launches and ends the run with
    fgpe_group_imag_finish.  The graph keeps its own copy of the argument

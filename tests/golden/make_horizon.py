@@ -9,8 +9,8 @@ oracle takes minutes there, too long to run inside a GPU test):
       the dipolar real-time RK4 physics (eps_dd = 1.3, cigar trap,
       dt = 5e-4, box 16 x 16 x 32) on the given grid over nsteps steps.
 
-At every checkpoint the fixture keeps psi on every 4th grid point per axis
-(`psi_s`), max |psi| over the whole grid (`psimax`, the scale of the
+At every checkpoint the fixture keeps psi on every 4th (config 2: every
+(nx/8)th) grid point per axis (`psi_s`), max |psi| over the whole grid (`psimax`, the scale of the
 relative error), sum |psi| dv (`absum`), and the norm, mu and E of the
 full state from the oracle's H (`obs`).  The computation is the oracle's
 (oracle/fgpe_oracle.c, built with contraction off), on this machine's cores.
@@ -32,19 +32,19 @@ sys.path.insert(0, ROOT)
 from fgpe_amd import configs  # noqa: E402
 from oracle import oracle as orc  # noqa: E402
 
-SUB = 4   # sampling stride per axis
+SUB = 4   # sampling stride per axis (config 2: nx / 8, so its fixtures stay small)
 
 
-def summary(og, ham, psi, V, t):
+def summary(og, ham, psi, V, t, sub=SUB):
     hmu, hen = orc.hpsi(og, ham, t, psi, V)
     nrm = orc.norm(og, psi)
     mu, en = orc.obs2(og, nrm, psi, hmu, hen)
-    return dict(psi_s=psi[::SUB, ::SUB, ::SUB].copy(), psimax=float(np.abs(psi).max()),
+    return dict(psi_s=psi[::sub, ::sub, ::sub].copy(), psimax=float(np.abs(psi).max()),
                 absum=float(np.abs(psi).sum() * og.dv), obs=np.array([nrm, mu, en]))
 
 
-def write(name, steps, sums, **meta):
-    out = dict(steps=np.array(steps), sub=SUB, **meta)
+def write(name, steps, sums, sub=SUB, **meta):
+    out = dict(steps=np.array(steps), sub=sub, **meta)
     for key in ("psi_s", "psimax", "absum", "obs"):
         out[key] = np.array([s[key] for s in sums])
     path = os.path.join(HERE, name)
@@ -84,9 +84,9 @@ def config2(nx, ny, nz, nsteps, every, snapdir=None):
         else:
             psi = orc.rk4(og, ham, psi, V, w.dt, every, t0=s * w.dt)
         steps.append(s + every)
-        sums.append(summary(og, ham, psi, V, (s + every) * w.dt))
+        sums.append(summary(og, ham, psi, V, (s + every) * w.dt, sub=nx // 8))
         print(s + every, sums[-1]["obs"], flush=True)
-    write(f"config2_{nx}x{ny}x{nz}_{nsteps}.npz", steps, sums, dt=w.dt)
+    write(f"config2_{nx}x{ny}x{nz}_{nsteps}.npz", steps, sums, sub=nx // 8, dt=w.dt)
 
 
 def main():

@@ -141,3 +141,44 @@ def test_small_sweep_converges_to_oracle_ground_states(torch_cuda):
               f"oracle {e_ref:.12f} rel {rel:.2e}; iterations gpu {iters[k]} oracle {nref}")
         assert nref < 6000
         assert rel < 1e-6, (k, rel)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sweep_shapes_against_oracle(torch_cuda, precision):
+    """The sweep configuration's own grids (96^3, 64 x 64 x 192, 2D 512^2),
+    two instances each (dipolar and soft-core), run as batched groups on the
+    branches of one captured graph: 40 heavy-ball iterations (two replays of
+    20) against the oracle's 40 iterations of each instance, the FP64 form
+    within 1e-11 on psi and 1e-11 on the energy, the mixed form within 1e-6
+    on the energy."""
+    from fgpe_amd import configs
+    from fgpe_amd.sweep import Sweep
+    from tests.test_gpu_parity import oracle_ham
+    torch = torch_cuda
+    side = torch.cuda.Stream()
+    insts = configs.sweep_instances(6, seed=7, shapes=((96, 96, 96), (64, 64, 192), (512, 512, 1)))
+    sw = Sweep(insts, precision=precision, stream=side.cuda_stream)
+    try:
+        iters, conv, wall, launches = sw.run(DT, BETA, 0.0, iters_per_graph=20, max_iters=40, sync=side.synchronize)
+        assert list(iters) == [40] * len(insts) and not conv.any(), (iters, conv)
+        states = [c.download_psi() for c in sw.ctxs]
+        obs = sw.energies()
+    finally:
+        sw.close()
+    orc.set_threads(16)
+    for k, si in enumerate(insts):
+        c = Case(si.grid.nx, si.grid.ny, si.grid.nz, L=(si.grid.lx, si.grid.ly, si.grid.lz))
+        h = dict(a=(si.a,), a_dd=si.a_dd, dip=si.dip, sc_U=si.sc_U, sc_R=si.sc_R, sc_dim=si.sc_dim)
+        ham, V = oracle_ham(c, h), si.trap()
+        ref, nref, href = orc.imag_heavy_ball(c.grid, ham, si.initial_state(), V, DT, BETA, 1.0, 0.0, 40)
+        # the energy of the final state (the history's E lags one iteration)
+        hmu, hen = orc.hpsi(c.grid, ham, 0.0, ref, V)
+        e_ref = orc.obs2(c.grid, orc.norm(c.grid, ref), ref, hmu, hen)[1]
+        e_psi = float(np.abs(states[k] - ref).max() / np.abs(ref).max())
+        e_en = abs(obs[k, 2] - e_ref) / abs(e_ref)
+        print(f"precision {precision} instance {k} ({si.kind}, {si.grid.nx}x{si.grid.ny}x{si.grid.nz}): "
+              f"psi {e_psi:.2e}, E {e_en:.2e}", flush=True)
+        if precision == 0:
+            assert e_psi < 1e-11 and e_en < 1e-11, (k, e_psi, e_en)
+        else:
+            assert e_en < 1e-6, (k, e_en)

@@ -81,6 +81,37 @@ def test_config1_imaginary_rk4_64cubed():
     assert all(e1 < e0 for e0, e1 in zip(energies, energies[1:])), energies
 
 
+def test_config2_horizon_64x64x128():
+    """The dipolar real-time physics (eps_dd = 1.3, cigar trap, dt = 5e-4)
+    over its full 4000 steps on a 64 x 64 x 128 grid, against the oracle's
+    run (a fixture, tests/golden/make_horizon.py) at steps 1000 ... 4000:
+    FP64 within 1e-12 on psi, sum |psi|, norm, mu and E; the mixed form
+    within 1e-6 of the FP64 oracle in energy (its psi error, which the
+    instability amplifies from FP32 rounding, within 1e-3).  The oracle
+    built with and without FMA contraction stays within 5e-14 of itself over
+    these 4000 steps, so 1e-12 leaves room for rounding, not for a fault."""
+    w = configs.dipolar_real(64, 64, 128, nsteps=4000)
+    print(f"config 2 horizon: {w.name}", flush=True)
+    fx = fixture("config2_64x64x128_4000.npz")
+    every = int(fx["steps"][1] - fx["steps"][0])
+    for prec in (FGPE_PREC_F64, FGPE_PREC_MIXED):
+        with ctx_for(w, prec) as c:
+            for k, step in enumerate(fx["steps"]):
+                c.rk4_steps(w.dt, every, (step - every) * w.dt)
+                o = c.observables(step * w.dt)
+                if prec == FGPE_PREC_F64:
+                    compare_to_fixture(fx, k, c.download_psi(), o, w.grid.dv)
+                else:
+                    sub = int(fx["sub"])
+                    e_psi = float(np.abs(c.download_psi()[::sub, ::sub, ::sub] - fx["psi_s"][k]).max()
+                                  / fx["psimax"][k])
+                    e_en = abs(o[2] - fx["obs"][k][2]) / abs(fx["obs"][k][2])
+                    print(f"  mixed step {int(step)}: psi {e_psi:.2e}, E {e_en:.2e}", flush=True)
+                    # the energy is the bound; psi's error grows with the unstable
+                    # dynamics (measured 1.6e-7, 7.6e-7, 2.9e-6, 8.0e-5 at steps 1000-4000)
+                    assert e_en < 1e-6 and e_psi < 1e-3, (k, e_en, e_psi)
+
+
 def test_config2_dipolar_real_time_full_grid():
     """128 x 128 x 256 dipolar real-time RK4 (eps_dd = 1.3), dt = 5e-4: two
     steps against the oracle at the full size; over 400 steps the norm never

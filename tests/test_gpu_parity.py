@@ -569,3 +569,36 @@ def test_rk4_nontemporal_x_stage_f64():
     ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 2)
     err = rel_max(gpu, ref)
     assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_two_component_imaginary_time_with_softcore(prec):
+    """Two components with the soft-core kernel on n1 + n2 in imaginary time
+    (the renormalisation applied at once, since the two components' scales
+    differ): heavy ball with the same iterations, state and norm / mu / E
+    history as the oracle (1e-11), and RK4 with renormalisation (1e-12); the
+    mixed form within 1e-6."""
+    from fgpe_amd import FGPE_IMAG_HEAVY_BALL, FGPE_IMAG_RK4_RENORM
+    c, V, psi = two_comp_case()
+    h = dict(ncomp=2, a=(0.02, 0.03, 0.01), sc_U=2.0, sc_R=1.2, sc_dim=3)
+    tol = 1e-11 if prec == 0 else 1e-6
+    dt, beta, npart, eps, n = 2e-3, 0.9, 1.0, 1e-9, 50
+    ref, nref, href = orc.imag_heavy_ball(c.grid, two_comp_ham(c, h), psi, V, dt, beta, npart, eps, n)
+    with c.ctx(prec) as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        done, hist = ctx.imag_steps(dt, n, scheme=FGPE_IMAG_HEAVY_BALL, beta=beta, npart=npart, eps=eps,
+                                    history=True)
+        gpu = ctx.download_psi()
+    assert done == nref
+    assert rel_max(gpu, ref) < tol
+    assert np.abs(hist - href).max() <= tol * np.abs(href).max()
+    ref = orc.rk4(c.grid, two_comp_ham(c, h), psi, V, 1e-3, 30, imaginary=True, npart=npart)
+    with c.ctx(prec) as ctx:
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ctx.imag_steps(1e-3, 30, scheme=FGPE_IMAG_RK4_RENORM, npart=npart)
+        err = rel_max(ctx.download_psi(), ref)
+    assert err < (1e-12 if prec == 0 else 1e-6), err
