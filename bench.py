@@ -319,6 +319,56 @@ def run_sweep(torch, args):
     return 0
 
 
+def run_sweep_kernels(torch, args):
+    """Per-kernel-class HBM throughput of the sweep's batched launches: each
+    shape group of the 64-instance sweep runs 20 heavy-ball iterations as
+    eager batched launches with every instance active (eps = 0), bracketed
+    by HIP events per launch; the bytes per launch are the byte model's over
+    the group's instances (fgpe_amd/roofline.py)."""
+    from fgpe_amd import FGPE_PREC_MIXED, configs, roofline
+    from fgpe_amd.sweep import Sweep
+    torch.cuda.set_device(0)
+    insts = configs.sweep_instances(args.sweep_n)
+    side = torch.cuda.Stream()
+    sw = Sweep(insts, precision=FGPE_PREC_MIXED, stream=side.cuda_stream)
+    sw.autotune(3)
+    sc, sv, s0 = roofline.sizes(True)
+    rows = []
+    for shape, members, group in zip(sw.shapes, sw.members, sw.groups):
+        nx, ny, nz = shape
+        c0 = sw.ctxs[members[0]]
+        group.imag_steps(2e-3, 2, beta=0.9, npart=1.0, eps=0.0)   # warm-up
+        side.synchronize()
+        c0.timing_enable(True)
+        iters = 20
+        group.imag_steps(2e-3, iters, beta=0.9, npart=1.0, eps=0.0)
+        kt = c0.timing_read()
+        c0.timing_enable(False)
+        pts = nx * ny * nz
+        spec = 2 * roofline.channels(1, True, nx) * sc
+        bpp = {"xstage": sum(spec + 2 * s0 + 2 * sc + sv for _ in members),
+               "strided": sum(roofline.pass_bytes(True, 1, True, nx) for _ in members),
+               "last": sum(roofline.pass_bytes(True, 1, True, nx, table=bool(insts[i].sc_U)) for i in members)}
+        for cls in ("xstage", "strided", "last"):
+            ms, n = kt[cls]
+            if n == 0:
+                continue
+            per = ms / n
+            gbs = bpp[cls] * pts / (per * 1e-3) / 1e9
+            rows.append({"shape": list(shape), "instances": len(members), "kernel": cls, "launches": n,
+                         "avg_us": round(1e3 * per, 2), "bytes_per_launch": round(bpp[cls] * pts),
+                         "GBs": round(gbs, 1), "frac_of_peak": round(gbs / roofline.HBM_PEAK_GBS, 4)})
+            print(f"# {shape} x{len(members)} {cls}: {1e3 * per:.1f} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+        ms, n = kt["other"]
+        if n:
+            rows.append({"shape": list(shape), "instances": len(members), "kernel": "finalize", "launches": n,
+                         "avg_us": round(1e3 * ms / n, 2)})
+    sw.close()
+    print(json.dumps({"metric": "sweep batched launches, per kernel class: HBM GB/s of the byte model",
+                      "unit": "GB/s", "n_gpus": 1, "dtype": "f32 bulk, f64 state", "rows": rows}), flush=True)
+    return 0
+
+
 def run_init(torch, args):
     """Device initialisation of the state and trap at the RK4 workload's grid:
     harmonic psi with complex noise, normalisation, a vortex imprint, the
@@ -417,6 +467,8 @@ def main():
                          "default line, 16 with --adjoint)")
     ap.add_argument("--init", action="store_true",
                     help="time the device initialisers (state and trap) at --grid instead")
+    ap.add_argument("--sweep-kernels", action="store_true",
+                    help="time the sweep's batched launches per kernel class and shape group (HBM GB/s)")
     ap.add_argument("--sweep-n", type=int, default=64)
     ap.add_argument("--sweep-max-iters", type=int, default=20000)
     ap.add_argument("--sweep-recapture", type=float, default=0.5,
@@ -442,6 +494,8 @@ def main():
         print(json.dumps({"copy_calib_bytes_each_way": n * 8, "launches": 3}), flush=True)
         return 0
     from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, configs, roofline
+    if args.sweep_kernels:
+        return run_sweep_kernels(torch, args)
     if args.sweep:
         return run_sweep(torch, args)
     if args.adjoint:

@@ -803,11 +803,26 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
                                                        (1.0 + m * a.dt) * ps.y - a.dt * h.y + a.beta * (ps.y - old.y)};
                             stn<NT>(PS[c], g, from_d2<T>(ps));
                             stn<NT>(P0[c], g, nw);
+                            const double nh = nw.x * h.x + nw.y * h.y;
                             sums[3 * c + 0] += nw.x * nw.x + nw.y * nw.y;
-                            sums[3 * c + 1] += nw.x * h.x + nw.y * h.y;
-                            sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
+                            sums[3 * c + 1] += nh;
+                            if constexpr (UTerm::enabled) {
+                                sums[3 * c + 2] += nw.x * he.x + nw.y * he.y;
+                            } else {
+                                // he = h - d psi with d = rX - eX, so <nw, he> = <nw, h> - d <nw, psi>:
+                                // he itself is not formed (holding it for every element of the
+                                // pencil doubled the kernel's registers at E >= 8)
+                                double d = (g4s - g2s) * n;
+                                if (NC == 2) d += (g4o - g2o) * no;
+                                if (DENS) d += 0.5 * (double)phi_at(q);
+                                sums[3 * c + 2] += nh - d * (nw.x * ps.x + nw.y * ps.y);
+                            }
                             v[q] = from_d2<T>(nw);
                             if (DENS) nd[q] += v[q].x * v[q].x + v[q].y * v[q].y;
+                            // Two elements at a time: without the fence the scheduler starts every
+                            // element's loads and FP64 arithmetic at once and the kernel runs at
+                            // one wave per SIMD (128^3 FP32: 512 -> 164 registers with both changes).
+                            if (q % 2 == 1) __builtin_amdgcn_sched_barrier(0);
                         } else {
                             if (a.stage != 1) {
                                 const double2 r = ldn<NT>(P0[c], g);
