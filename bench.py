@@ -80,13 +80,15 @@ def run_form(torch, w, psi, V, precision, steps, warmup):
     t += steps * w.dt
     obs = ctx.observables(t)
     out = ctx.download_psi()
-    chosen = {k: v for k, v in ctx.variants().items() if k[1] == {"xstage": 0, "strided": 1, "last": 2}[k[0]]}
+    la = ctx.kmul_axis()
+    axis_of = {"xstage": 0, "strided": 3 - la if g.nz > 1 else 1, "last": la}
+    chosen = {k: v for k, v in ctx.variants().items() if k[1] == axis_of[k[0]]}
     nslab = ctx.slabs()
     order = ctx.order()
     ctx.close()
     return ms / steps, kt, obs, out, {"autotune_ms": {k: [round(x, 4) for x in v] for k, v in tune.items()},
                                       "variant": {k[0]: v for k, v in chosen.items()}, "slabs": nslab,
-                                      "order": order}
+                                      "order": order, "kmul_axis": la}
 
 
 def cpu_model():

@@ -29,7 +29,7 @@ EXPORTS = (
     "fgpe_observables", "fgpe_adjoint_forward", "fgpe_adjoint_backward", "fgpe_calc_px", "fgpe_calc_cos", "fgpe_rk4_steps", "fgpe_imag_steps", "fgpe_imag_converged",
     "fgpe_timing_enable",
     "fgpe_timing_read", "fgpe_autotune", "fgpe_set_variant", "fgpe_get_variant", "fgpe_set_slabs",
-    "fgpe_get_slabs", "fgpe_set_order", "fgpe_get_order",
+    "fgpe_get_slabs", "fgpe_set_order", "fgpe_get_order", "fgpe_set_kmul_axis", "fgpe_get_kmul_axis",
     "fgpe_group_create", "fgpe_group_destroy", "fgpe_group_autotune", "fgpe_group_imag_steps",
     "fgpe_group_imag_status", "fgpe_group_imag_finish", "fgpe_graph_capture_imag", "fgpe_graph_launch", "fgpe_graph_destroy",
     "fgpe_psi_init", "fgpe_psi_add_noise", "fgpe_psi_imprint_vortex_tube", "fgpe_psi_force_lattice",
@@ -128,6 +128,8 @@ def load() -> C.CDLL:
     L.fgpe_get_slabs.argtypes = [_vp, C.POINTER(C.c_int)]
     L.fgpe_set_order.argtypes = [_vp, C.c_int]
     L.fgpe_get_order.argtypes = [_vp, C.POINTER(C.c_int)]
+    L.fgpe_set_kmul_axis.argtypes = [_vp, C.c_int]
+    L.fgpe_get_kmul_axis.argtypes = [_vp, C.POINTER(C.c_int)]
     L.fgpe_timing_enable.argtypes = [_vp, C.c_int]
     L.fgpe_timing_read.argtypes = [_vp, _dp]
     _ip = C.POINTER(C.c_int)
@@ -334,7 +336,7 @@ class Context:
         check(load().fgpe_adjoint_forward(self.h, dt, t0, nsteps, ckpt_every))
 
     def adjoint_backward(self, dev_lamT: int, dev_lam0: int | None, dev_gradV: int):
-        g = np.zeros(2)
+        g = np.zeros(3)
         check(load().fgpe_adjoint_backward(self.h, dev_lamT, dev_lam0, dev_gradV, _dptr(g)))
         return g
 
@@ -355,11 +357,13 @@ class Context:
         return bool(f.value)
 
     def autotune(self, reps=5):
-        """Time both plan variants per kernel class, then 1/2/4 column slabs of
-        the y/last/y sequence; returns {class: (ms v0, ms v1), "slabs": (ms 1, 2, 4)}."""
-        out = np.zeros(9)
+        """Time the plan variants per kernel class, the block orders, the
+        k-space multiply axis (3D) and 1/2/4 column slabs of the strided/last/
+        strided sequence; returns {class: (ms v0, ms v1), "xstage": (ms v0,
+        ms v1, ms v2 or 0), "slabs": (ms 1, 2, 4)}."""
+        out = np.zeros(10)
         check(load().fgpe_autotune(self.h, reps, _dptr(out)))
-        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6]),
+        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": (out[4], out[5], out[9]),
                 "slabs": tuple(out[6:9])}
 
     def set_slabs(self, nslab: int):
@@ -376,6 +380,14 @@ class Context:
     def order(self) -> int:
         v = C.c_int()
         check(load().fgpe_get_order(self.h, C.byref(v)))
+        return v.value
+
+    def set_kmul_axis(self, axis: int):
+        check(load().fgpe_set_kmul_axis(self.h, axis))
+
+    def kmul_axis(self) -> int:
+        v = C.c_int()
+        check(load().fgpe_get_kmul_axis(self.h, C.byref(v)))
         return v.value
 
     def set_variant(self, kclass: int, axis: int, variant: int):
@@ -432,9 +444,9 @@ class Group:
             pass
 
     def autotune(self, reps=3):
-        out = np.zeros(9)
+        out = np.zeros(10)
         check(load().fgpe_group_autotune(self.h, reps, _dptr(out)))
-        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": tuple(out[4:6])}
+        return {"strided": tuple(out[0:2]), "last": tuple(out[2:4]), "xstage": (out[4], out[5], out[9])}
 
     def imag_steps(self, dt, nsteps, beta=0.0, npart=1.0, eps=0.0):
         np_ = np.ascontiguousarray(np.broadcast_to(np.asarray(npart, dtype=np.float64), (2,)))

@@ -4,13 +4,17 @@
 the one-component GPE: its forward runs the HIP path with a checkpoint every
 `ckpt_every` steps (about sqrt(nsteps)), its backward runs the discrete
 adjoint of the RK4 step (fgpe_adjoint_backward) and returns gradients with
-respect to the trap V, the contact length a and the dipolar length a_dd
-(and the initial state).  Gradients follow torch's convention for complex
-tensors: dL/dRe + i dL/dIm.
+respect to the trap V, the contact length a, the strength s of the solver's
+k-space kernel (and the initial state).  Gradients follow torch's convention
+for complex tensors: dL/dRe + i dL/dIm.
 
-    solver = AdjointSolver(grid, dt, nsteps, ckpt_every, a_dd_dir=(0, 0, 1))
-    psiT = GPEEvolve.apply(V, a, a_dd, psi0, solver)
+    solver = AdjointSolver(grid, dt, nsteps, ckpt_every, dip=(0, 0, 1))
+    psiT = GPEEvolve().apply(V, a, a_dd, psi0, solver)
     loss = fidelity(psiT, target, grid.dv); loss.backward()
+
+The kernel is dipolar (s = a_dd, direction `dip`) unless the solver is made
+with softcore=(dim, R): then s is the soft-core strength U of the step
+potential of radius R (src/parallel.cu:371-477), and a_dd is 0.
 
 In terms of g = 4 pi a and eps_dd = a_dd / a:
     dL/dg = (dL/da + eps_dd dL/da_dd) / (4 pi),  dL/deps_dd = a dL/da_dd.
@@ -32,21 +36,29 @@ class AdjointSolver:
     synchronisation."""
 
     def __init__(self, grid, dt, nsteps, ckpt_every=None, dip=(0.0, 0.0, 1.0), precision=FGPE_PREC_F64,
-                 torch=None):
+                 torch=None, softcore=None):
         self.grid = grid
         self.dt = float(dt)
         self.nsteps = int(nsteps)
         self.every = int(ckpt_every or max(1, round(math.sqrt(nsteps))))
         self.dip = tuple(dip)
+        self.softcore = tuple(softcore) if softcore is not None else None
         self.ctx = Context(grid.nx, grid.ny, grid.nz, precision)
         if torch is not None:
             self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
         kx, ky, kz = grid.kaxes()
         self.ctx.set_kspace(kx, ky, kz, grid.dv)
 
+    def _hamiltonian(self, a, s):
+        if self.softcore is None:
+            self.ctx.set_hamiltonian(a=(float(a),), a_dd=float(s), dip=self.dip)
+        else:
+            dim, R = self.softcore
+            self.ctx.set_hamiltonian(a=(float(a),), sc_U=float(s), sc_R=float(R), sc_dim=int(dim))
+
     def configure(self, V, a, a_dd, psi0):
-        """V, psi0 as host arrays."""
-        self.ctx.set_hamiltonian(a=(float(a),), a_dd=float(a_dd), dip=self.dip)
+        """V, psi0 as host arrays; a_dd: the kernel strength (U for soft-core)."""
+        self._hamiltonian(a, a_dd)
         self.ctx.upload_vext(np.asarray(V, dtype=np.float64).astype(np.complex128))
         self.ctx.upload_psi(np.asarray(psi0, dtype=np.complex128))
 
@@ -62,7 +74,7 @@ class AdjointSolver:
         V = V.detach().to(torch.float64).contiguous()
         psi0 = psi0.detach().to(torch.complex128).contiguous()
         torch.cuda.synchronize()   # the tensors' producers may run on another stream
-        self.ctx.set_hamiltonian(a=(float(a),), a_dd=float(a_dd), dip=self.dip)
+        self._hamiltonian(a, a_dd)
         self.ctx.upload_vext_real_dev(V.data_ptr())
         self.ctx.upload_psi_dev(psi0.data_ptr())
         self.ctx.adjoint_forward(self.dt, self.nsteps, self.every)
@@ -79,7 +91,7 @@ class AdjointSolver:
         torch.cuda.synchronize()
         g = self.ctx.adjoint_backward(lam.data_ptr(), lam0.data_ptr(), gV.data_ptr())
         self.ctx.sync()
-        return gV, g[0], g[1], lam0
+        return gV, g[0], (g[1] if self.softcore is None else g[2]), lam0
 
 
 def _autograd():

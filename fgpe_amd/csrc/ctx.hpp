@@ -46,6 +46,12 @@ int fail(int code, const std::string& msg);
 #define FGPE_FOR_EACH_N(X) X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
 #endif
 
+// Lengths whose x kernels have a distinct third plan (Plan<N, 2>).
+#define FGPE_FOR_EACH_N_X2(X) X(128) X(256)
+
+// Whether the x-stage class of an nx-long grid has a distinct variant 2.
+constexpr bool x_variant2(int nx) { return nx == 128 || nx == 256; }
+
 // Scalar slots of fgpe_ctx::scal (doubles) and ::flags (ints) on the device.
 enum {
     SC_SCALE = 0,   // [2] pending renormalisation per component
@@ -118,6 +124,9 @@ struct fgpe_ctx {
     int var[FGPE_NKCLASS][3] = {};   // plan variant per kernel class and axis
     int nslab = 1;                   // column slabs of the y / last / y pass sequence
     int order = 0;                   // bit k set: pass k (y fwd, last, y inv, x) runs its blocks reversed
+    // 3D grids: the k-space multiply rides on the y pass (z pencils run the
+    // strided forward and inverse passes) instead of the z pass
+    bool ylast = false;
 
     // launch timing (fgpe_timing_enable)
     bool timing = false;

@@ -161,7 +161,10 @@ template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
 // Variant 0 favours few threads per pencil (E = 16 elements per thread for
 // the long pencils); variant 1 uses more threads and fewer registers (E = 8)
 // where a second decomposition exists.  `distinct` says whether variant 1
-// differs from variant 0.
+// differs from variant 0.  Variant 2 (the x pencils only) takes E = 4
+// elements per thread in radix-4 stages where that is yet another plan: at
+// N = 256 a pencil is then exactly one wave; on other lengths it repeats
+// variant 1 (`distinct` false: differs from variant 1 or not).
 template <int N, int V = 0> struct Plan;
 #define FGPE_PLAN(N_, V_, L_, T_, R0, R1, R2, D_)                 \
     template <> struct Plan<N_, V_> {                            \
@@ -197,6 +200,11 @@ FGPE_PLAN(256, 1, 3, 32, 8, 4, 8, true)
 FGPE_PLAN4(384, 1, 32, 12, 4, 4, 2, true)
 FGPE_PLAN(512, 1, 3, 64, 8, 8, 8, false)
 FGPE_PLAN(1024, 1, 3, 64, 16, 8, 8, false)
+template <int N> struct Plan<N, 2> : Plan<N, 1> {
+    static constexpr bool distinct = false;
+};
+FGPE_PLAN4(128, 2, 32, 4, 4, 4, 2, true)
+FGPE_PLAN4(256, 2, 64, 4, 4, 4, 4, true)
 #undef FGPE_PLAN
 #undef FGPE_PLAN4
 template <int N, int V, int DIR, int S> __host__ __device__ constexpr int radix_of()

@@ -180,6 +180,10 @@ template <typename T> std::vector<c2_t<T>> twiddle_table(int n)
 
 // the last axis of a 3D (2) or 2D (1) grid; 0 for 1D grids, which have only x
 int last_axis(fgpe_ctx* c) { return c->n[2] > 1 ? 2 : (c->n[1] > 1 ? 1 : 0); }
+// The axis whose pass carries the k-space multiply in a context's own
+// (unbatched) passes: z, or y when the autotuner found that faster on a 3D
+// grid; the other of y and z runs the strided forward and inverse passes.
+int kmul_axis(const fgpe_ctx* c) { return c->n[2] > 1 ? (c->ylast ? 1 : 2) : (c->n[1] > 1 ? 1 : 0); }
 bool has_filter(const fgpe_hparams& h) { return h.a_dd != 0.0 || h.sc_U != 0.0; }
 
 // d_quench_as, hamiltonians/BEC_oneComponent.cu:167-178
@@ -227,14 +231,15 @@ template <typename T> int spectral_passes_on(fgpe_ctx* c, void* spec, int nch, i
                                              const int* stop, bool half, bool vpx = false)
 {
     const bool filter = has_filter(c->hp);
-    if (last_axis(c) == 0) return kmul_1d<T>(c, spec, nch, nc, scale, filter, stop, half, vpx);
-    if (last_axis(c) == 2) {
+    const int la = kmul_axis(c);
+    if (la == 0) return kmul_1d<T>(c, spec, nch, nc, scale, filter, stop, half, vpx);
+    if (c->n[2] > 1) {
         // slab by slab of columns: a slab's three passes run while it is still in the memory-side cache
-        const int ns = half ? c->nslab : 1;
+        const int ns = half ? c->nslab : 1, sa = 3 - la;
         for (int sl = 0; sl < ns; ++sl) {
-            FGPE_RET(Launch<T>::strided(c, spec, 1, -1, nch, nc, scale, stop, half, nullptr, 1, sl, ns));
-            FGPE_RET(Launch<T>::last(c, spec, 2, nch, nc, nullptr, filter, stop, half, nullptr, 1, sl, ns, vpx));
-            FGPE_RET(Launch<T>::strided(c, spec, 1, +1, nch, nc, nullptr, stop, half, nullptr, 1, sl, ns));
+            FGPE_RET(Launch<T>::strided(c, spec, sa, -1, nch, nc, scale, stop, half, nullptr, 1, sl, ns));
+            FGPE_RET(Launch<T>::last(c, spec, la, nch, nc, nullptr, filter, stop, half, nullptr, 1, sl, ns, vpx));
+            FGPE_RET(Launch<T>::strided(c, spec, sa, +1, nch, nc, nullptr, stop, half, nullptr, 1, sl, ns));
         }
     } else {
         FGPE_RET(Launch<T>::last(c, spec, 1, nch, nc, scale, filter, stop, half, nullptr, 1, 0, 1, vpx));
@@ -571,7 +576,8 @@ int adj_check(fgpe_ctx* c)
     FGPE_RET(ensure_ready(c));
     if (c->hp.ncomp != 1) return fail(FGPE_EUNSUP, "the adjoint is one-component");
     if (c->hp.quench_on) return fail(FGPE_EUNSUP, "the adjoint needs a constant a (no a(t) ramp)");
-    if (c->hp.sc_U != 0.0) return fail(FGPE_EUNSUP, "the adjoint supports the dipolar k-space kernel only");
+    if (c->hp.sc_U != 0.0 && c->hp.a_dd != 0.0)
+        return fail(FGPE_EUNSUP, "the adjoint takes one k-space kernel: dipolar or soft-core");
     if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
     if (c->hp.vpx[0] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
     if (c->hp.user_id != 0) return fail(FGPE_EUNSUP, "the adjoint does not know the user term's derivative");
@@ -728,8 +734,10 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
         s0 += h[2 * b];
         s1 += h[2 * b + 1];
     }
+    // s1 = sum <r, Phi[n]>: the kernel's strength (a_dd or U) scales Phi
     grads[0] = 4 * kPi * s0;
     grads[1] = c->hp.a_dd != 0.0 ? s1 / c->hp.a_dd : 0.0;
+    grads[2] = c->hp.sc_U != 0.0 ? s1 / c->hp.sc_U : 0.0;
     c->spec_valid = false;
     return FGPE_OK;
 }
@@ -743,17 +751,17 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
     FGPE_RET(ensure_ready(c));
     const int nc = c->hp.ncomp;
     const bool dens = c->nch > nc;
-    const int la = last_axis(c);
+    const int la0 = last_axis(c);
     hipEvent_t e0, e1;
     FGPE_HIPCHK(hipEventCreate(&e0));
     FGPE_HIPCHK(hipEventCreate(&e1));
     const bool timing = c->timing;
     c->timing = false;
-    int rc = FGPE_OK;
     auto run = [&](int cls) -> int {
+        const int la = kmul_axis(c);
         if (cls == FGPE_KCLASS_STRIDED) {
-            FGPE_RET(Launch<T>::strided(c, c->spec, 1, -1, c->nch, nc, nullptr, nullptr, true));
-            return Launch<T>::strided(c, c->spec, 1, +1, c->nch, nc, nullptr, nullptr, true);
+            FGPE_RET(Launch<T>::strided(c, c->spec, 3 - la, -1, c->nch, nc, nullptr, nullptr, true));
+            return Launch<T>::strided(c, c->spec, 3 - la, +1, c->nch, nc, nullptr, nullptr, true);
         }
         if (cls == FGPE_KCLASS_LAST)
             return Launch<T>::last(c, c->spec, la, c->nch, nc, nullptr, has_filter(c->hp), nullptr, true);
@@ -763,95 +771,128 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         a.stage = 2;
         return Launch<T>::xstage(c, a, nc, dens, MODE_RK);
     };
-    const int classes[3] = {FGPE_KCLASS_STRIDED, FGPE_KCLASS_LAST, FGPE_KCLASS_XSTAGE};
-    for (int ci = 0; ci < 3 && rc == FGPE_OK; ++ci) {
-        const int cls = classes[ci];
-        if (cls == FGPE_KCLASS_STRIDED && la != 2) continue;
-        if (cls == FGPE_KCLASS_LAST && la == 0) continue;
-        const int axis = cls == FGPE_KCLASS_STRIDED ? 1 : (cls == FGPE_KCLASS_LAST ? la : 0);
+    // average time of reps calls of f after one untimed call
+    auto timed = [&](auto&& f, double* per) -> int {
+        FGPE_RET(f());
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) FGPE_RET(f());
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) return fail(FGPE_EHIP, "autotune timing failed");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *per = ms / reps;
+        return FGPE_OK;
+    };
+    // the plan variants of one kernel class on one axis (the x class has a
+    // third where its length has one); keeps the fastest
+    double tt[7] = {};
+    auto tune = [&](int cls, int axis, int ci) -> int {
+        const int nv = cls == FGPE_KCLASS_XSTAGE && x_variant2(c->n[0]) ? 3 : 2;
         double best = 1e300;
         int bestv = 0;
-        for (int v = 0; v < 2 && rc == FGPE_OK; ++v) {
+        for (int v = 0; v < nv; ++v) {
             c->var[cls][axis] = v;
-            rc = run(cls);                                  // warm-up
-            if (rc != FGPE_OK) break;
-            (void)hipEventRecord(e0, c->stream);
-            for (int r = 0; r < reps && rc == FGPE_OK; ++r) rc = run(cls);
-            (void)hipEventRecord(e1, c->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per = ms / reps;
-            if (out) out[2 * ci + v] = per;
-            if (per < best) { best = per; bestv = v; }
+            double per = 0.0;
+            FGPE_RET(timed([&] { return run(cls); }, &per));
+            tt[v < 2 ? 2 * ci + v : 6] = per;
+            if (per < best) {
+                best = per;
+                bestv = v;
+            }
         }
         c->var[cls][axis] = bestv;
-    }
-    // column slabs of the y / last / y sequence (3D only): 1, 2 or 4, timed
-    // as whole RK stages (passes + x stage), since the x stage between two
-    // pass sequences decides what the memory-side cache still holds
-    if (la == 2 && rc == FGPE_OK) {
-        const int cand[3] = {1, 2, 4};
-        double best = 1e300;
-        int bests = 1;
-        auto stage = [&]() -> int {
-            FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
-            return run(FGPE_KCLASS_XSTAGE);
-        };
-        for (int k = 0; k < 3 && rc == FGPE_OK; ++k) {
-            c->nslab = cand[k];
-            int r = stage();   // warm-up
-            if (r == FGPE_EUNSUP) continue;   // slab width not a multiple of a tile width
-            if (r != FGPE_OK) {
-                rc = r;
-                break;
-            }
-            (void)hipEventRecord(e0, c->stream);
-            for (int i = 0; i < reps && rc == FGPE_OK; ++i) rc = stage();
-            (void)hipEventRecord(e1, c->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per = ms / reps;
-            if (out) out[6 + k] = per;
-            // More slabs have to win clearly: at 256^3 the timing of two slabs
-            // has tied with one slab here (0.05% apart) and then run 10% slower
-            // in a step, whose x stages differ from the timing mode's.
-            if (per < best * (cand[k] == 1 ? 1.0 : 0.95)) {
-                best = per;
-                bests = cand[k];
-            }
-        }
-        c->nslab = bests;
-    }
+        return FGPE_OK;
+    };
+    // one RK stage: the spectral passes and an x stage
+    auto stage = [&]() -> int {
+        FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
+        return run(FGPE_KCLASS_XSTAGE);
+    };
     // block order of the four passes of a stage (reversed or not per pass),
-    // timed the same way; keeps the fastest of the 16 combinations
-    if (rc == FGPE_OK) {
+    // timed as whole stages; keeps the fastest of the 16 combinations
+    auto tune_order = [&](double* best_ms) -> int {
         double best = 1e300;
         int besto = 0;
-        auto stage = [&]() -> int {
-            FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
-            return run(FGPE_KCLASS_XSTAGE);
-        };
-        for (int o = 0; o < 16 && rc == FGPE_OK; ++o) {
-            if (la != 2 && (o & 5)) continue;   // 2D grids have no y passes
-            if (la == 0 && (o & 2)) continue;   // 1D grids no last-axis pass
+        for (int o = 0; o < 16; ++o) {
+            if (la0 != 2 && (o & 5)) continue;   // 2D grids have no strided passes
+            if (la0 == 0 && (o & 2)) continue;   // 1D grids no last-axis pass
             c->order = o;
-            rc = stage();   // warm-up
-            if (rc != FGPE_OK) break;
-            (void)hipEventRecord(e0, c->stream);
-            for (int i = 0; i < reps && rc == FGPE_OK; ++i) rc = stage();
-            (void)hipEventRecord(e1, c->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms / reps < best) {
-                best = ms / reps;
+            double per = 0.0;
+            FGPE_RET(timed(stage, &per));
+            if (per < best) {
+                best = per;
                 besto = o;
             }
         }
         c->order = besto;
-    }
+        *best_ms = best;
+        return FGPE_OK;
+    };
+    auto body = [&]() -> int {
+        c->nslab = 1;
+        FGPE_RET(tune(FGPE_KCLASS_XSTAGE, 0, 2));
+        if (la0 == 1) FGPE_RET(tune(FGPE_KCLASS_LAST, 1, 1));
+        double best = 0.0;
+        if (la0 == 2) {
+            // 3D: the k-space multiply on the z pass or on the y pass.  Each
+            // choice gets its pass variants and block order tuned, then the
+            // faster whole stage wins (y by at least 1%).
+            double keep[7] = {};
+            int keep_order = 0;
+            bool besty = false;
+            best = 1e300;
+            for (int yl = 0; yl < 2; ++yl) {
+                c->ylast = yl != 0;
+                const int la = kmul_axis(c);
+                FGPE_RET(tune(FGPE_KCLASS_STRIDED, 3 - la, 0));
+                FGPE_RET(tune(FGPE_KCLASS_LAST, la, 1));
+                double per = 0.0;
+                FGPE_RET(tune_order(&per));
+                if (per < best * (yl ? 0.99 : 1.0)) {
+                    best = per;
+                    besty = yl != 0;
+                    keep_order = c->order;
+                    std::memcpy(keep, tt, sizeof(tt));
+                }
+            }
+            c->ylast = besty;
+            c->order = keep_order;
+            std::memcpy(tt, keep, sizeof(tt));
+        } else {
+            FGPE_RET(tune_order(&best));
+        }
+        if (out) {
+            for (int k = 0; k < 6; ++k) out[k] = tt[k];
+            out[9] = tt[6];
+        }
+        // column slabs of the strided / last / strided sequence (3D only): 1, 2
+        // or 4, timed as whole RK stages (passes + x stage), since the x stage
+        // between two pass sequences decides what the memory-side cache holds
+        if (la0 == 2) {
+            const int cand[3] = {1, 2, 4};
+            int bests = 1;
+            if (out) out[6] = best;
+            for (int k = 1; k < 3; ++k) {
+                c->nslab = cand[k];
+                double per = 0.0;
+                const int r = timed(stage, &per);
+                if (r == FGPE_EUNSUP) continue;   // slab width not a multiple of a tile width
+                FGPE_RET(r);
+                if (out) out[6 + k] = per;
+                // More slabs have to win clearly: at 256^3 the timing of two
+                // slabs has tied with one slab here (0.05% apart) and then run
+                // 10% slower in a step, whose x stages differ from the timing
+                // mode's.
+                if (per < best * 0.95) {
+                    best = per;
+                    bests = cand[k];
+                }
+            }
+            c->nslab = bests;
+        }
+        return FGPE_OK;
+    };
+    const int rc = body();
     c->timing = timing;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1063,7 +1104,8 @@ template <typename T> int group_autotune(fgpe_group* g, int reps, double* out)
         const int axis = cls == FGPE_KCLASS_STRIDED ? 1 : (cls == FGPE_KCLASS_LAST ? la : 0);
         double best = 1e300;
         int bestv = 0;
-        for (int v = 0; v < 2 && rc == FGPE_OK; ++v) {
+        const int nv = cls == FGPE_KCLASS_XSTAGE && x_variant2(c0->n[0]) ? 3 : 2;
+        for (int v = 0; v < nv && rc == FGPE_OK; ++v) {
             for (fgpe_ctx* c : g->c) c->var[cls][axis] = v;
             rc = group_launch<T>(g, t, dt, cls, MODE_RK);   // warm-up
             if (rc != FGPE_OK) break;
@@ -1074,7 +1116,7 @@ template <typename T> int group_autotune(fgpe_group* g, int reps, double* out)
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, e0, e1);
             const double per = ms / reps;
-            if (out) out[2 * ci + v] = per;
+            if (out) out[v < 2 ? 2 * ci + v : 9] = per;
             if (per < best) {
                 best = per;
                 bestv = v;
@@ -1579,7 +1621,8 @@ int fgpe_autotune(fgpe_ctx* c, int reps, double* host_ms)
 int fgpe_set_variant(fgpe_ctx* c, int kclass, int axis, int variant)
 {
     if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
-    if (kclass < 0 || kclass >= FGPE_NKCLASS || axis < 0 || axis > 2 || variant < 0 || variant > 1)
+    if (kclass < 0 || kclass >= FGPE_NKCLASS || axis < 0 || axis > 2 || variant < 0 ||
+        variant > (kclass == FGPE_KCLASS_XSTAGE ? 2 : 1))
         return fail(FGPE_EINVAL, "kclass, axis or variant out of range");
     c->var[kclass][axis] = variant;
     c->spec_valid = false;
@@ -1607,6 +1650,26 @@ int fgpe_get_order(fgpe_ctx* c, int* mask)
 {
     if (!c || !mask) return fail(FGPE_EINVAL, "NULL argument");
     *mask = c->order;
+    return FGPE_OK;
+}
+
+int fgpe_set_kmul_axis(fgpe_ctx* c, int axis)
+{
+    if (!c) return fail(FGPE_EINVAL, "ctx is NULL");
+    if (c->n[2] > 1) {
+        if (axis != 1 && axis != 2) return fail(FGPE_EINVAL, "3D grids take the k-space multiply on axis 1 or 2");
+    } else if (axis != last_axis(c)) {
+        return fail(FGPE_EINVAL, "a 1D or 2D grid has one k-space multiply axis");
+    }
+    c->ylast = c->n[2] > 1 && axis == 1;
+    c->spec_valid = false;
+    return FGPE_OK;
+}
+
+int fgpe_get_kmul_axis(fgpe_ctx* c, int* axis)
+{
+    if (!c || !axis) return fail(FGPE_EINVAL, "NULL argument");
+    *axis = kmul_axis(c);
     return FGPE_OK;
 }
 

@@ -30,10 +30,11 @@ constexpr double kPi = 3.141592653589793;  // hpp/header.hpp:28
 // pencils per block of the x kernels and tile width of the strided kernels
 // Variant 0: 128-thread x blocks, 128-byte row segments in the strided
 // passes; variant 1 (with its plan of more threads per pencil): 256-thread x
-// blocks and 256-byte row segments.
+// blocks and 256-byte row segments; variant 2 (x kernels only): 128-thread
+// x blocks.
 template <typename T, int N, int V = 0> __host__ __device__ constexpr int px_of()
 {
-    constexpr int th = V == 0 ? 128 : 256;
+    constexpr int th = V == 1 ? 256 : 128;
     return Plan<N, V>::T >= th ? 1 : th / Plan<N, V>::T;
 }
 template <typename T, int N, int V = 0> __host__ __device__ constexpr int ps_of()
@@ -196,7 +197,8 @@ struct TileAt {
     bool valid;         // this lane's column is stored
 };
 
-template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArgs& a, int p)
+template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArgs& a, int p, long long tile,
+                                                               long long ntiles)
 {
     // Tiles are enumerated outer index by outer index, all channels of one
     // outer index together (complex-layout tiles, then half-layout ones):
@@ -211,7 +213,7 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     const long long per = nfo + (a.hcols ? (long long)(a.nchan - a.ncomp) * nth : 0);
     // the tile this block takes: in launch order, or reversed so that the pass
     // starts on what the previous pass wrote last (still in the memory-side cache)
-    const long long bx = a.rev ? (long long)gridDim.x - 1 - blockIdx.x : (long long)blockIdx.x;
+    const long long bx = a.rev ? ntiles - 1 - tile : tile;
     r.outer = bx / per;
     const long long w = bx - r.outer * per;
     const bool half = w >= nfo;
@@ -248,7 +250,7 @@ k_strided(const c2_t<T>* __restrict__ twg, StridedArgs a0, const StridedArgs* __
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const gp<T2> data = glob((T2*)a.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
-    const TileAt at = strided_tile<P>(a, p);
+    const TileAt at = strided_tile<P>(a, p, blockIdx.x, gridDim.x);
     const gp<T2> base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
     T2 v[E];
@@ -463,7 +465,7 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
     const lds_table<T2> kA = (lds_table<T2>)kal;
     const gp<T2> data = glob((T2*)a.s.data);
     const int p = threadIdx.x % P, t = threadIdx.x / P;
-    const TileAt at = strided_tile<P>(a.s, p);
+    const TileAt at = strided_tile<P>(a.s, p, blockIdx.x, gridDim.x);
     const gp<T2> base = data + at.off;
     T2* my = lds + p * lds_stride<N>();
     // this column's k_x and outer-axis k pairs, loaded before the pencil so

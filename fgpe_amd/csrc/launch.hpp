@@ -164,9 +164,35 @@ template <typename T, int N, int V> int adj_n(fgpe_ctx* c, const AdjArgs& a, boo
     EXT template int detail::xstage_n<T, N, V>(fgpe_ctx*, const XArgs<T>&, int, bool, int, const void*, int);     \
     EXT template int detail::adj_n<T, N, V>(fgpe_ctx*, const AdjArgs&, bool, bool, bool);
 
+// The x-class launchers of the lengths with a distinct variant 2.
+#define FGPE_LAUNCH_INST_X(EXT, T, N, V)                                                                          \
+    EXT template int detail::xstage_n<T, N, V>(fgpe_ctx*, const XArgs<T>&, int, bool, int, const void*, int);     \
+    EXT template int detail::adj_n<T, N, V>(fgpe_ctx*, const AdjArgs&, bool, bool, bool);
+
+#define FGPE_X2_CHECK(N_) static_assert(Plan<N_, 2>::distinct, "FGPE_FOR_EACH_N_X2 lists a length without a third plan");
+FGPE_FOR_EACH_N_X2(FGPE_X2_CHECK)
+#undef FGPE_X2_CHECK
+#define FGPE_X2_CHECK(N_) static_assert(Plan<N_, 2>::distinct == x_variant2(N_), "x_variant2 out of date");
+FGPE_FOR_EACH_N(FGPE_X2_CHECK)
+#undef FGPE_X2_CHECK
+
 // Dispatch on the runtime length to the compiled one.
 #define FGPE_DISPATCH_CASE(N_) \
     case N_: return var ? CALL(N_, 1) : CALL(N_, 0);
+// The x class: variant 2 where the length has a third plan (else variant 1).
+#define FGPE_DISPATCH_X_CASE(N_)                                   \
+    case N_:                                                       \
+        if constexpr (Plan<N_, 2>::distinct) {                     \
+            if (var == 2) return CALL(N_, 2);                      \
+        }                                                          \
+        return var ? CALL(N_, 1) : CALL(N_, 0);
+#define FGPE_DISPATCH_X(len, variant)            \
+    {                                            \
+        const int var = (variant);               \
+        switch (len) {                           \
+            FGPE_FOR_EACH_N(FGPE_DISPATCH_X_CASE) \
+        }                                        \
+    }
 #define FGPE_DISPATCH(len, variant)          \
     {                                        \
         const int var = (variant);           \
@@ -178,7 +204,7 @@ template <typename T, int N, int V> int adj_n(fgpe_ctx* c, const AdjArgs& a, boo
 template <typename T> int Launch<T>::adj(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
 {
 #define CALL(N_, V_) detail::adj_n<T, N_, V_>(c, a, y64, dens, epi)
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
+    FGPE_DISPATCH_X(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }
@@ -186,7 +212,7 @@ template <typename T> int Launch<T>::adj(fgpe_ctx* c, const AdjArgs& a, bool y64
 template <typename T> long long Launch<T>::xblocks(fgpe_ctx* c)
 {
 #define CALL(N_, V_) detail::x_groups<px_of<T, N_, V_>()>(c, N_)
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
+    FGPE_DISPATCH_X(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
 #undef CALL
     return 0;
 }
@@ -225,7 +251,7 @@ template <typename T> int Launch<T>::xstage(fgpe_ctx* c, const XArgs<T>& a, int 
                                             const void* tab, int ninst)
 {
 #define CALL(N_, V_) detail::xstage_n<T, N_, V_>(c, a, nc, dens, mode, tab, ninst)
-    FGPE_DISPATCH(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
+    FGPE_DISPATCH_X(c->n[0], c->var[FGPE_KCLASS_XSTAGE][0])
 #undef CALL
     return fail(FGPE_EUNSUP, "unsupported nx");
 }

@@ -6,6 +6,9 @@ namespace fgpe {
 #define FGPE_DECL(N_) FGPE_LAUNCH_INST(extern, float, N_, 0) FGPE_LAUNCH_INST(extern, float, N_, 1)
 FGPE_FOR_EACH_N(FGPE_DECL)
 #undef FGPE_DECL
+#define FGPE_DECL(N_) FGPE_LAUNCH_INST_X(extern, float, N_, 2)
+FGPE_FOR_EACH_N_X2(FGPE_DECL)
+#undef FGPE_DECL
 }  // namespace fgpe
 
 template struct fgpe::Launch<float>;

@@ -11,4 +11,8 @@
 namespace fgpe {
 FGPE_LAUNCH_INST(, FGPE_INST_T, FGPE_INST_N, 0)
 FGPE_LAUNCH_INST(, FGPE_INST_T, FGPE_INST_N, 1)
+// the x class's third plan (the lengths of FGPE_FOR_EACH_N_X2)
+#if FGPE_INST_N == 128 || FGPE_INST_N == 256
+FGPE_LAUNCH_INST_X(, FGPE_INST_T, FGPE_INST_N, 2)
+#endif
 }  // namespace fgpe

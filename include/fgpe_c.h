@@ -266,15 +266,18 @@ int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------
    Gradients of a real objective J(psi(T)) of nsteps real-time RK4 steps with
-   respect to the trap V (N doubles), a and a_dd, by the discrete adjoint of
-   the RK4 step (one component, constant a, dipolar kernel, real V).
+   respect to the trap V (N doubles), a, and the strength of the k-space
+   kernel (a_dd or the soft-core U), by the discrete adjoint of the RK4 step
+   (one component, constant a, real V, at most one k-space kernel: dipolar or
+   soft-core; the soft-core radius and dimension stay fixed).
    fgpe_adjoint_forward runs the steps from the current state and keeps a
    checkpoint every ckpt_every steps (about sqrt(nsteps) keeps the memory at
    O(sqrt(nsteps)) states); psi(T) is then the current state.
    fgpe_adjoint_backward takes dev_lamT = dJ/dRe(psi) + i dJ/dIm(psi) at T
    (N complex doubles, device), recomputes each segment from its checkpoint,
    and writes dJ/dpsi(0) in the same convention to dev_lam0 (or NULL),
-   dJ/dV to dev_gradV and host_grad[0] = dJ/da, host_grad[1] = dJ/da_dd.
+   dJ/dV to dev_gradV and host_grad[0] = dJ/da, host_grad[1] = dJ/da_dd,
+   host_grad[2] = dJ/dU (3 doubles; 0 for the kernel that is off).
    The state is left at the start of the last segment. */
 int fgpe_adjoint_forward(fgpe_ctx* ctx, double dt, double t0, int nsteps, int ckpt_every);
 int fgpe_adjoint_backward(fgpe_ctx* ctx, const void* dev_lamT, void* dev_lam0, double* dev_gradV,
@@ -300,13 +303,20 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
    (16 elements per thread on long pencils, 128-thread x blocks, 128-byte row
    segments) and variant 1 (8 elements per thread where a second radix split
    exists, 256-thread x blocks, 256-byte row segments).  fgpe_autotune times
-   both for the strided (y), last-axis and x-stage classes on the context's
-   own buffers (reps launches each; the state is preserved) and keeps the
-   faster; then, on 3D grids, it times whole RK stages with the y / last / y
-   sequence split into 1, 2 or 4 column slabs (a slab's three passes run back
-   to back while it is in the memory-side cache) and keeps the fastest.  host_ms (9 doubles or
-   NULL) gets the times per class and variant (6), then per slab count (3;
-   0 where not applicable).  Needs the Hamiltonian, trap and state set. */
+   both for the x-stage, strided and last-axis classes on the context's own
+   buffers (reps launches each; the state is preserved) and keeps the faster.
+   It then times whole RK stages: the 16 block orders (fgpe_set_order), on 3D
+   grids for each choice of the k-space multiply axis (fgpe_set_kmul_axis,
+   with that choice's pass variants tuned first), keeping the fastest, and
+   last the y / last / y sequence split into 1, 2 or 4 column slabs (a slab's
+   three passes run back to back while it is in the memory-side cache).
+   The x-stage class has a third plan, variant 2 (4 elements per thread in
+   radix-4 stages), on nx = 128 and 256; elsewhere variant 2 runs variant 1.
+   host_ms (10 doubles or NULL) gets the times per class and variant 0/1 of
+   the chosen configuration (6: strided, last-axis, x stage), then per slab
+   count (3; 0 where not applicable), then the x stage's variant 2 (0 where
+   the length has none).  Needs the Hamiltonian, trap and state set.
+   fgpe_set_variant takes 0 or 1, and 2 for FGPE_KCLASS_XSTAGE. */
 int fgpe_autotune(fgpe_ctx* ctx, int reps, double* host_ms);
 int fgpe_set_variant(fgpe_ctx* ctx, int kclass, int axis, int variant);
 int fgpe_get_variant(fgpe_ctx* ctx, int kclass, int axis, int* variant);
@@ -320,6 +330,15 @@ int fgpe_get_slabs(fgpe_ctx* ctx, int* nslab);
    whole RK stages). */
 int fgpe_set_order(fgpe_ctx* ctx, int mask);
 int fgpe_get_order(fgpe_ctx* ctx, int* mask);
+/* The axis whose pass carries the k-space multiply (the forward FFT, the
+   kinetic and k-space-kernel factors and the inverse FFT in one pass): on a
+   3D grid 2 (z, the default; y runs the strided forward and inverse passes)
+   or 1 (y; z runs them), on a 2D grid 1, on a 1D grid 0.  fgpe_autotune picks
+   it on 3D grids (each choice's pass variants tuned, one RK stage timed per
+   choice).  The two 3D choices agree to rounding (the y and z transforms run
+   in the other order).  Batched groups always use z. */
+int fgpe_set_kmul_axis(fgpe_ctx* ctx, int axis);
+int fgpe_get_kmul_axis(fgpe_ctx* ctx, int* axis);
 
 /* ---- batched sweeps (no counterpart in the reference, which runs one
    instance per process: src/oneComponentGPSolver.cu:26-104) ----------------
@@ -333,8 +352,8 @@ typedef struct fgpe_group fgpe_group;
 int fgpe_group_create(fgpe_ctx* const* ctxs, int n, fgpe_group** out);
 int fgpe_group_destroy(fgpe_group* group);
 /* Times both plan variants of each kernel class as batched launches and sets
-   the faster on every context (host_ms: 9 doubles or NULL, as fgpe_autotune;
-   groups run unslabbed, the last 3 stay 0).  A heavy-ball run in progress
+   the faster on every context (host_ms: 10 doubles or NULL, as fgpe_autotune;
+   groups run unslabbed, the slab slots stay 0).  A heavy-ball run in progress
    ends here (its pending renormalisation is applied); the next
    fgpe_group_imag_steps starts a fresh run from the normalised states. */
 int fgpe_group_autotune(fgpe_group* group, int reps, double* host_ms);

@@ -36,18 +36,29 @@ def J_of(psiT, target, dv):
     return float(abs(ov) ** 2), ov
 
 
-def gpu_forward(c, V, a, a_dd, psi0, dt, n, prec=0):
+SC = (3, 1.2)   # soft-core dimension and radius of the soft-core cases
+
+
+def hamiltonian(ctx, a, s, softcore):
+    """a_dd = s (dipolar along z) or, with softcore, the soft-core U = s."""
+    if softcore:
+        ctx.set_hamiltonian(a=(a,), sc_U=s, sc_R=SC[1], sc_dim=SC[0])
+    else:
+        ctx.set_hamiltonian(a=(a,), a_dd=s, dip=(0.0, 0.0, 1.0))
+
+
+def gpu_forward(c, V, a, a_dd, psi0, dt, n, prec=0, softcore=False):
     with c.ctx(prec) as ctx:
-        ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0))
+        hamiltonian(ctx, a, a_dd, softcore)
         ctx.upload_vext(V.astype(np.complex128))
         ctx.upload_psi(psi0)
         ctx.rk4_steps(dt, n)
         return ctx.download_psi()
 
 
-def gpu_gradient(torch, c, V, a, a_dd, psi0, target, dt, n, every, prec=0):
+def gpu_gradient(torch, c, V, a, a_dd, psi0, target, dt, n, every, prec=0, softcore=False):
     with c.ctx(prec) as ctx:
-        ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0))
+        hamiltonian(ctx, a, a_dd, softcore)
         ctx.upload_vext(V.astype(np.complex128))
         ctx.upload_psi(psi0)
         ctx.adjoint_forward(dt, n, every)
@@ -139,3 +150,103 @@ def test_adjoint_mixed_precision(torch_cuda):
     assert rel < 1e-4
     assert abs(s32[0] - s64[0]) <= 1e-4 * abs(s64[0])
     assert abs(s32[1] - s64[1]) <= 1e-4 * abs(s64[1])
+
+
+def test_softcore_adjoint_against_finite_differences(torch_cuda):
+    """The soft-core kernel (a_dd = 0): dJ/dV, dJ/da and dJ/dU against central
+    differences of the HIP forward, dJ/dV also against the CPU oracle's."""
+    c, V, psi0, target, a, _ = setup_case()
+    U = 1.5
+    dt, n, every = 5e-4, 30, 6
+    J, gV, g, _ = gpu_gradient(torch_cuda, c, V, a, U, psi0, target, dt, n, every, softcore=True)
+    assert g[1] == 0.0
+    h = 1e-4
+    for j in range(2):
+        dV = smooth_field(c, 300 + j)
+        Jp = J_of(gpu_forward(c, V + h * dV, a, U, psi0, dt, n, softcore=True), target, c.dv)[0]
+        Jm = J_of(gpu_forward(c, V - h * dV, a, U, psi0, dt, n, softcore=True), target, c.dv)[0]
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(np.sum(gV * dV))
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (j, ad, fd)
+    for k, (pa, pu) in ((0, (h, 0.0)), (2, (0.0, h))):
+        Jp = J_of(gpu_forward(c, V, a + pa, U + pu, psi0, dt, n, softcore=True), target, c.dv)[0]
+        Jm = J_of(gpu_forward(c, V, a - pa, U - pu, psi0, dt, n, softcore=True), target, c.dv)[0]
+        fd = (Jp - Jm) / (2 * h)
+        assert abs(g[k] - fd) <= 1e-6 * abs(fd), (k, g[k], fd)
+    dV = smooth_field(c, 310)
+    Js = []
+    for sgn in (1, -1):
+        hm = orc.Ham(1, (a,), vtilde=orc.vsoftcore(c.grid, SC[0], SC[1], U))
+        out = orc.rk4(c.grid, hm, psi0, (V + sgn * h * dV).astype(np.complex128), dt, n)
+        Js.append(J_of(out, target, c.dv)[0])
+    fd = (Js[0] - Js[1]) / (2 * h)
+    ad = float(np.sum(gV * dV))
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+
+
+def test_softcore_adjoint_through_autograd(torch_cuda):
+    torch = torch_cuda
+    from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
+    c, V, psi0, target, a, _ = setup_case()
+    U, dt, n = 1.5, 5e-4, 20
+    solver = AdjointSolver(c.spec, dt, n, torch=torch, softcore=SC)
+    Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
+    at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    ut = torch.tensor(U, dtype=torch.float64, requires_grad=True)
+    p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda", requires_grad=True)
+    tg = torch.tensor(target, dtype=torch.complex128, device="cuda")
+    loss = fidelity(GPEEvolve().apply(Vt, at, ut, p0, solver), tg, c.dv)
+    loss.backward()
+    J, gV, g, lam0 = gpu_gradient(torch, c, V, a, U, psi0, target, dt, n, solver.every, softcore=True)
+    assert abs(loss.item() - J) < 1e-14
+    assert np.abs(Vt.grad.cpu().numpy() - gV).max() <= 1e-13 * np.abs(gV).max()
+    assert abs(at.grad.item() - g[0]) <= 1e-13 * abs(g[0])
+    assert abs(ut.grad.item() - g[2]) <= 1e-13 * abs(g[2])
+    solver.ctx.close()
+
+
+def test_adjoint_refuses_two_kernels():
+    from fgpe_amd import FgpeError
+    c, V, psi0, _, a, a_dd = setup_case()
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0), sc_U=1.0, sc_R=1.2, sc_dim=3)
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        with pytest.raises(FgpeError):
+            ctx.adjoint_forward(5e-4, 4, 2)
+
+
+def test_adjoint_x_stage_third_plan(torch_cuda):
+    """The adjoint's x kernels with the x class's third plan (nx = 128):
+    the same gradient as with variant 1 (to rounding) and central differences."""
+    c = Case(128, 16, 32, L=(16.0, 8.0, 10.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25)).real
+    psi0 = configs.normalise(np.exp(-0.5 * configs.harmonic_trap(c.spec, (1, 1, 0.5)).real) +
+                             0.01 * c.rand_psi(43), c.dv)
+    target = configs.normalise(np.exp(-0.6 * configs.harmonic_trap(c.spec, (1, 1, 0.4)).real) *
+                               np.exp(0.2j * c.rand_psi(44).real), c.dv)
+    a, a_dd, dt, n, every = 0.6, 0.5, 5e-4, 40, 7
+    grads = []
+    for variant in (1, 2):
+        with c.ctx() as ctx:
+            ctx.set_variant(0, 0, variant)
+            ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0))
+            ctx.upload_vext(V.astype(np.complex128))
+            ctx.upload_psi(psi0)
+            ctx.adjoint_forward(dt, n, every)
+            J, ov = J_of(ctx.download_psi(), target, c.dv)
+            lamT = torch_cuda.from_numpy(2 * ov * c.dv * target).cuda()
+            gV = torch_cuda.empty(c.shape, dtype=torch_cuda.float64, device="cuda")
+            g = ctx.adjoint_backward(lamT.data_ptr(), 0, gV.data_ptr())
+            ctx.sync()
+            grads.append((gV.cpu().numpy(), g))
+    (g1, s1), (g2, s2) = grads
+    assert np.abs(g2 - g1).max() <= 1e-10 * np.abs(g1).max()
+    assert abs(s2[0] - s1[0]) <= 1e-10 * abs(s1[0]) and abs(s2[1] - s1[1]) <= 1e-10 * abs(s1[1])
+    h = 1e-4
+    dV = smooth_field(c, 400)
+    Jp = J_of(gpu_forward(c, V + h * dV, a, a_dd, psi0, dt, n), target, c.dv)[0]
+    Jm = J_of(gpu_forward(c, V - h * dV, a, a_dd, psi0, dt, n), target, c.dv)[0]
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(np.sum(g2 * dV))
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)

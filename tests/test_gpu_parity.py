@@ -602,3 +602,147 @@ def test_two_component_imaginary_time_with_softcore(prec):
         ctx.imag_steps(1e-3, 30, scheme=FGPE_IMAG_RK4_RENORM, npart=npart)
         err = rel_max(ctx.download_psi(), ref)
     assert err < (1e-12 if prec == 0 else 1e-6), err
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["dip+sc", "moving"])
+def test_kmul_on_y_pass(precision, name):
+    """3D grids with the k-space multiply on the y pass (z pencils run the
+    strided forward and inverse passes): RK4 real time and renormalised
+    imaginary time against the oracle (FP64 1e-12, mixed 1e-6), for both plan
+    variants of the passes."""
+    from fgpe_amd import configs
+    c = Case(64, 32, 128, L=(10.0, 8.0, 20.0))
+    h = HAMS[name]
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(configs.harmonic_trap(c.spec, (1, 1, 0.5)))) +
+                            0.01 * c.rand_psi(71), c.dv)
+    tol = 1e-12 if precision == 0 else 1e-6
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 3)
+    for variant in (0, 1):
+        with c.ctx(precision) as ctx:
+            for k in range(5):
+                for axis in range(3):
+                    ctx.set_variant(k, axis, variant)
+            ctx.set_kmul_axis(1)
+            assert ctx.kmul_axis() == 1
+            ctx.set_hamiltonian(**h)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            ctx.rk4_steps(5e-4, 3)
+            err = rel_max(ctx.download_psi(), ref)
+            assert err < tol, (variant, err)
+    if name == "dip+sc":
+        with c.ctx(precision) as ctx:
+            ctx.set_kmul_axis(1)
+            ctx.set_hamiltonian(**h)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            ctx.imag_steps(1e-3, 4)
+            gpu = ctx.download_psi()
+        refi = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 1e-3, 4, imaginary=True, npart=1.0)
+        err = rel_max(gpu, refi)
+        assert err < tol, err
+
+
+def test_kmul_axis_choices():
+    """A 3D grid takes the multiply on axis 1 or 2; 2D and 1D grids have one choice."""
+    from fgpe_amd import FgpeError
+    c3, c2, c1 = Case(32, 32, 32), Case(64, 32, 1), Case(64, 1, 1)
+    with c3.ctx() as ctx:
+        assert ctx.kmul_axis() == 2
+        ctx.set_kmul_axis(1)
+        assert ctx.kmul_axis() == 1
+        ctx.set_kmul_axis(2)
+        assert ctx.kmul_axis() == 2
+        with pytest.raises(FgpeError):
+            ctx.set_kmul_axis(0)
+    with c2.ctx() as ctx:
+        assert ctx.kmul_axis() == 1
+        ctx.set_kmul_axis(1)
+        with pytest.raises(FgpeError):
+            ctx.set_kmul_axis(2)
+    with c1.ctx() as ctx:
+        assert ctx.kmul_axis() == 0
+        with pytest.raises(FgpeError):
+            ctx.set_kmul_axis(1)
+
+
+XSTAGE = 0   # FGPE_KCLASS_XSTAGE
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("shape", [(256, 32, 32), (128, 32, 64), (256, 64, 1), (128, 128, 1)])
+def test_x_stage_third_plan(precision, shape):
+    """The x stage's third plan (4 elements per thread, radix-4 stages; at
+    nx = 256 one wave per pencil): dipolar + soft-core RK4 against the oracle
+    (FP64 1e-12, mixed 1e-6), and in FP64 the heavy ball with its on-device
+    history (1e-11)."""
+    from fgpe_amd import FGPE_IMAG_HEAVY_BALL, configs
+    c = Case(*shape, L=(16.0, 8.0, 8.0) if shape[2] > 1 else (16.0, 12.0, 1.0))
+    h = dict(HAMS["dip+sc"])
+    if shape[2] == 1:
+        h.update(sc_dim=2, dip=(1.0, 0.0, 0.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.5))
+    psi = configs.normalise(np.exp(-0.5 * np.abs(configs.harmonic_trap(c.spec, (1, 1, 0.5)))) +
+                            0.01 * c.rand_psi(81), c.dv)
+    with c.ctx(precision) as ctx:
+        ctx.set_variant(XSTAGE, 0, 2)
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ctx.rk4_steps(5e-4, 3)
+        gpu = ctx.download_psi()
+    ref = orc.rk4(c.grid, oracle_ham(c, h), psi, V, 5e-4, 3)
+    err = rel_max(gpu, ref)
+    assert err < (1e-12 if precision == 0 else 1e-6), err
+    if precision == 1:
+        return
+    dt, beta, npart, eps, n = 2e-3, 0.9, 1.0, 1e-7, 40
+    ref, nref, href = orc.imag_heavy_ball(c.grid, oracle_ham(c, h), psi, V, dt, beta, npart, eps, n)
+    with c.ctx() as ctx:
+        ctx.set_variant(XSTAGE, 0, 2)
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        done, hist = ctx.imag_steps(dt, n, scheme=FGPE_IMAG_HEAVY_BALL, beta=beta, npart=npart,
+                                    eps=eps, history=True)
+        gpu = ctx.download_psi()
+    assert done == nref, (done, nref)
+    assert rel_max(gpu, ref) < 1e-11
+    assert np.abs(hist - href).max() <= 1e-11 * np.abs(href).max()
+
+
+def test_x_stage_third_plan_two_components():
+    from fgpe_amd import configs
+    c = Case(128, 16, 32, L=(16.0, 8.0, 8.0))
+    V1 = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.5))
+    V = np.stack([V1, V1 + 0.1])
+    p1 = configs.gaussian_noisy(c.spec, 7, offset=(1.0, 0, 0), noise=0.02)
+    p2 = configs.gaussian_noisy(c.spec, 8, offset=(-1.0, 0, 0), noise=0.02)
+    psi = np.stack([p1, p2 * np.exp(0.3j)])
+    f = 1.0 / (4 * np.pi)
+    h = dict(ncomp=2, a=(100 * f, 100 * f, 80 * f), sc_U=5.0, sc_R=1.5, sc_dim=3)
+    with c.ctx() as ctx:
+        ctx.set_variant(XSTAGE, 0, 2)
+        ctx.set_hamiltonian(**h)
+        ctx.upload_psi(psi)
+        ctx.upload_vext(V)
+        ctx.rk4_steps(5e-4, 4)
+        gpu = ctx.download_psi()
+    ref = orc.rk4(c.grid, two_comp_ham(c, h), psi, V, 5e-4, 4)
+    err = rel_max(gpu, ref)
+    assert err < 1e-12, err
+
+
+def test_x_stage_variant_range():
+    """Variant 2 exists for the x-stage class only; on a length without a
+    third plan it runs variant 1."""
+    from fgpe_amd import FgpeError
+    c = Case(64, 32, 32)
+    with c.ctx() as ctx:
+        ctx.set_variant(XSTAGE, 0, 2)
+        with pytest.raises(FgpeError):
+            ctx.set_variant(1, 1, 2)
+        with pytest.raises(FgpeError):
+            ctx.set_variant(XSTAGE, 0, 3)
