@@ -538,7 +538,7 @@ def main():
         v = f64["tune"]["variant"]["xstage"]
         # the 256^3 working set exceeds the 256 MB memory-side cache, so the
         # x stage runs its non-temporal form (last template argument true)
-        traffic = pmc_traffic(f"fgpe::k_xstage<double, 256, {v}, 1, true, 1, true>")
+        traffic = pmc_traffic(f"fgpe::k_xstage<double, 256, {v}, 1, true, 1, true, 0>")
     step_gbs = roofline.step_bytes_per_point(False, 1, True, nx) * N * steps_s / 1e9
     mixed_steps = 1000.0 / mixed["ms"]
     l2 = float(np.sqrt(np.sum(np.abs(mixed["psi"] - f64["psi"]) ** 2) / np.sum(np.abs(f64["psi"]) ** 2)))

@@ -590,9 +590,14 @@ template <typename T> struct XArgs {
 // Minimum waves per SIMD the register allocator targets for an x-stage
 // kernel: 4 for the FP32 RK stages of the variant-1 plans up to N = 256
 // (256: 138 -> 128 VGPRs with 3 spilled, 9% faster; shorter pencils fit
-// without spills); 1 (no target) elsewhere, where forcing it spills heavily.
+// without spills); for the one-component RK stages of the variant-2 plans
+// (4 elements per thread) 7 in FP32 (N = 256: 81 -> 70 VGPRs) and 5 in FP64
+// (108 -> 96 VGPRs), none spilled (at 256^3 the x stage went from 345 to
+// 335 us FP64 and from 191 to 181 us FP32); 1 (no target) elsewhere, where
+// forcing it spills heavily.
 template <typename T, int N, int V, int NC, int MODE> constexpr int xstage_waves()
 {
+    if constexpr (V == 2 && NC == 1 && MODE == MODE_RK) return sizeof(T) == 4 ? 7 : 5;
     return (sizeof(T) == 4 && V == 1 && NC == 1 && MODE == MODE_RK && N <= 256 && Plan<N, V>::E <= 8) ? 4 : 1;
 }
 
