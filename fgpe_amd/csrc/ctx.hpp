@@ -237,6 +237,7 @@ inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int 
 {
     LastArgs a;
     a.s = strided_args(c, data, axis, scale, nch, ncomp, half, stop);
+    a.s.gather = 1;
     const int oaxis = axis == 1 ? 2 : 1;
     a.kA32 = c->k32[axis];
     a.kX32 = c->k32[0];

@@ -17,33 +17,60 @@ using namespace fgpe;
 
 namespace fgpe {
 
-__global__ void __launch_bounds__(256) k_finalize(FinArgs f0, const FinArgs* __restrict__ tab)
+__global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs f0, const FinArgs* __restrict__ tab)
 {
-    __shared__ double sh[6][256];
     const FinArgs f = args_of(f0, tab, blockIdx.x);
     if (f.stop && *glob(f.stop)) return;
     const gp<const double> partials = glob(f.partials);
     const gp<double> out = glob(f.out), scale = glob(f.scale), mu_out = glob(f.mu), eprev = glob(f.eprev);
     const int t = threadIdx.x;
     const int nv = 3 * f.nc;
+    // thread t sums the partials of blocks t, t + n, t + 2n, ... in that order
+    // (n = kFinThreads); four of them are loaded per round, so that the loads
+    // are in flight together rather than one L2 round trip after another
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int b = t; b < f.nblocks; b += 256)
-        for (int q = 0; q < nv; ++q) s[q] += partials[(long long)b * nv + q];
-    for (int q = 0; q < 6; ++q) sh[q][t] = s[q];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w)
-            for (int q = 0; q < 6; ++q) sh[q][t] += sh[q][t + w];
-        __syncthreads();
+    constexpr int n = kFinThreads;
+    for (int b = t; b < f.nblocks; b += 4 * n) {
+        double x[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                x[u][q] = (q < nv && b + u * n < f.nblocks) ? partials[(long long)(b + u * n) * nv + q] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s[q] += x[u][q];
     }
+    // the block's sum in a fixed order: wave64 shuffles, then the waves in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s[q] += __shfl_down(s[q], off, 64);
+    __shared__ double red[6][kFinThreads / 64];
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) red[q][t >> 6] = s[q];
+    __syncthreads();
     if (t != 0) return;
+    double S[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double a = 0.0;
+        for (int w = 0; w < kFinThreads / 64; ++w) a += red[q][w];
+        S[q] = a;
+    }
     bool conv = true;
-    for (int c = 0; c < f.nc; ++c) {
-        const double norm = sh[3 * c][0] * f.dv;
+    // (unrolled over the two possible components, so that every index is a
+    // constant: a runtime one would put S and the argument struct in scratch)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (c >= f.nc) break;
+        const double norm = S[3 * c] * f.dv;
         if (f.kind == FIN_OBS) {
             out[3 * c] = norm;
-            out[3 * c + 1] = sh[3 * c + 1][0] * f.dv / norm;
-            out[3 * c + 2] = sh[3 * c + 2][0] * f.dv / norm;
+            out[3 * c + 1] = S[3 * c + 1] * f.dv / norm;
+            out[3 * c + 2] = S[3 * c + 2] * f.dv / norm;
         } else if (f.kind == FIN_RENORM) {
             // the RK4 scheme evaluates no energy operator: only the norm is
             // recorded, the mu and E slots say so
@@ -54,8 +81,8 @@ __global__ void __launch_bounds__(256) k_finalize(FinArgs f0, const FinArgs* __r
         } else {
             const double sc = sqrt(f.npart[c] / norm);
             const double narg = f.nc == 1 ? f.npart[c] : norm;
-            const double mu = sc * sh[3 * c + 1][0] * f.dv / narg;
-            const double en = sc * sh[3 * c + 2][0] * f.dv / narg;
+            const double mu = sc * S[3 * c + 1] * f.dv / narg;
+            const double en = sc * S[3 * c + 2] * f.dv / narg;
             out[3 * c] = norm;
             out[3 * c + 1] = mu;
             out[3 * c + 2] = en;
@@ -313,7 +340,7 @@ int finalize(fgpe_ctx* c, long long nblocks, int kind, const double* npart, doub
 {
     const FinArgs f = fin_args(c, nblocks, kind, npart, out, eps);
     TScope ts(c, FGPE_KCLASS_OTHER);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, f, (const FinArgs*)nullptr);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, c->stream, f, (const FinArgs*)nullptr);
     FGPE_LAUNCHCHK("k_finalize");
     return FGPE_OK;
 }
@@ -1023,7 +1050,7 @@ template <typename T> int group_launch(fgpe_group* g, const GroupTabs& t, const 
         FGPE_RET(Launch<T>::xstage(c0, XArgs<T>{}, nc, dens, mode, dtab + t.o_x, n));
     if (cls < 0) {
         TScope ts(c0, FGPE_KCLASS_OTHER);
-        hipLaunchKernelGGL(k_finalize, dim3((unsigned)n), dim3(256), 0, c0->stream, FinArgs{},
+        hipLaunchKernelGGL(k_finalize, dim3((unsigned)n), dim3(kFinThreads), 0, c0->stream, FinArgs{},
                            (const FinArgs*)(dtab + t.o_f));
         FGPE_LAUNCHCHK("k_finalize");
     }

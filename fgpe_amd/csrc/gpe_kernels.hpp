@@ -175,17 +175,30 @@ struct StridedArgs {
     const double* scale;    // per-component pending scale (forward only) or null
     const int* stop;        // nonzero: skip (a converged imaginary-time loop)
     int rev;                // 1: blocks take the tiles in reverse order
+    int gather;             // 1: the half layout's leftover columns go to remainder tiles (below)
 };
 
 // Blocks of one strided launch: per outer index the complex-layout tiles,
-// then the half-layout ones.  A last partial tile of either layout has spare
-// lanes that compute on zeros and store nothing.
+// then the half-layout ones.  A half-layout row has hcols = nx/2 + 1
+// columns, so hcols % P of them (the Nyquist column kx = nx/2, for P
+// dividing nx/2) are left over per outer index.  Without `gather` each outer
+// index spends a partial tile on them; with it, the per-outer-index
+// half-layout tiles are the full ones only, and remainder tiles after all
+// outer indices gather P leftover columns of consecutive outer indices (and
+// channels), one per lane.  The last-axis pass gathers (its partial tiles run
+// two transforms for one column; measured at 256^3, FP32: 72 -> 69 us); the
+// y passes do not (their remainder pencils stride whole planes apart, and
+// the slow tiles at the end of the launch cost more than they save: 128^3
+// batched, 131 -> 151 us).  A partial tile's spare lanes compute on zeros and
+// store nothing.
 template <int P> __host__ __device__ inline long long strided_blocks(const StridedArgs& a)
 {
     const long long ntf = (a.ncols + P - 1) / P;
     if (a.hcols == 0) return (long long)a.nchan * ntf * a.nouter;
-    const long long nth = (a.hcols + P - 1) / P;
-    return ((long long)a.ncomp * ntf + (long long)(a.nchan - a.ncomp) * nth) * a.nouter;
+    const long long nreal = a.nchan - a.ncomp;
+    if (!a.gather) return ((long long)a.ncomp * ntf + nreal * ((a.hcols + P - 1) / P)) * a.nouter;
+    const long long nhf = a.hcols / P, hr = a.hcols % P;
+    return ((long long)a.ncomp * ntf + nreal * nhf) * a.nouter + (nreal * a.nouter * hr + P - 1) / P;
 }
 
 struct TileAt {
@@ -203,22 +216,45 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     // Tiles are enumerated outer index by outer index, all channels of one
     // outer index together (complex-layout tiles, then half-layout ones):
     // the x stage writes both channels of a row together, so a pass that
-    // starts where it ended finds both in the memory-side cache.  Selects,
-    // not branches: every value is block-uniform.
+    // starts where it ended finds both in the memory-side cache; the
+    // remainder tiles come last.  Selects, not branches: every value but those
+    // of a remainder tile is block-uniform.
     TileAt r;
     const int ntf = (a.ncols + P - 1) / P;
-    const int nth = a.hcols ? (a.hcols + P - 1) / P : 1;
+    // half-layout tiles per outer index and channel, and (gather) the columns
+    // per outer index and channel left to the remainder tiles
+    const int nhf = a.gather ? a.hcols / P : (a.hcols + P - 1) / P;
+    const int hr = a.gather ? a.hcols - nhf * P : 0;
     const int nfc = a.hcols ? a.ncomp : a.nchan;          // complex-layout channels
     const long long nfo = (long long)nfc * ntf;           // their tiles per outer index
-    const long long per = nfo + (a.hcols ? (long long)(a.nchan - a.ncomp) * nth : 0);
+    const long long per = nfo + (a.hcols ? (long long)(a.nchan - a.ncomp) * nhf : 0);
     // the tile this block takes: in launch order, or reversed so that the pass
     // starts on what the previous pass wrote last (still in the memory-side cache)
     const long long bx = a.rev ? ntiles - 1 - tile : tile;
+    const long long nmain = per * a.nouter;
+    if (bx >= nmain) {
+        // remainder tile: lane p takes remainder pencil g (channel-major, then
+        // outer index, then column)
+        const long long npen = (long long)(a.nchan - a.ncomp) * a.nouter * hr;
+        const long long g0 = (bx - nmain) * P + p;
+        r.valid = g0 < npen;
+        const long long g = r.valid ? g0 : 0;
+        const long long perch = (long long)a.nouter * hr;
+        const int cl = (int)(g / perch);
+        const long long w = g - cl * perch;
+        r.outer = w / hr;
+        r.ix = a.hx0 + nhf * P + (int)(w - r.outer * hr);
+        r.ch = a.ncomp + cl;
+        r.sp_off = r.outer * a.outer_stride + r.ix;
+        r.off = a.hbase + cl * a.hchan + r.outer * a.houter + r.ix;
+        r.st = a.hstride;
+        return r;
+    }
     r.outer = bx / per;
     const long long w = bx - r.outer * per;
     const bool half = w >= nfo;
     const long long wl = half ? w - nfo : w;
-    const int nt = half ? nth : ntf;
+    const int nt = half ? nhf : ntf;
     const int cl = (int)(wl / nt);            // channel within its layout
     const int xt = (int)(wl - (long long)cl * nt);
     r.ch = half ? a.ncomp + cl : cl;
@@ -1147,7 +1183,8 @@ struct FinArgs {
     double eps;
 };
 
-// one block per instance (blockIdx.x indexes tab when it is non-null)
+// one block of kFinThreads threads per instance (blockIdx.x indexes tab when it is non-null)
+constexpr int kFinThreads = 1024;
 __global__ void k_finalize(FinArgs f0, const FinArgs* __restrict__ tab);   // defined in fgpe_capi.hip
 
 template <typename TD, typename TS>
