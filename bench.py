@@ -172,6 +172,7 @@ def adjoint_leg(torch, nsteps, every, ndirs):
         c.set_hamiltonian(a=(ax,), a_dd=addx, dip=(0.0, 0.0, 1.0))
         c.upload_vext(Vx.astype(np.complex128))
         c.upload_psi(psi0)
+        c.autotune(3)   # plan variants and pass order for this grid (the state is left as uploaded)
         return c
 
     def J(psiT):

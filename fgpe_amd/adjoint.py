@@ -36,7 +36,8 @@ class AdjointSolver:
     synchronisation."""
 
     def __init__(self, grid, dt, nsteps, ckpt_every=None, dip=(0.0, 0.0, 1.0), precision=FGPE_PREC_F64,
-                 torch=None, softcore=None):
+                 torch=None, softcore=None, autotune=True):
+        self.tune = autotune   # run the plan autotuner once, at the first forward
         self.grid = grid
         self.dt = float(dt)
         self.nsteps = int(nsteps)
@@ -56,11 +57,17 @@ class AdjointSolver:
             dim, R = self.softcore
             self.ctx.set_hamiltonian(a=(float(a),), sc_U=float(s), sc_R=float(R), sc_dim=int(dim))
 
+    def _autotune_once(self):
+        if self.tune:
+            self.ctx.autotune(3)   # leaves the uploaded state as it is
+            self.tune = False
+
     def configure(self, V, a, a_dd, psi0):
         """V, psi0 as host arrays; a_dd: the kernel strength (U for soft-core)."""
         self._hamiltonian(a, a_dd)
         self.ctx.upload_vext(np.asarray(V, dtype=np.float64).astype(np.complex128))
         self.ctx.upload_psi(np.asarray(psi0, dtype=np.complex128))
+        self._autotune_once()
 
     def forward(self, V, a, a_dd, psi0):
         """Host arrays in, psi(T) as a host array."""
@@ -77,6 +84,7 @@ class AdjointSolver:
         self._hamiltonian(a, a_dd)
         self.ctx.upload_vext_real_dev(V.data_ptr())
         self.ctx.upload_psi_dev(psi0.data_ptr())
+        self._autotune_once()
         self.ctx.adjoint_forward(self.dt, self.nsteps, self.every)
         out = torch.empty_like(psi0)
         self.ctx.download_psi_dev(out.data_ptr())

@@ -602,7 +602,6 @@ int adj_check(fgpe_ctx* c)
 {
     FGPE_RET(ensure_ready(c));
     if (c->hp.ncomp != 1) return fail(FGPE_EUNSUP, "the adjoint is one-component");
-    if (c->hp.quench_on) return fail(FGPE_EUNSUP, "the adjoint needs a constant a (no a(t) ramp)");
     if (c->hp.sc_U != 0.0 && c->hp.a_dd != 0.0)
         return fail(FGPE_EUNSUP, "the adjoint takes one k-space kernel: dipolar or soft-core");
     if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
@@ -679,7 +678,9 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
     const double one[2] = {1.0, 1.0};
     const double2 cc = double2{0.0, dt};   // conj(c), c = -i dt
     auto scl = [](double s, double2 z) { return double2{s * z.x, s * z.y}; };
-    const double g4 = 4 * kPi * c->hp.a[0];
+    // 4 pi a(t): the contact coupling at the time each stage evaluated H (the
+    // a(t) ramp, when on, is part of the dynamics the adjoint differentiates)
+    auto g4_at = [&](double t) { return 4 * kPi * a_at(c->hp, t); };
 
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int len = std::min(every, nsteps - seg * every);
@@ -693,14 +694,16 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
         }
         for (int j = len - 1; j >= 0; --j) {
             double2* Y0 = c->segbuf[j];
-            // stage states y1, y2, y3 of this step (src/oneComponentGPSolver.cu:165-182)
+            const double tj = c->adj_t0 + (double)(step0 + j) * dt;   // this step's start
+            // stage states y1, y2, y3 of this step (src/oneComponentGPSolver.cu:165-182),
+            // stage s evaluating H at PARS[T] as the forward step did
             {
-                XArgs<T> a = base_xargs<T>(c, 0.0);
+                XArgs<T> a = base_xargs<T>(c, tj);
                 a.P0[0] = Y0;
                 FGPE_RET(Launch<T>::xstage(c, a, 1, dens, MODE_PROLOGUE));
                 for (int s = 1; s <= 3; ++s) {
                     FGPE_RET(spectral_passes<T>(c, nullptr, nullptr));
-                    XArgs<T> b = base_xargs<T>(c, 0.0);
+                    XArgs<T> b = base_xargs<T>(c, s == 1 ? tj : tj + 0.5 * dt);
                     b.P0[0] = Y0;
                     b.PS[0] = (c2_t<T>*)(s == 1 ? c->ystage[0] : c->ystage[s - 2]);
                     b.PSo[0] = (c2_t<T>*)c->ystage[s - 1];
@@ -716,13 +719,13 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             a.spec = c->spec3;
             a.ntot = c->ntot;
             a.Vr = c->Vr[0];
-            a.g4 = g4;
             a.lam_in = LIN;
             a.lam = LAM;
             a.abar = GB;
             a.vbar = vbar;
             // s = 3: abar = conj(c)/6 lam_in at y3; lam = lam_in + G; next abar = conj(c)/3 lam_in + conj(c) G
             a.y = c->ystage[2];
+            a.g4 = g4_at(tj + dt);
             a.src = LIN;
             a.alpha = scl(1.0 / 6.0, cc);
             a.init = 1;
@@ -732,6 +735,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
             // s = 2 at y2: next abar = conj(c)/3 lam_in + conj(c)/2 G
             a.y = c->ystage[1];
+            a.g4 = g4_at(tj + 0.5 * dt);
             a.src = nullptr;
             a.init = 0;
             a.ga = scl(1.0 / 3.0, cc);
@@ -739,11 +743,13 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
             // s = 1 at y1: next abar = conj(c)/6 lam_in + conj(c)/2 G
             a.y = c->ystage[0];
+            a.g4 = g4_at(tj + 0.5 * dt);
             a.ga = scl(1.0 / 6.0, cc);
             a.gg = scl(0.5, cc);
             FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
             // s = 0 at y
             a.y = Y0;
+            a.g4 = g4_at(tj);
             a.gb_out = nullptr;
             FGPE_RET(vjp<T>(c, a, true, dens, ptmp));
             std::swap(LIN, LAM);
@@ -761,7 +767,9 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
         s0 += h[2 * b];
         s1 += h[2 * b + 1];
     }
-    // s1 = sum <r, Phi[n]>: the kernel's strength (a_dd or U) scales Phi
+    // s0 = sum <n, r>: dJ/da, or with the a(t) ramp on, the derivative along a
+    // uniform shift of a(t); s1 = sum <r, Phi[n]>: the kernel's strength (a_dd
+    // or U) scales Phi
     grads[0] = 4 * kPi * s0;
     grads[1] = c->hp.a_dd != 0.0 ? s1 / c->hp.a_dd : 0.0;
     grads[2] = c->hp.sc_U != 0.0 ? s1 / c->hp.sc_U : 0.0;

@@ -268,8 +268,10 @@ int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
    Gradients of a real objective J(psi(T)) of nsteps real-time RK4 steps with
    respect to the trap V (N doubles), a, and the strength of the k-space
    kernel (a_dd or the soft-core U), by the discrete adjoint of the RK4 step
-   (one component, constant a, real V, at most one k-space kernel: dipolar or
-   soft-core; the soft-core radius and dimension stay fixed).
+   (one component, real V, at most one k-space kernel: dipolar or soft-core;
+   the soft-core radius and dimension stay fixed).  With the a(t) ramp on
+   (fgpe_hparams::quench_on) the ramp is part of the differentiated dynamics
+   and host_grad[0] is the derivative along a uniform shift of a(t).
    fgpe_adjoint_forward runs the steps from the current state and keeps a
    checkpoint every ckpt_every steps (about sqrt(nsteps) keeps the memory at
    O(sqrt(nsteps)) states); psi(T) is then the current state.

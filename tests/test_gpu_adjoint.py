@@ -121,7 +121,9 @@ def test_adjoint_through_autograd(torch_cuda):
     from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
     c, V, psi0, target, a, a_dd = setup_case()
     dt, n = 5e-4, 30
-    solver = AdjointSolver(c.spec, dt, n, torch=torch)   # on torch's stream: device tensors pass straight in
+    # on torch's stream: device tensors pass straight in; no autotuning, so that the
+    # plans match the plain context's below bit for bit
+    solver = AdjointSolver(c.spec, dt, n, torch=torch, autotune=False)
     Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
     at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
     addt = torch.tensor(a_dd, dtype=torch.float64, requires_grad=True)
@@ -189,7 +191,7 @@ def test_softcore_adjoint_through_autograd(torch_cuda):
     from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
     c, V, psi0, target, a, _ = setup_case()
     U, dt, n = 1.5, 5e-4, 20
-    solver = AdjointSolver(c.spec, dt, n, torch=torch, softcore=SC)
+    solver = AdjointSolver(c.spec, dt, n, torch=torch, softcore=SC, autotune=False)
     Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
     at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
     ut = torch.tensor(U, dtype=torch.float64, requires_grad=True)
@@ -250,3 +252,68 @@ def test_adjoint_x_stage_third_plan(torch_cuda):
     fd = (Jp - Jm) / (2 * h)
     ad = float(np.sum(g2 * dV))
     assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+
+
+def test_adjoint_solver_autotuned(torch_cuda):
+    """AdjointSolver's default: the plan autotuner runs at the first forward;
+    the gradient then agrees with the untuned one to rounding."""
+    torch = torch_cuda
+    from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
+    c, V, psi0, target, a, a_dd = setup_case()
+    dt, n = 5e-4, 20
+    grads = []
+    for tune in (False, True):
+        solver = AdjointSolver(c.spec, dt, n, torch=torch, autotune=tune)
+        Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
+        p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda")
+        tg = torch.tensor(target, dtype=torch.complex128, device="cuda")
+        at = torch.tensor(a, dtype=torch.float64)
+        addt = torch.tensor(a_dd, dtype=torch.float64)
+        loss = fidelity(GPEEvolve().apply(Vt, at, addt, p0, solver), tg, c.dv)
+        loss.backward()
+        grads.append(Vt.grad.cpu().numpy())
+        solver.ctx.close()
+    assert np.abs(grads[1] - grads[0]).max() <= 1e-10 * np.abs(grads[0]).max()
+
+
+def test_adjoint_with_a_ramp(torch_cuda):
+    """The a(t) quench ramp in the differentiated dynamics: dJ/dV and the
+    derivative along a uniform shift of a(t) (a0, a1, a2 moved together)
+    against central differences of the HIP forward."""
+    c, V, psi0, target, a, a_dd = setup_case()
+    dt, n, every = 5e-4, 30, 6
+    # a(t): 0.6 -> 0.9 over (0, 0.004], held to 0.008, then to 0.45 at 0.012
+    q = (0.6, 0.9, 0.45, 0.004, 0.008, 0.012)
+
+    def ham(ctx, qq):
+        ctx.set_hamiltonian(a=(qq[0],), a_dd=a_dd, dip=(0.0, 0.0, 1.0), quench=qq)
+
+    def forward(Vx, qq):
+        with c.ctx() as ctx:
+            ham(ctx, qq)
+            ctx.upload_vext(Vx.astype(np.complex128))
+            ctx.upload_psi(psi0)
+            ctx.rk4_steps(dt, n)
+            return ctx.download_psi()
+
+    with c.ctx() as ctx:
+        ham(ctx, q)
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        ctx.adjoint_forward(dt, n, every)
+        J, ov = J_of(ctx.download_psi(), target, c.dv)
+        lamT = torch_cuda.from_numpy(2 * ov * c.dv * target).cuda()
+        gV = torch_cuda.empty(c.shape, dtype=torch_cuda.float64, device="cuda")
+        g = ctx.adjoint_backward(lamT.data_ptr(), 0, gV.data_ptr())
+        ctx.sync()
+        gV = gV.cpu().numpy()
+    assert abs(J - J_of(forward(V, q), target, c.dv)[0]) < 1e-14
+    h = 1e-4
+    dV = smooth_field(c, 500)
+    fd = (J_of(forward(V + h * dV, q), target, c.dv)[0] - J_of(forward(V - h * dV, q), target, c.dv)[0]) / (2 * h)
+    ad = float(np.sum(gV * dV))
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+    qp = tuple(v + h if i < 3 else v for i, v in enumerate(q))
+    qm = tuple(v - h if i < 3 else v for i, v in enumerate(q))
+    fd = (J_of(forward(V, qp), target, c.dv)[0] - J_of(forward(V, qm), target, c.dv)[0]) / (2 * h)
+    assert abs(g[0] - fd) <= 1e-6 * abs(fd), (g[0], fd)

@@ -746,3 +746,28 @@ def test_x_stage_variant_range():
             ctx.set_variant(1, 1, 2)
         with pytest.raises(FgpeError):
             ctx.set_variant(XSTAGE, 0, 3)
+
+
+@pytest.mark.parametrize("nx", [128, 256])
+def test_x_stage_third_plan_1d(nx):
+    """1D grids (one x row, fewer rows than a block's pencils) with the x
+    stage's third plan and the 1D soft-core kernel: RK4 against the oracle
+    (FP64 1e-12, mixed 1e-6)."""
+    from fgpe_amd import configs
+    c = Case(nx, 1, 1, L=(12.0, 1.0, 1.0))
+    h = dict(a=(0.3,), a_dd=0.15, dip=(1.0, 0.0, 0.0), sc_U=2.0, sc_R=1.1, sc_dim=1)
+    V = configs.harmonic_trap(c.spec, (1.0, 1.0, 1.0))
+    psi = configs.gaussian_noisy(c.spec, 63, width=1.5, noise=0.02)
+    vt = c.vtilde(h["a_dd"], h["dip"], sc=(1, h["sc_R"], h["sc_U"]))
+    kmax = float(np.max(np.abs(c.kx)))
+    dt, n = min(1e-3, 0.5 / (19.739208802 * kmax ** 2)), 12
+    ref = orc.rk4(c.grid, orc.Ham(1, h["a"], vtilde=vt), psi, V, dt, n)
+    for prec, tol in ((0, 1e-12), (1, 1e-6)):
+        with c.ctx(prec) as ctx:
+            ctx.set_variant(XSTAGE, 0, 2)
+            ctx.set_hamiltonian(**h)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            ctx.rk4_steps(dt, n)
+            err = rel_max(ctx.download_psi(), ref)
+        assert err < tol, (prec, err)
