@@ -336,7 +336,9 @@ class Context:
         check(load().fgpe_adjoint_forward(self.h, dt, t0, nsteps, ckpt_every))
 
     def adjoint_backward(self, dev_lamT: int, dev_lam0: int | None, dev_gradV: int):
-        g = np.zeros(3)
+        """Device pointers to ncomp x N arrays (component-major); returns
+        (dJ/da or dJ/da_11, dJ/da_dd, dJ/dU, dJ/da_22, dJ/da_12)."""
+        g = np.zeros(5)
         check(load().fgpe_adjoint_backward(self.h, dev_lamT, dev_lam0, dev_gradV, _dptr(g)))
         return g
 

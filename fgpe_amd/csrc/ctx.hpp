@@ -112,6 +112,7 @@ struct fgpe_ctx {
     void* spec3 = nullptr;
     double2* lam[3] = {nullptr, nullptr, nullptr};   // LIN, LAM, GB
     double* pacc = nullptr;
+    int adj_nc = 0;   // components the adjoint buffers hold (0: none allocated)
     int adj_every = 0, adj_nsteps = 0;
     double adj_dt = 0.0, adj_t0 = 0.0;
     bool adj_ready = false;

@@ -601,20 +601,63 @@ __global__ void k_accumulate_impl(double* __restrict__ acc, const double* __rest
 int adj_check(fgpe_ctx* c)
 {
     FGPE_RET(ensure_ready(c));
-    if (c->hp.ncomp != 1) return fail(FGPE_EUNSUP, "the adjoint is one-component");
     if (c->hp.sc_U != 0.0 && c->hp.a_dd != 0.0)
         return fail(FGPE_EUNSUP, "the adjoint takes one k-space kernel: dipolar or soft-core");
-    if (c->vcomplex[0]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
-    if (c->hp.vpx[0] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
+    for (int i = 0; i < c->hp.ncomp; ++i)
+        if (c->vcomplex[i]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
+    if (c->hp.vpx[0] != 0.0 || c->hp.vpx[1] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
     if (c->hp.user_id != 0) return fail(FGPE_EUNSUP, "the adjoint does not know the user term's derivative");
+    return FGPE_OK;
+}
+
+// The adjoint's buffers, sized for nc components: checkpoints and segment
+// states (FP64, nc states each), stage states (bulk), the VJP's spectral
+// channels (nc + 2), the adjoint vectors and the parameter accumulators.
+// A context that switches to more components drops the smaller ones.
+int adj_reserve(fgpe_ctx* c, int nc)
+{
+    if (c->adj_nc >= nc) return FGPE_OK;
+    const size_t N = (size_t)c->ntot;
+    for (double2*& p : c->ckpt) dfree(c, (void**)&p, sizeof(double2) * N * c->adj_nc);
+    for (double2*& p : c->segbuf) dfree(c, (void**)&p, sizeof(double2) * N * c->adj_nc);
+    c->ckpt.clear();
+    c->segbuf.clear();
+    for (int i = 0; i < 3; ++i) {
+        dfree(c, &c->ystage[i], 2 * c->tsz() * N * c->adj_nc);
+        dfree(c, (void**)&c->lam[i], sizeof(double2) * N * c->adj_nc);
+    }
+    dfree(c, &c->spec3, (size_t)(c->adj_nc + 2) * 2 * c->tsz() * N);
+    c->adj_nc = nc;
+    for (int i = 0; i < 3; ++i) {
+        FGPE_RET(dalloc(c, &c->ystage[i], 2 * c->tsz() * N * nc));
+        FGPE_RET(dalloc(c, (void**)&c->lam[i], sizeof(double2) * N * nc));
+    }
+    FGPE_RET(dalloc(c, &c->spec3, (size_t)(nc + 2) * 2 * c->tsz() * N));
+    if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)(c->ntot / 32 + 1024)));
+    c->adj_ready = false;
+    return FGPE_OK;
+}
+
+// nc FP64 states, component-major: copy between the solver's per-component
+// state buffers and one of the adjoint's
+int copy_states(fgpe_ctx* c, double2* buf, bool to_buf)
+{
+    const size_t b64 = sizeof(double2) * (size_t)c->ntot;
+    for (int i = 0; i < c->hp.ncomp; ++i) {
+        double2* b = buf + (size_t)i * c->ntot;
+        FGPE_HIPCHK(hipMemcpyAsync(to_buf ? b : c->P0[i], to_buf ? c->P0[i] : b, b64, hipMemcpyDeviceToDevice,
+                                   c->stream));
+    }
     return FGPE_OK;
 }
 
 template <typename T> int adj_forward(fgpe_ctx* c, double dt, double t0, int nsteps, int every)
 {
     FGPE_RET(adj_check(c));
+    const int nc = c->hp.ncomp;
+    FGPE_RET(adj_reserve(c, nc));
     const int nseg = (nsteps + every - 1) / every;
-    const size_t bytes = sizeof(double2) * c->ntot;
+    const size_t bytes = sizeof(double2) * c->ntot * c->adj_nc;
     while ((int)c->ckpt.size() < nseg) {
         double2* p = nullptr;
         FGPE_RET(dalloc(c, (void**)&p, bytes));
@@ -623,7 +666,7 @@ template <typename T> int adj_forward(fgpe_ctx* c, double dt, double t0, int nst
     const double one[2] = {1.0, 1.0};
     for (int seg = 0; seg < nseg; ++seg) {
         const int len = std::min(every, nsteps - seg * every);
-        FGPE_HIPCHK(hipMemcpyAsync(c->ckpt[seg], c->P0[0], bytes, hipMemcpyDeviceToDevice, c->stream));
+        FGPE_RET(copy_states(c, c->ckpt[seg], true));
         FGPE_RET(rk4_run<T>(c, dt, t0 + (double)seg * every * dt, len, false, one, nullptr));
     }
     c->adj_every = every;
@@ -634,14 +677,16 @@ template <typename T> int adj_forward(fgpe_ctx* c, double dt, double t0, int nst
     return FGPE_OK;
 }
 
-// One VJP: prologue (abar, n, r channels), spectral passes, epilogue.
-template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, double* ptmp)
+// One VJP: prologue (abar_c, n, r channels), spectral passes, epilogue; the
+// epilogue's per-block parameter sums are added to the accumulators.
+template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, double* ptmp, int ns)
 {
+    const int nc = c->hp.ncomp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, false));
-    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? 3 : 1, 1, nullptr, nullptr, false));
+    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? nc + 2 : nc, nc, nullptr, nullptr, false));
     a.partials = ptmp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, true));
-    const long long n = 2 * Launch<T>::xblocks(c);
+    const long long n = (long long)ns * Launch<T>::xblocks(c);
     hipLaunchKernelGGL(k_accumulate_impl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->pacc,
                        ptmp, n);
     FGPE_LAUNCHCHK("k_accumulate");
@@ -651,107 +696,125 @@ template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, doubl
 template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0, double* vbar, double* grads)
 {
     FGPE_RET(adj_check(c));
-    if (!c->adj_ready) return fail(FGPE_ESTATE, "fgpe_adjoint_forward has not been run");
+    if (!c->adj_ready || c->adj_nc < c->hp.ncomp) return fail(FGPE_ESTATE, "fgpe_adjoint_forward has not been run");
+    const int nc = c->hp.ncomp;
+    const int ns = nc == 1 ? 2 : 4;   // parameter sums per block
     const size_t N = (size_t)c->ntot;
-    const size_t b64 = sizeof(double2) * N;
-    const bool dens = c->nch > 1;
+    const size_t b64 = sizeof(double2) * N * nc;   // nc FP64 states
+    const size_t bT = 2 * c->tsz() * N;             // one bulk state
+    const bool dens = c->nch > nc;
     const int every = c->adj_every, nsteps = c->adj_nsteps;
     const int nseg = (nsteps + every - 1) / every;
     const double dt = c->adj_dt;
     const long long nb = Launch<T>::xblocks(c);
     while ((int)c->segbuf.size() < every) {
         double2* p = nullptr;
-        FGPE_RET(dalloc(c, (void**)&p, b64));
+        FGPE_RET(dalloc(c, (void**)&p, sizeof(double2) * N * c->adj_nc));
         c->segbuf.push_back(p);
     }
-    for (int i = 0; i < 3; ++i) {
-        if (!c->ystage[i]) FGPE_RET(dalloc(c, &c->ystage[i], 2 * c->tsz() * N));
-        if (!c->lam[i]) FGPE_RET(dalloc(c, (void**)&c->lam[i], b64));
-    }
-    if (!c->spec3) FGPE_RET(dalloc(c, &c->spec3, 3 * 2 * c->tsz() * N));
-    if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)(c->ntot / 32 + 1024)));
-    double* ptmp = c->pacc + 2 * nb;
-    FGPE_HIPCHK(hipMemsetAsync(c->pacc, 0, sizeof(double) * 2 * nb, c->stream));
-    FGPE_HIPCHK(hipMemsetAsync(vbar, 0, sizeof(double) * N, c->stream));
+    double* ptmp = c->pacc + ns * nb;
+    FGPE_HIPCHK(hipMemsetAsync(c->pacc, 0, sizeof(double) * ns * nb, c->stream));
+    FGPE_HIPCHK(hipMemsetAsync(vbar, 0, sizeof(double) * N * nc, c->stream));
     double2 *LIN = c->lam[0], *LAM = c->lam[1], *GB = c->lam[2];
     FGPE_HIPCHK(hipMemcpyAsync(LIN, lamT, b64, hipMemcpyDefault, c->stream));
     const double one[2] = {1.0, 1.0};
     const double2 cc = double2{0.0, dt};   // conj(c), c = -i dt
     auto scl = [](double s, double2 z) { return double2{s * z.x, s * z.y}; };
-    // 4 pi a(t): the contact coupling at the time each stage evaluated H (the
-    // a(t) ramp, when on, is part of the dynamics the adjoint differentiates)
-    auto g4_at = [&](double t) { return 4 * kPi * a_at(c->hp, t); };
+    auto comp = [&](void* base, int i, size_t stride) { return (void*)((char*)base + (size_t)i * stride); };
+    // 4 pi a_cc' at the time each stage evaluated H (the one-component a(t)
+    // ramp, when on, is part of the dynamics the adjoint differentiates)
+    auto set_g4 = [&](AdjArgs& a, double t) {
+        if (nc == 1) {
+            a.g4[0][0] = 4 * kPi * a_at(c->hp, t);
+            a.g4[0][1] = a.g4[1][0] = a.g4[1][1] = 0.0;
+        } else {
+            const double* aa = c->hp.a;   // a_11, a_22, a_12
+            a.g4[0][0] = 4 * kPi * aa[0];
+            a.g4[1][1] = 4 * kPi * aa[1];
+            a.g4[0][1] = a.g4[1][0] = 4 * kPi * aa[2];
+        }
+    };
 
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int len = std::min(every, nsteps - seg * every);
         const int step0 = seg * every;
         // recompute the states of this segment
-        FGPE_HIPCHK(hipMemcpyAsync(c->P0[0], c->ckpt[seg], b64, hipMemcpyDeviceToDevice, c->stream));
+        FGPE_RET(copy_states(c, c->ckpt[seg], false));
         c->spec_valid = false;
         for (int j = 0; j < len; ++j) {
-            FGPE_HIPCHK(hipMemcpyAsync(c->segbuf[j], c->P0[0], b64, hipMemcpyDeviceToDevice, c->stream));
+            FGPE_RET(copy_states(c, c->segbuf[j], true));
             if (j + 1 < len) FGPE_RET(rk4_run<T>(c, dt, c->adj_t0 + (step0 + j) * dt, 1, false, one, nullptr));
         }
         for (int j = len - 1; j >= 0; --j) {
             double2* Y0 = c->segbuf[j];
             const double tj = c->adj_t0 + (double)(step0 + j) * dt;   // this step's start
-            // stage states y1, y2, y3 of this step (src/oneComponentGPSolver.cu:165-182),
-            // stage s evaluating H at PARS[T] as the forward step did
+            // stage states y1, y2, y3 of this step (src/oneComponentGPSolver.cu:165-182,
+            // src/twoComponentGPSolver.cu:150-182), stage s evaluating H at PARS[T]
+            // as the forward step did
             {
                 XArgs<T> a = base_xargs<T>(c, tj);
-                a.P0[0] = Y0;
-                FGPE_RET(Launch<T>::xstage(c, a, 1, dens, MODE_PROLOGUE));
+                for (int i = 0; i < nc; ++i) a.P0[i] = Y0 + (size_t)i * N;
+                FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_PROLOGUE));
                 for (int s = 1; s <= 3; ++s) {
                     FGPE_RET(spectral_passes<T>(c, nullptr, nullptr));
                     XArgs<T> b = base_xargs<T>(c, s == 1 ? tj : tj + 0.5 * dt);
-                    b.P0[0] = Y0;
-                    b.PS[0] = (c2_t<T>*)(s == 1 ? c->ystage[0] : c->ystage[s - 2]);
-                    b.PSo[0] = (c2_t<T>*)c->ystage[s - 1];
+                    for (int i = 0; i < nc; ++i) {
+                        b.P0[i] = Y0 + (size_t)i * N;
+                        b.PS[i] = (c2_t<T>*)comp(s == 1 ? c->ystage[0] : c->ystage[s - 2], i, bT);
+                        b.PSo[i] = (c2_t<T>*)comp(c->ystage[s - 1], i, bT);
+                        if (sizeof(T) == 4 && s == 3) b.PS1[i] = (c2_t<T>*)comp(c->ystage[0], i, bT);   // stage 1's state
+                    }
                     b.cr = 0.0;
                     b.ci = (s <= 2) ? -0.5 * dt : -1.0 * dt;
                     b.stage = s;
-                    if (sizeof(T) == 4 && s == 3) b.PS1[0] = (c2_t<T>*)c->ystage[0];   // stage 1's state
-                    FGPE_RET(Launch<T>::xstage(c, b, 1, dens, MODE_RK));
+                    FGPE_RET(Launch<T>::xstage(c, b, nc, dens, MODE_RK));
                 }
                 c->spec_valid = false;
             }
             AdjArgs a{};
             a.spec = c->spec3;
             a.ntot = c->ntot;
-            a.Vr = c->Vr[0];
-            a.lam_in = LIN;
-            a.lam = LAM;
-            a.abar = GB;
-            a.vbar = vbar;
+            for (int i = 0; i < nc; ++i) {
+                a.Vr[i] = c->Vr[i];
+                a.lam_in[i] = LIN + (size_t)i * N;
+                a.lam[i] = LAM + (size_t)i * N;
+                a.abar[i] = GB + (size_t)i * N;
+                a.vbar[i] = vbar + (size_t)i * N;
+            }
+            auto set_y = [&](void* base, size_t stride) {
+                for (int i = 0; i < nc; ++i) a.y[i] = comp(base, i, stride);
+            };
             // s = 3: abar = conj(c)/6 lam_in at y3; lam = lam_in + G; next abar = conj(c)/3 lam_in + conj(c) G
-            a.y = c->ystage[2];
-            a.g4 = g4_at(tj + dt);
-            a.src = LIN;
+            set_y(c->ystage[2], bT);
+            set_g4(a, tj + dt);
+            for (int i = 0; i < nc; ++i) {
+                a.src[i] = LIN + (size_t)i * N;
+                a.gb_out[i] = GB + (size_t)i * N;
+            }
             a.alpha = scl(1.0 / 6.0, cc);
             a.init = 1;
-            a.gb_out = GB;
             a.ga = scl(1.0 / 3.0, cc);
             a.gg = cc;
-            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
+            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp, ns));
             // s = 2 at y2: next abar = conj(c)/3 lam_in + conj(c)/2 G
-            a.y = c->ystage[1];
-            a.g4 = g4_at(tj + 0.5 * dt);
-            a.src = nullptr;
+            set_y(c->ystage[1], bT);
+            set_g4(a, tj + 0.5 * dt);
+            for (int i = 0; i < nc; ++i) a.src[i] = nullptr;
             a.init = 0;
             a.ga = scl(1.0 / 3.0, cc);
             a.gg = scl(0.5, cc);
-            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
+            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp, ns));
             // s = 1 at y1: next abar = conj(c)/6 lam_in + conj(c)/2 G
-            a.y = c->ystage[0];
-            a.g4 = g4_at(tj + 0.5 * dt);
+            set_y(c->ystage[0], bT);
+            set_g4(a, tj + 0.5 * dt);
             a.ga = scl(1.0 / 6.0, cc);
             a.gg = scl(0.5, cc);
-            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp));
+            FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp, ns));
             // s = 0 at y
-            a.y = Y0;
-            a.g4 = g4_at(tj);
-            a.gb_out = nullptr;
-            FGPE_RET(vjp<T>(c, a, true, dens, ptmp));
+            set_y(Y0, sizeof(double2) * N);
+            set_g4(a, tj);
+            for (int i = 0; i < nc; ++i) a.gb_out[i] = nullptr;
+            FGPE_RET(vjp<T>(c, a, true, dens, ptmp, ns));
             std::swap(LIN, LAM);
         }
     }
@@ -759,20 +822,21 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
     c->lam[1] = LAM;
     if (lam0) FGPE_HIPCHK(hipMemcpyAsync(lam0, LIN, b64, hipMemcpyDefault, c->stream));
     // fixed-order sum of the per-block accumulators
-    std::vector<double> h(2 * (size_t)nb);
-    FGPE_HIPCHK(hipMemcpyAsync(h.data(), c->pacc, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> h((size_t)ns * nb);
+    FGPE_HIPCHK(hipMemcpyAsync(h.data(), c->pacc, sizeof(double) * ns * nb, hipMemcpyDeviceToHost, c->stream));
     FGPE_HIPCHK(hipStreamSynchronize(c->stream));
-    double s0 = 0.0, s1 = 0.0;
-    for (long long b = 0; b < nb; ++b) {
-        s0 += h[2 * b];
-        s1 += h[2 * b + 1];
-    }
-    // s0 = sum <n, r>: dJ/da, or with the a(t) ramp on, the derivative along a
-    // uniform shift of a(t); s1 = sum <r, Phi[n]>: the kernel's strength (a_dd
-    // or U) scales Phi
-    grads[0] = 4 * kPi * s0;
-    grads[1] = c->hp.a_dd != 0.0 ? s1 / c->hp.a_dd : 0.0;
-    grads[2] = c->hp.sc_U != 0.0 ? s1 / c->hp.sc_U : 0.0;
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long long b = 0; b < nb; ++b)
+        for (int k = 0; k < ns; ++k) sum[k] += h[(size_t)ns * b + k];
+    // the contact sums: dJ/da (or, with the a(t) ramp on, the derivative along a
+    // uniform shift of a(t)); the kernel sum <r, Phi[n]>: the kernel's strength
+    // (a_dd or U) scales Phi
+    const double sk = nc == 1 ? sum[1] : sum[3];
+    grads[0] = 4 * kPi * sum[0];
+    grads[1] = c->hp.a_dd != 0.0 ? sk / c->hp.a_dd : 0.0;
+    grads[2] = c->hp.sc_U != 0.0 ? sk / c->hp.sc_U : 0.0;
+    grads[3] = nc == 2 ? 4 * kPi * sum[1] : 0.0;
+    grads[4] = nc == 2 ? 4 * kPi * sum[2] : 0.0;
     c->spec_valid = false;
     return FGPE_OK;
 }

@@ -1011,37 +1011,41 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Discrete adjoint of the one-component RK4 step.  With g(psi) = H(psi) psi,
-// H = T + V + g4 |psi|^2 + Phi[|psi|^2] (real V), the transposed Jacobian
-// (real inner product Re<u, v>) acting on abar is
-//     G^T abar = H(psi) abar + 2 (g4 r + Phi[r]) psi,   r = Re(conj(psi) abar)
-// and the parameter derivatives contribute  dV += r,  d(4 pi a) += n r,
-// d(a_dd) += Phi1[n] r  (Phi1 = Phi / a_dd).  One VJP is the prologue below,
-// the spectral passes on three channels (abar: kinetic; n and r: the k-space
-// kernel), and the epilogue.
+// Discrete adjoint of the RK4 step, one or two components.  With g(psi) =
+// H(psi) psi, H_c = T + V_c + sum_c' g4[c][c'] n_c' + Phi[n] (real V,
+// n = n_1 (+ n_2)), the transposed Jacobian (real inner product Re<u, v>)
+// acting on abar is, per component,
+//     (G^T abar)_c = H_c abar_c + 2 psi_c (sum_c' g4[c][c'] r_c' + Phi[r]),
+//     r_c = Re(conj(psi_c) abar_c),  r = r_1 (+ r_2)
+// (g4 symmetric), and the parameter derivatives contribute dV_c += r_c,
+// d(4 pi a_cc) += n_c r_c, d(4 pi a_12) += n_1 r_2 + n_2 r_1, and
+// d(strength) += Phi1[n] r for the k-space kernel (Phi1 = Phi / strength: the
+// kernel is linear in a_dd, or in the soft-core U).  One VJP is the prologue
+// below, the spectral passes on NC + 2 channels (abar_c: kinetic; n and r:
+// the k-space kernel), and the epilogue.
 
 struct AdjArgs {
-    void* spec;             // 3 channels of T2 (abar, n, r)
+    void* spec;             // channels of T2: abar_c (c < NC), then n, r
     long long ntot;
-    const void* y;          // state the Jacobian is taken at (YT2)
-    const double2* src;     // prologue: abar = alpha * src (null: abar read from `abar`)
+    const void* y[2];       // state components the Jacobian is taken at (YT2)
+    const double2* src[2];  // prologue: abar_c = alpha * src_c (null: abar read from `abar`)
     double2 alpha;
-    double2* abar;          // abar of this VJP (written by the prologue when src is set)
-    const void* Vr;         // Re V (T)
-    double g4;              // 4 pi a
+    double2* abar[2];       // abar of this VJP (written by the prologue when src is set)
+    const void* Vr[2];      // Re V_c (T)
+    double g4[2][2];        // 4 pi a_cc'
     // epilogue
-    const double2* lam_in;  // adjoint of the step output
-    double2* lam;           // accumulator: lam = lam_in + G (init) or lam += G
+    const double2* lam_in[2];   // adjoint of the step output
+    double2* lam[2];            // accumulator: lam = lam_in + G (init) or lam += G
     int init;
-    double2* gb_out;        // next abar = ga * lam_in + gg * G (null: none)
+    double2* gb_out[2];         // next abar = ga * lam_in + gg * G (null: none; may be abar itself)
     double2 ga, gg;
-    double* vbar;           // += r
-    double* partials;       // per block: += (sum n r, sum Phi[n] r)
+    double* vbar[2];            // += r_c
+    double* partials;           // per block, NC = 1: (n r, Phi[n] r); NC = 2: (n1 r1, n2 r2, n1 r2 + n2 r1, Phi[n] r)
 };
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-template <typename T, typename YT, int N, int V, bool DENS>
+template <typename T, typename YT, int N, int V, bool DENS, int NC>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ twg)
 {
@@ -1055,50 +1059,57 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ twg)
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
     const gp<T2> sp = glob((T2*)a.spec);
-    const gp<const Y2> y = glob((const Y2*)a.y);
-    const gp<double2> abar = glob(a.abar);
     T2 v[E];
     double nn[E], rr[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-        const long long g = row0 + elemA<N, V>(t, q);
-        double2 ab;
-        if (a.src) {
-            ab = zmul(a.alpha, glob(a.src)[g]);
-            abar[g] = ab;
-        } else {
-            ab = abar[g];
-        }
-        const Y2 yr = y[g];
-        const double yx = (double)yr.x, yy = (double)yr.y;
-        nn[q] = yx * yx + yy * yy;
-        rr[q] = yx * ab.x + yy * ab.y;
-        v[q] = T2{(T)ab.x, (T)ab.y};
-    }
-    fft_forward<T2, N, V, true>(v, t, my, tw);
+    for (int q = 0; q < E; ++q) nn[q] = rr[q] = 0.0;
 #pragma unroll
-    for (int q = 0; q < E; ++q) sp[row0 + elemB<N, V>(t, q)] = v[q];
+    for (int c = 0; c < NC; ++c) {
+        const gp<const Y2> y = glob((const Y2*)a.y[c]);
+        const gp<double2> abar = glob(a.abar[c]);
+        const gp<const double2> src = glob(a.src[c]);
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const long long g = row0 + elemA<N, V>(t, q);
+            double2 ab;
+            if (a.src[0]) {
+                ab = zmul(a.alpha, src[g]);
+                abar[g] = ab;
+            } else {
+                ab = abar[g];
+            }
+            const Y2 yr = y[g];
+            const double yx = (double)yr.x, yy = (double)yr.y;
+            nn[q] += yx * yx + yy * yy;
+            rr[q] += yx * ab.x + yy * ab.y;
+            v[q] = T2{(T)ab.x, (T)ab.y};
+        }
+        fft_forward<T2, N, V, true>(v, t, my, tw);
+#pragma unroll
+        for (int q = 0; q < E; ++q) sp[c * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
+    }
     if (DENS) {
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)nn[q], (T)0};
         fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) sp[a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) sp[NC * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)rr[q], (T)0};
         fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) sp[2 * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) sp[(NC + 1) * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
     }
 }
 
-template <typename T, typename YT, int N, int V, bool DENS>
+template <typename T, typename YT, int N, int V, bool DENS, int NC>
 __global__ void __launch_bounds__((px_of<T, N, V>() * Plan<N, V>::T))
 k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
 {
     using T2 = c2_t<T>;
     using Y2 = c2_t<YT>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
+    constexpr int NS = NC == 1 ? 2 : 4;
     __shared__ T2 lds[P * lds_stride<N>()];
     __shared__ T2 twl[N];
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
@@ -1106,54 +1117,87 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
     const long long row0 = ((long long)blockIdx.x * P + p) * N;
     T2* my = lds + p * lds_stride<N>();
     const gp<T2> sp = glob((T2*)a.spec);
-    const gp<const Y2> y = glob((const Y2*)a.y);
-    const gp<const T> Vr = glob((const T*)a.Vr);
-    const gp<double2> lam = glob(a.lam), gb_out = glob(a.gb_out);
-    const gp<double> vbar = glob(a.vbar);
     T2 v[E];
     double phin[E], phir[E];
     if (DENS) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[a.ntot + row0 + elemB<N, V>(t, q)];
+        for (int q = 0; q < E; ++q) v[q] = sp[NC * a.ntot + row0 + elemB<N, V>(t, q)];
         fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phin[q] = (double)v[q].x;
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[2 * a.ntot + row0 + elemB<N, V>(t, q)];
+        for (int q = 0; q < E; ++q) v[q] = sp[(NC + 1) * a.ntot + row0 + elemB<N, V>(t, q)];
         fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phir[q] = (double)v[q].x;
     }
+    double s[NS];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = sp[row0 + elemB<N, V>(t, q)];
-    fft_inverse<T2, N, V, true>(v, t, my, tw);   // T abar
-    double s[2] = {0.0, 0.0};
+    for (int k = 0; k < NS; ++k) s[k] = 0.0;
+    // NC == 2: component 0's n and r, kept for component 1's coupling terms
+    // (gb_out may alias abar, so component 0's abar is overwritten by then)
+    double nsv[E], rsv[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-        const long long g = row0 + elemA<N, V>(t, q);
-        const Y2 yr = y[g];
-        const double2 yv = double2{(double)yr.x, (double)yr.y};
-        const double2 ab = glob(a.abar)[g];
-        const double n = yv.x * yv.x + yv.y * yv.y;
-        const double r = yv.x * ab.x + yv.y * ab.y;
-        const double pn = DENS ? phin[q] : 0.0;
-        const double pr = DENS ? phir[q] : 0.0;
-        const double w = (double)Vr[g] + a.g4 * n + pn;
-        const double c2 = 2.0 * (a.g4 * r + pr);
-        const double2 G = double2{(double)v[q].x + w * ab.x + c2 * yv.x, (double)v[q].y + w * ab.y + c2 * yv.y};
-        const double2 li = glob(a.lam_in)[g];
-        const double2 base = a.init ? li : lam[g];
-        lam[g] = double2{base.x + G.x, base.y + G.y};
-        if (a.gb_out) {
-            const double2 u = zmul(a.ga, li), wv = zmul(a.gg, G);
-            gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
+    for (int c = 0; c < NC; ++c) {
+        const gp<const Y2> y = glob((const Y2*)a.y[c]);
+        const gp<const Y2> yo = glob((const Y2*)a.y[1]);
+        const gp<const double2> abar = glob((const double2*)a.abar[c]);
+        const gp<const double2> abo = glob((const double2*)a.abar[1]);
+        const gp<const T> Vr = glob((const T*)a.Vr[c]);
+        const gp<double2> lam = glob(a.lam[c]), gb_out = glob(a.gb_out[c]);
+        const gp<const double2> lam_in = glob(a.lam_in[c]);
+        const gp<double> vbar = glob(a.vbar[c]);
+        const double gs = a.g4[c][c], go = a.g4[c][1 - c];
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = sp[c * a.ntot + row0 + elemB<N, V>(t, q)];
+        fft_inverse<T2, N, V, true>(v, t, my, tw);   // T abar_c
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const long long g = row0 + elemA<N, V>(t, q);
+            const Y2 yr = y[g];
+            const double2 yv = double2{(double)yr.x, (double)yr.y};
+            const double2 ab = abar[g];
+            const double n = yv.x * yv.x + yv.y * yv.y;
+            const double r = yv.x * ab.x + yv.y * ab.y;
+            double no = 0.0, ro = 0.0;   // the other component's density and r
+            if (NC == 2) {
+                if (c == 0) {
+                    const Y2 yq = yo[g];
+                    const double2 yw = double2{(double)yq.x, (double)yq.y};
+                    const double2 aq = abo[g];
+                    no = yw.x * yw.x + yw.y * yw.y;
+                    ro = yw.x * aq.x + yw.y * aq.y;
+                    nsv[q] = n;
+                    rsv[q] = r;
+                } else {
+                    no = nsv[q];
+                    ro = rsv[q];
+                }
+            }
+            const double pn = DENS ? phin[q] : 0.0;
+            const double pr = DENS ? phir[q] : 0.0;
+            const double w = (double)Vr[g] + gs * n + go * no + pn;
+            const double c2 = 2.0 * (gs * r + go * ro + pr);
+            const double2 G = double2{(double)v[q].x + w * ab.x + c2 * yv.x, (double)v[q].y + w * ab.y + c2 * yv.y};
+            const double2 li = lam_in[g];
+            const double2 base = a.init ? li : lam[g];
+            lam[g] = double2{base.x + G.x, base.y + G.y};
+            if (a.gb_out[0]) {
+                const double2 u = zmul(a.ga, li), wv = zmul(a.gg, G);
+                gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
+            }
+            vbar[g] += r;
+            if constexpr (NC == 1) {
+                s[0] += n * r;
+                s[1] += pn * r;
+            } else {
+                s[c] += n * r;           // a_cc
+                if (c == 0) s[2] += n * ro + no * r;   // a_12: n1 r2 + n2 r1, once
+                s[3] += pn * r;          // the kernel: Phi[n] (r1 + r2)
+            }
         }
-        vbar[g] += r;
-        s[0] += n * r;
-        s[1] += pn * r;
     }
-    double tot[2] = {s[0], s[1]};
-    block_sum_store<2>(tot, glob(a.partials) + 2 * (long long)blockIdx.x + 0);
+    block_sum_store<NS>(s, glob(a.partials) + NS * (long long)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

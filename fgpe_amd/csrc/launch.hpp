@@ -132,23 +132,33 @@ int xstage_n(fgpe_ctx* c, const XArgs<T>& a, int nc, bool dens, int mode, const 
                 : xstage_m<T, N, V, 2, false>(c, a, mode, tab, ninst);
 }
 
-template <typename T, typename YT, int N, int V, bool DENS> int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
+template <typename T, typename YT, int N, int V, bool DENS, int NC>
+int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
 {
     constexpr int P = px_of<T, N, V>();
     const long long rows = c->ntot / N;
     if (rows % P) return fail(FGPE_EUNSUP, "the adjoint needs whole blocks of x rows");
     dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_XSTAGE);
-    if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, V, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
-    else hipLaunchKernelGGL((k_adj_pro<T, YT, N, V, DENS>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, V, DENS, NC>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
+    else hipLaunchKernelGGL((k_adj_pro<T, YT, N, V, DENS, NC>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
     FGPE_LAUNCHCHK("k_adj");
     return FGPE_OK;
 }
 
+template <typename T, typename YT, int N, int V, int NC>
+int adj_d(fgpe_ctx* c, const AdjArgs& a, bool dens, bool epi)
+{
+    return dens ? adj_t<T, YT, N, V, true, NC>(c, a, epi) : adj_t<T, YT, N, V, false, NC>(c, a, epi);
+}
+
+// y64: the state the Jacobian is taken at is FP64 (else T); the component
+// count is the context's
 template <typename T, int N, int V> int adj_n(fgpe_ctx* c, const AdjArgs& a, bool y64, bool dens, bool epi)
 {
-    if (y64) return dens ? adj_t<T, double, N, V, true>(c, a, epi) : adj_t<T, double, N, V, false>(c, a, epi);
-    return dens ? adj_t<T, T, N, V, true>(c, a, epi) : adj_t<T, T, N, V, false>(c, a, epi);
+    if (c->hp.ncomp == 2)
+        return y64 ? adj_d<T, double, N, V, 2>(c, a, dens, epi) : adj_d<T, T, N, V, 2>(c, a, dens, epi);
+    return y64 ? adj_d<T, double, N, V, 1>(c, a, dens, epi) : adj_d<T, T, N, V, 1>(c, a, dens, epi);
 }
 
 }  // namespace detail

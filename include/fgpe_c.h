@@ -266,20 +266,22 @@ int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------
    Gradients of a real objective J(psi(T)) of nsteps real-time RK4 steps with
-   respect to the trap V (N doubles), a, and the strength of the k-space
-   kernel (a_dd or the soft-core U), by the discrete adjoint of the RK4 step
-   (one component, real V, at most one k-space kernel: dipolar or soft-core;
-   the soft-core radius and dimension stay fixed).  With the a(t) ramp on
-   (fgpe_hparams::quench_on) the ramp is part of the differentiated dynamics
-   and host_grad[0] is the derivative along a uniform shift of a(t).
+   respect to the trap V, the contact lengths and the strength of the k-space
+   kernel (a_dd or the soft-core U), by the discrete adjoint of the RK4 step:
+   one or two components, real V, at most one k-space kernel (dipolar or
+   soft-core; its direction, radius and dimension stay fixed).  With the
+   one-component a(t) ramp on (fgpe_hparams::quench_on) the ramp is part of the
+   differentiated dynamics and the derivative in a is along a uniform shift of
+   a(t).
    fgpe_adjoint_forward runs the steps from the current state and keeps a
    checkpoint every ckpt_every steps (about sqrt(nsteps) keeps the memory at
    O(sqrt(nsteps)) states); psi(T) is then the current state.
    fgpe_adjoint_backward takes dev_lamT = dJ/dRe(psi) + i dJ/dIm(psi) at T
-   (N complex doubles, device), recomputes each segment from its checkpoint,
-   and writes dJ/dpsi(0) in the same convention to dev_lam0 (or NULL),
-   dJ/dV to dev_gradV and host_grad[0] = dJ/da, host_grad[1] = dJ/da_dd,
-   host_grad[2] = dJ/dU (3 doubles; 0 for the kernel that is off).
+   (ncomp x N complex doubles, component-major, device), recomputes each
+   segment from its checkpoint, and writes dJ/dpsi(0) in the same convention
+   to dev_lam0 (or NULL), dJ/dV_c to dev_gradV (ncomp x N doubles) and
+   host_grad (5 doubles): [0] dJ/da (one component) or dJ/da_11, [1] dJ/da_dd,
+   [2] dJ/dU, [3] dJ/da_22, [4] dJ/da_12 (0 where a parameter is absent).
    The state is left at the start of the last segment. */
 int fgpe_adjoint_forward(fgpe_ctx* ctx, double dt, double t0, int nsteps, int ckpt_every);
 int fgpe_adjoint_backward(fgpe_ctx* ctx, const void* dev_lamT, void* dev_lam0, double* dev_gradV,

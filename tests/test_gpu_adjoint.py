@@ -317,3 +317,120 @@ def test_adjoint_with_a_ramp(torch_cuda):
     qm = tuple(v - h if i < 3 else v for i, v in enumerate(q))
     fd = (J_of(forward(V, qp), target, c.dv)[0] - J_of(forward(V, qm), target, c.dv)[0]) / (2 * h)
     assert abs(g[0] - fd) <= 1e-6 * abs(fd), (g[0], fd)
+
+
+def two_comp_setup():
+    c = Case(32, 32, 64, L=(12.0, 12.0, 16.0))
+    V1 = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.5)).real
+    V = np.stack([V1, V1 + 0.1])
+    p1 = configs.gaussian_noisy(c.spec, 7, offset=(1.0, 0, 0), noise=0.02)
+    p2 = configs.gaussian_noisy(c.spec, 8, offset=(-1.0, 0, 0), noise=0.02)
+    psi0 = np.stack([p1, p2 * np.exp(0.3j)])
+    t1 = configs.normalise(np.exp(-0.6 * configs.harmonic_trap(c.spec, (1, 1, 0.4)).real) *
+                           np.exp(0.2j * c.rand_psi(45).real), c.dv)
+    t2 = configs.normalise(np.exp(-0.5 * configs.harmonic_trap(c.spec, (1, 1, 0.6)).real) *
+                           np.exp(-0.3j * c.rand_psi(46).real), c.dv)
+    f = 1.0 / (4 * np.pi)
+    a = (60 * f, 50 * f, 40 * f)   # a_11, a_22, a_12
+    return c, V, psi0, np.stack([t1, t2]), a
+
+
+def J2(psiT, target, dv):
+    ov = np.sum(np.conj(target) * psiT, axis=(1, 2, 3)) * dv
+    return float(np.sum(np.abs(ov) ** 2)), ov
+
+
+def ham2(ctx, a, U):
+    if U:
+        ctx.set_hamiltonian(ncomp=2, a=a, sc_U=U, sc_R=1.5, sc_dim=3)
+    else:
+        ctx.set_hamiltonian(ncomp=2, a=a)
+
+
+def fwd2(c, V, a, U, psi0, dt, n, prec=0):
+    with c.ctx(prec) as ctx:
+        ham2(ctx, a, U)
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        ctx.rk4_steps(dt, n)
+        return ctx.download_psi()
+
+
+def grad2(torch, c, V, a, U, psi0, target, dt, n, every, prec=0):
+    with c.ctx(prec) as ctx:
+        ham2(ctx, a, U)
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        ctx.adjoint_forward(dt, n, every)
+        J, ov = J2(ctx.download_psi(), target, c.dv)
+        lamT = torch.from_numpy(np.ascontiguousarray(2 * ov[:, None, None, None] * c.dv * target)).cuda()
+        gV = torch.empty((2,) + c.shape, dtype=torch.float64, device="cuda")
+        lam0 = torch.empty_like(lamT)
+        g = ctx.adjoint_backward(lamT.data_ptr(), lam0.data_ptr(), gV.data_ptr())
+        ctx.sync()
+        return J, gV.cpu().numpy(), g, lam0.cpu().numpy()
+
+
+@pytest.mark.parametrize("U", [0.0, 5.0])
+def test_two_component_adjoint_against_finite_differences(torch_cuda, U):
+    """Two coupled components (contact, and with the soft-core kernel on the
+    total density): dJ/dV_c, dJ/da_11, dJ/da_22, dJ/da_12, dJ/dU and dJ/dpsi_0
+    against central differences of the HIP forward, one V direction against
+    the CPU oracle's."""
+    c, V, psi0, target, a = two_comp_setup()
+    dt, n, every = 5e-4, 24, 5
+    J, gV, g, lam0 = grad2(torch_cuda, c, V, a, U, psi0, target, dt, n, every)
+    assert abs(J - J2(fwd2(c, V, a, U, psi0, dt, n), target, c.dv)[0]) < 1e-14
+    h = 1e-4
+
+    def fd(**kw):
+        Vx, ax, Ux, px = kw.get("V", V), kw.get("a", a), kw.get("U", U), kw.get("psi", psi0)
+        return J2(fwd2(c, Vx, ax, Ux, px, dt, n), target, c.dv)[0]
+
+    for comp in range(2):
+        dV = np.zeros_like(V)
+        dV[comp] = smooth_field(c, 600 + comp)
+        f = (fd(V=V + h * dV) - fd(V=V - h * dV)) / (2 * h)
+        ad = float(np.sum(gV * dV))
+        assert abs(ad - f) <= 1e-6 * abs(f), (comp, ad, f)
+    for k, idx in ((0, 0), (3, 1), (4, 2)):
+        # Richardson-extrapolated central differences (steps 2h and h): the
+        # a_12 derivative is small here, and the plain difference's h^2 term is
+        # not negligible against it
+        fs = []
+        for hh in (2 * h, h):
+            ap = list(a); am = list(a)
+            ap[idx] += hh; am[idx] -= hh
+            fs.append((fd(a=tuple(ap)) - fd(a=tuple(am))) / (2 * hh))
+        f = (4 * fs[1] - fs[0]) / 3
+        assert abs(g[k] - f) <= 1e-6 * abs(f), (k, g[k], f, fs)
+    if U:
+        f = (fd(U=U + h) - fd(U=U - h)) / (2 * h)
+        assert abs(g[2] - f) <= 1e-6 * abs(f), (g[2], f)
+    else:
+        assert g[1] == 0.0 and g[2] == 0.0
+    dpsi = np.stack([c.rand_psi(78, 1e-2), c.rand_psi(79, 1e-2)])
+    f = (fd(psi=psi0 + h * dpsi) - fd(psi=psi0 - h * dpsi)) / (2 * h)
+    ad = float(np.sum(np.real(np.conj(lam0) * dpsi)))
+    assert abs(ad - f) <= 1e-6 * abs(f), (ad, f)
+    # the FP64 reference: central differences of the CPU oracle along one direction
+    dV = np.stack([smooth_field(c, 610), smooth_field(c, 611)])
+    vt = c.vtilde(0.0, sc=(3, 1.5, U)) if U else None
+    Js = []
+    for sgn in (1, -1):
+        out = orc.rk4(c.grid, orc.Ham(2, a, vtilde=vt), psi0, (V + sgn * h * dV).astype(np.complex128), dt, n)
+        Js.append(J2(out, target, c.dv)[0])
+    f = (Js[0] - Js[1]) / (2 * h)
+    ad = float(np.sum(gV * dV))
+    assert abs(ad - f) <= 1e-6 * abs(f), (ad, f)
+
+
+def test_two_component_adjoint_mixed_precision(torch_cuda):
+    c, V, psi0, target, a = two_comp_setup()
+    dt, n, every = 5e-4, 24, 5
+    _, g64, s64, _ = grad2(torch_cuda, c, V, a, 5.0, psi0, target, dt, n, every, prec=0)
+    _, g32, s32, _ = grad2(torch_cuda, c, V, a, 5.0, psi0, target, dt, n, every, prec=1)
+    rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    assert rel < 1e-4, rel
+    for k in (0, 2, 3, 4):
+        assert abs(s32[k] - s64[k]) <= 1e-4 * abs(s64[k]), (k, s32[k], s64[k])
