@@ -108,7 +108,7 @@ struct fgpe_ctx {
     // adjoint: checkpoints, segment states, stage states, VJP buffers
     std::vector<double2*> ckpt;
     std::vector<double2*> segbuf;
-    void* ystage[3] = {nullptr, nullptr, nullptr};
+    std::vector<void*> ystep;   // per step of a segment: its stage states y1..y3 (3 x adj_nc bulk states)
     void* spec3 = nullptr;
     double2* lam[3] = {nullptr, nullptr, nullptr};   // LIN, LAM, GB
     double* pacc = nullptr;

@@ -361,10 +361,14 @@ template <typename T> int renorm_now(fgpe_ctx* c)
 }
 
 // RK4 steps; imaginary: c = -dt and renormalisation to npart after each step
+// keep (one step only): a buffer of 3 x ncomp bulk states that receives the
+// step's stage states y1, y2, y3 (stage s, component i at ((s - 1) ncomp + i)
+// states in), which the stages then read from there instead of PS / PSb.
 template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps, bool imaginary,
-                                  const double* npart, double* hist_dev)
+                                  const double* npart, double* hist_dev, void* keep = nullptr)
 {
     FGPE_RET(ensure_ready(c));
+    if (keep && nsteps != 1) return fail(FGPE_EINVAL, "stage states are kept for one step at a time");
     const int nc = c->hp.ncomp;
     const bool dens = c->nch > nc;
     if (!c->spec_valid) FGPE_RET(prologue<T>(c));
@@ -392,7 +396,16 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
             a.stage = s;
             a.scale = scale;
             a.want_norm = (imaginary && s == 4) ? 1 : 0;
-            if (keep1)
+            if (keep) {
+                auto y = [&](int st, int i) {
+                    return (c2_t<T>*)keep + ((size_t)(st - 1) * nc + i) * (size_t)c->ntot;
+                };
+                for (int i = 0; i < nc; ++i) {
+                    if (s > 1) a.PS[i] = y(s - 1, i);   // stage 1 reads the state
+                    if (s < 4) a.PSo[i] = y(s, i);
+                    if (keep1 && s == 3) a.PS1[i] = y(1, i);
+                }
+            } else if (keep1) {
                 for (int i = 0; i < nc; ++i) {
                     c2_t<T>* const ps1 = (c2_t<T>*)c->PS[i];
                     c2_t<T>* const ps2 = (c2_t<T>*)c->PSb[i];
@@ -403,6 +416,7 @@ template <typename T> int rk4_run(fgpe_ctx* c, double dt, double t0, int nsteps,
                         a.PSo[i] = ps1;
                     }
                 }
+            }
             FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_RK));
             if (imaginary && s == 4) {
                 FGPE_RET(finalize(c, nb, FIN_RENORM, npart, hist_dev ? hist_dev + 3 * nc * it : c->scal + SC_TMP,
@@ -611,27 +625,24 @@ int adj_check(fgpe_ctx* c)
 }
 
 // The adjoint's buffers, sized for nc components: checkpoints and segment
-// states (FP64, nc states each), stage states (bulk), the VJP's spectral
-// channels (nc + 2), the adjoint vectors and the parameter accumulators.
-// A context that switches to more components drops the smaller ones.
+// states (FP64, nc states each), per-step stage states (allocated by the
+// backward sweep), the VJP's spectral channels (nc + 2), the adjoint vectors
+// and the parameter accumulators.  A context that switches to more
+// components drops the smaller ones.
 int adj_reserve(fgpe_ctx* c, int nc)
 {
     if (c->adj_nc >= nc) return FGPE_OK;
     const size_t N = (size_t)c->ntot;
     for (double2*& p : c->ckpt) dfree(c, (void**)&p, sizeof(double2) * N * c->adj_nc);
     for (double2*& p : c->segbuf) dfree(c, (void**)&p, sizeof(double2) * N * c->adj_nc);
+    for (void*& p : c->ystep) dfree(c, &p, 3 * 2 * c->tsz() * N * c->adj_nc);
     c->ckpt.clear();
     c->segbuf.clear();
-    for (int i = 0; i < 3; ++i) {
-        dfree(c, &c->ystage[i], 2 * c->tsz() * N * c->adj_nc);
-        dfree(c, (void**)&c->lam[i], sizeof(double2) * N * c->adj_nc);
-    }
+    c->ystep.clear();
+    for (int i = 0; i < 3; ++i) dfree(c, (void**)&c->lam[i], sizeof(double2) * N * c->adj_nc);
     dfree(c, &c->spec3, (size_t)(c->adj_nc + 2) * 2 * c->tsz() * N);
     c->adj_nc = nc;
-    for (int i = 0; i < 3; ++i) {
-        FGPE_RET(dalloc(c, &c->ystage[i], 2 * c->tsz() * N * nc));
-        FGPE_RET(dalloc(c, (void**)&c->lam[i], sizeof(double2) * N * nc));
-    }
+    for (int i = 0; i < 3; ++i) FGPE_RET(dalloc(c, (void**)&c->lam[i], sizeof(double2) * N * nc));
     FGPE_RET(dalloc(c, &c->spec3, (size_t)(nc + 2) * 2 * c->tsz() * N));
     if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)(c->ntot / 32 + 1024)));
     c->adj_ready = false;
@@ -712,6 +723,11 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
         FGPE_RET(dalloc(c, (void**)&p, sizeof(double2) * N * c->adj_nc));
         c->segbuf.push_back(p);
     }
+    while ((int)c->ystep.size() < every) {
+        void* p = nullptr;
+        FGPE_RET(dalloc(c, &p, 3 * bT * c->adj_nc));
+        c->ystep.push_back(p);
+    }
     double* ptmp = c->pacc + ns * nb;
     FGPE_HIPCHK(hipMemsetAsync(c->pacc, 0, sizeof(double) * ns * nb, c->stream));
     FGPE_HIPCHK(hipMemsetAsync(vbar, 0, sizeof(double) * N * nc, c->stream));
@@ -738,39 +754,21 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int len = std::min(every, nsteps - seg * every);
         const int step0 = seg * every;
-        // recompute the states of this segment
+        // recompute the states of this segment, keeping every step's stage
+        // states y1..y3 (src/oneComponentGPSolver.cu:165-182,
+        // src/twoComponentGPSolver.cu:150-182), stage s evaluating H at PARS[T]
+        // as the forward step did
         FGPE_RET(copy_states(c, c->ckpt[seg], false));
         c->spec_valid = false;
         for (int j = 0; j < len; ++j) {
             FGPE_RET(copy_states(c, c->segbuf[j], true));
-            if (j + 1 < len) FGPE_RET(rk4_run<T>(c, dt, c->adj_t0 + (step0 + j) * dt, 1, false, one, nullptr));
+            FGPE_RET(rk4_run<T>(c, dt, c->adj_t0 + (step0 + j) * dt, 1, false, one, nullptr, c->ystep[j]));
         }
         for (int j = len - 1; j >= 0; --j) {
             double2* Y0 = c->segbuf[j];
             const double tj = c->adj_t0 + (double)(step0 + j) * dt;   // this step's start
-            // stage states y1, y2, y3 of this step (src/oneComponentGPSolver.cu:165-182,
-            // src/twoComponentGPSolver.cu:150-182), stage s evaluating H at PARS[T]
-            // as the forward step did
-            {
-                XArgs<T> a = base_xargs<T>(c, tj);
-                for (int i = 0; i < nc; ++i) a.P0[i] = Y0 + (size_t)i * N;
-                FGPE_RET(Launch<T>::xstage(c, a, nc, dens, MODE_PROLOGUE));
-                for (int s = 1; s <= 3; ++s) {
-                    FGPE_RET(spectral_passes<T>(c, nullptr, nullptr));
-                    XArgs<T> b = base_xargs<T>(c, s == 1 ? tj : tj + 0.5 * dt);
-                    for (int i = 0; i < nc; ++i) {
-                        b.P0[i] = Y0 + (size_t)i * N;
-                        b.PS[i] = (c2_t<T>*)comp(s == 1 ? c->ystage[0] : c->ystage[s - 2], i, bT);
-                        b.PSo[i] = (c2_t<T>*)comp(c->ystage[s - 1], i, bT);
-                        if (sizeof(T) == 4 && s == 3) b.PS1[i] = (c2_t<T>*)comp(c->ystage[0], i, bT);   // stage 1's state
-                    }
-                    b.cr = 0.0;
-                    b.ci = (s <= 2) ? -0.5 * dt : -1.0 * dt;
-                    b.stage = s;
-                    FGPE_RET(Launch<T>::xstage(c, b, nc, dens, MODE_RK));
-                }
-                c->spec_valid = false;
-            }
+            void* const ys = c->ystep[j];
+            auto ystage = [&](int s) { return (void*)((char*)ys + (size_t)(s - 1) * nc * bT); };
             AdjArgs a{};
             a.spec = c->spec3;
             a.ntot = c->ntot;
@@ -785,7 +783,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
                 for (int i = 0; i < nc; ++i) a.y[i] = comp(base, i, stride);
             };
             // s = 3: abar = conj(c)/6 lam_in at y3; lam = lam_in + G; next abar = conj(c)/3 lam_in + conj(c) G
-            set_y(c->ystage[2], bT);
+            set_y(ystage(3), bT);
             set_g4(a, tj + dt);
             for (int i = 0; i < nc; ++i) {
                 a.src[i] = LIN + (size_t)i * N;
@@ -797,7 +795,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             a.gg = cc;
             FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp, ns));
             // s = 2 at y2: next abar = conj(c)/3 lam_in + conj(c)/2 G
-            set_y(c->ystage[1], bT);
+            set_y(ystage(2), bT);
             set_g4(a, tj + 0.5 * dt);
             for (int i = 0; i < nc; ++i) a.src[i] = nullptr;
             a.init = 0;
@@ -805,7 +803,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             a.gg = scl(0.5, cc);
             FGPE_RET(vjp<T>(c, a, sizeof(T) == 8, dens, ptmp, ns));
             // s = 1 at y1: next abar = conj(c)/6 lam_in + conj(c)/2 G
-            set_y(c->ystage[0], bT);
+            set_y(ystage(1), bT);
             set_g4(a, tj + 0.5 * dt);
             a.ga = scl(1.0 / 6.0, cc);
             a.gg = scl(0.5, cc);
@@ -1342,12 +1340,13 @@ int fgpe_ctx_destroy(fgpe_ctx* c)
     void* ptrs[] = {c->k[0], c->k[1], c->k[2], c->k32[0], c->k32[1], c->k32[2], c->k64[0], c->k64[1], c->k64[2], c->tw64[0], c->tw64[1], c->tw64[2], c->tw32[0], c->tw32[1],
                     c->tw32[2], c->P0[0], c->P0[1], c->PS[0], c->PS[1], c->PSb[0], c->PSb[1], c->K[0], c->K[1], c->Vr[0], c->Vr[1],
                     c->Vi[0], c->Vi[1], c->spec, c->wtab, c->partials, c->scal, c->flags, c->hist,
-                    c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->axbuf, c->ystage[0], c->ystage[1], c->ystage[2], c->spec3, c->lam[0],
+                    c->aux, c->xbuf, c->vstage[0], c->vstage[1], c->axbuf, c->spec3, c->lam[0],
                     c->lam[1], c->lam[2], c->pacc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (double2* p : c->ckpt) (void)hipFree(p);
     for (double2* p : c->segbuf) (void)hipFree(p);
+    for (void* p : c->ystep) (void)hipFree(p);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;

@@ -68,12 +68,12 @@ def adjoint_bytes_per_point(mixed: bool, nsteps: int, every: int, nx: int = 128)
     one-component dipolar GPE (fgpe_adjoint_forward + fgpe_adjoint_backward):
 
       forward     nsteps RK4 steps, a checkpoint copy (32 B) per segment
-      recompute   per segment: the checkpoint copied back, every step's state
-                  copied to the segment buffer (32 B), every step but the
-                  last of the segment stepped again
-      per step    the stage states y1..y3 again (an x prologue of the state,
-                  then three RK stages with their spectral passes), and four
-                  vector-Jacobian products: k_adj_pro (adjoint seed in, state
+      recompute   per segment: the checkpoint copied back and its x prologue,
+                  every step's state copied to the segment buffer (32 B) and
+                  the step run again, its RK stages writing their stage states
+                  y1..y3 to per-step buffers instead of the stage buffer (the
+                  same bytes)
+      per step    four vector-Jacobian products: k_adj_pro (adjoint seed in, state
                   in, three spectral channels out), the y / last / y passes on
                   three complex channels, k_adj_epi (channels, state, seed,
                   trap, lambda in and out, next seed out, dV accumulated).
@@ -84,9 +84,7 @@ def adjoint_bytes_per_point(mixed: bool, nsteps: int, every: int, nx: int = 128)
     step = step_bytes_per_point(mixed, 1, True, nx)
     nseg = (nsteps + every - 1) // every
     fwd = nsteps * step + nseg * 32
-    recompute = nseg * 32 + nsteps * 32 + (nsteps - nseg) * step
-    stages = (s0 + sc * (1 + h)) + sum(3 * pass_bytes(mixed, 1, True, nx) + xstage_bytes(s, mixed, 1, True, nx)
-                                       for s in (1, 2, 3))
+    recompute = nseg * (32 + s0 + sc * (1 + h)) + nsteps * (32 + step)
     passes3 = 3 * 2 * 3 * sc
     vjp = 0.0
     for s in (3, 2, 1, 0):
@@ -94,4 +92,4 @@ def adjoint_bytes_per_point(mixed: bool, nsteps: int, every: int, nx: int = 128)
         pro = (32 if s == 3 else 16) + y + 3 * sc
         epi = 3 * sc + y + 16 + sv + 16 + (0 if s == 3 else 16) + 16 + (16 if s > 0 else 0) + 16
         vjp += pro + passes3 + epi
-    return fwd + recompute + nsteps * (stages + vjp)
+    return fwd + recompute + nsteps * vjp

@@ -278,7 +278,10 @@ int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
    O(sqrt(nsteps)) states); psi(T) is then the current state.
    fgpe_adjoint_backward takes dev_lamT = dJ/dRe(psi) + i dJ/dIm(psi) at T
    (ncomp x N complex doubles, component-major, device), recomputes each
-   segment from its checkpoint, and writes dJ/dpsi(0) in the same convention
+   segment from its checkpoint (keeping, per step of the segment, the FP64
+   state and the three RK stage states in the context's precision: device
+   memory of 4 ckpt_every (FP64) or 2.5 ckpt_every (mixed) FP64 states per
+   component besides the checkpoints), and writes dJ/dpsi(0) in the same convention
    to dev_lam0 (or NULL), dJ/dV_c to dev_gradV (ncomp x N doubles) and
    host_grad (5 doubles): [0] dJ/da (one component) or dJ/da_11, [1] dJ/da_dd,
    [2] dJ/dU, [3] dJ/da_22, [4] dJ/da_12 (0 where a parameter is absent).
