@@ -244,10 +244,10 @@ template <typename T> int kmul_1d(fgpe_ctx* c, void* spec, int nch, int nc, cons
 {
     LastArgs a = last_args(c, spec, 1, nch, nc, scale, filter, stop, half, vpx);
     a.s.nx = c->n[0];
-    // the real-field channel's half spectrum: kx = 0..nx/2 (row pitch irrelevant, one row)
+    // the real-field channels' half spectra: kx = 0..nx/2 (one row each, hchan apart)
     if (!half || nch == nc) a.s.hcols = 0;
     if (nch > nc && !half) return fail(FGPE_EUNSUP, "1D grids keep the density as a half spectrum");
-    const int total = nc * c->n[0] + (nch > nc ? c->n[0] / 2 + 1 : 0);
+    const int total = nc * c->n[0] + (nch > nc ? (nch - nc) * (c->n[0] / 2 + 1) : 0);
     TScope ts(c, FGPE_KCLASS_LAST);
     hipLaunchKernelGGL(k_kmul_1d<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, a);
     FGPE_LAUNCHCHK("k_kmul_1d");
@@ -694,7 +694,7 @@ template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, doubl
 {
     const int nc = c->hp.ncomp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, false));
-    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? nc + 2 : nc, nc, nullptr, nullptr, false));
+    FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? nc + 2 : nc, nc, nullptr, nullptr, true));
     a.partials = ptmp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, true));
     const long long n = (long long)ns * Launch<T>::xblocks(c);
@@ -772,6 +772,8 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             AdjArgs a{};
             a.spec = c->spec3;
             a.ntot = c->ntot;
+            a.hpitch = (int)dens_pitch(c);
+            a.hchan = dens_pitch(c) * c->n[1] * c->n[2];
             for (int i = 0; i < nc; ++i) {
                 a.Vr[i] = c->Vr[i];
                 a.lam_in[i] = LIN + (size_t)i * N;

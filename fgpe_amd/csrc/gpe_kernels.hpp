@@ -533,7 +533,7 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
 // transforms (there is no pass along another axis to carry it), the same
 // factors as the last-axis pass with k = k_x: (19.739208802/N) k^2 - vpx_c k
 // on component c (src/parallel.cu:194-205) and Vs(k)/N on the real-field
-// channel's half spectrum (kx = 0..N/2).  One thread per spectral element,
+// channels' half spectra (kx = 0..N/2).  One thread per spectral element,
 // complex channels first (natural order), factors in FP64.  A pending scale
 // multiplies component c by scale[c] and the density by scale[0]^2.
 template <typename T> __global__ void __launch_bounds__(256) k_kmul_1d(LastArgs a)
@@ -541,7 +541,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_kmul_1d(LastArgs 
     using T2 = c2_t<T>;
     if (a.s.stop && *glob(a.s.stop)) return;
     const int nx = a.s.nx, nc = a.s.ncomp;
-    const int nh = a.s.nchan > nc ? a.s.hcols : 0;
+    const int nh = a.s.nchan > nc ? (a.s.nchan - nc) * a.s.hcols : 0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc * nx + nh) return;
     const gp<T2> data = glob((T2*)a.s.data);
@@ -557,7 +557,8 @@ template <typename T> __global__ void __launch_bounds__(256) k_kmul_1d(LastArgs 
         data[at] = from_d2<T>(double2{(a.lap_scale * z.x) * (k * k) - w * z.x, (a.lap_scale * z.y) * (k * k) - w * z.y});
         return;
     }
-    const int e = i - nc * nx;
+    const int j = i - nc * nx, ch = j / a.s.hcols;   // real-field channel ch, element e
+    const int e = j - ch * a.s.hcols;
     const double k = kx[e].x, kp = kx[e].y;
     double vt = 0.0;
     if (a.c_dd != 0.0) {
@@ -567,8 +568,9 @@ template <typename T> __global__ void __launch_bounds__(256) k_kmul_1d(LastArgs 
     if (a.wtab) vt = vt + (sizeof(T) == 8 ? (double)glob((const double*)a.wtab)[e] : (double)glob((const float*)a.wtab)[e]);
     double f = vt * a.inv_ntot;
     if (a.s.scale) f = f * (scale[0] * scale[0]);
-    const T2 v = data[a.s.hbase + e];
-    data[a.s.hbase + e] = T2{(T)((double)v.x * f), (T)((double)v.y * f)};
+    const long long at = a.s.hbase + ch * a.s.hchan + e;
+    const T2 v = data[at];
+    data[at] = T2{(T)((double)v.x * f), (T)((double)v.y * f)};
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,11 +1024,16 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
 // d(strength) += Phi1[n] r for the k-space kernel (Phi1 = Phi / strength: the
 // kernel is linear in a_dd, or in the soft-core U).  One VJP is the prologue
 // below, the spectral passes on NC + 2 channels (abar_c: kinetic; n and r:
-// the k-space kernel), and the epilogue.
+// the k-space kernel, both real fields kept as half spectra like the
+// solver's density), and the epilogue.  Pencils past the last row (grids
+// with fewer rows than a block's pencils, e.g. 1D) do no work: the x
+// exchanges are wave-scoped.
 
 struct AdjArgs {
     void* spec;             // channels of T2: abar_c (c < NC), then n, r
     long long ntot;
+    long long hchan;        // n and r: half spectra (kx = 0..N/2), n at spec + NC ntot,
+    int hpitch;             // r hchan elements on; row r at r * hpitch
     const void* y[2];       // state components the Jacobian is taken at (YT2)
     const double2* src[2];  // prologue: abar_c = alpha * src_c (null: abar read from `abar`)
     double2 alpha;
@@ -1056,7 +1063,9 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ twg)
     __shared__ T2 twl[N];
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
-    const long long row0 = ((long long)blockIdx.x * P + p) * N;
+    const long long row = (long long)blockIdx.x * P + p;
+    if (row >= a.ntot / N) return;
+    const long long row0 = row * N;
     T2* my = lds + p * lds_stride<N>();
     const gp<T2> sp = glob((T2*)a.spec);
     T2 v[E];
@@ -1089,16 +1098,23 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ twg)
         for (int q = 0; q < E; ++q) sp[c * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
     }
     if (DENS) {
+        const gp<T2> hn = sp + NC * a.ntot + row * a.hpitch, hr = hn + a.hchan;
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)nn[q], (T)0};
         fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) sp[NC * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) {
+            const int e = elemB<N, V>(t, q);
+            if (e <= N / 2) hn[e] = v[q];
+        }
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)rr[q], (T)0};
         fft_forward<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
-        for (int q = 0; q < E; ++q) sp[(NC + 1) * a.ntot + row0 + elemB<N, V>(t, q)] = v[q];
+        for (int q = 0; q < E; ++q) {
+            const int e = elemB<N, V>(t, q);
+            if (e <= N / 2) hr[e] = v[q];
+        }
     }
 }
 
@@ -1114,86 +1130,103 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
     __shared__ T2 twl[N];
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
     const int t = threadIdx.x % TT, p = threadIdx.x / TT;
-    const long long row0 = ((long long)blockIdx.x * P + p) * N;
+    const long long rowi = (long long)blockIdx.x * P + p;
+    const bool pv = rowi < a.ntot / N;
+    const long long row = pv ? rowi : 0;
+    const long long row0 = row * N;
     T2* my = lds + p * lds_stride<N>();
     const gp<T2> sp = glob((T2*)a.spec);
     T2 v[E];
     double phin[E], phir[E];
-    if (DENS) {
+    double s[NS];
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[NC * a.ntot + row0 + elemB<N, V>(t, q)];
+    for (int k = 0; k < NS; ++k) s[k] = 0.0;
+    // the half spectra of a real field: kx > N/2 from the conjugate of N - kx
+    auto load_half = [&](gp<const T2> h) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = elemB<N, V>(t, q);
+            if (e <= N / 2) {
+                v[q] = h[e];
+            } else {
+                const T2 w = h[N - e];
+                v[q] = T2{w.x, -w.y};
+            }
+        }
+    };
+    if (DENS && pv) {
+        const gp<const T2> hn = sp + NC * a.ntot + row * a.hpitch;
+        load_half(hn);
         fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phin[q] = (double)v[q].x;
-#pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[(NC + 1) * a.ntot + row0 + elemB<N, V>(t, q)];
+        load_half(hn + a.hchan);
         fft_inverse<T2, N, V, true>(v, t, my, tw);
 #pragma unroll
         for (int q = 0; q < E; ++q) phir[q] = (double)v[q].x;
     }
-    double s[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) s[k] = 0.0;
     // NC == 2: component 0's n and r, kept for component 1's coupling terms
     // (gb_out may alias abar, so component 0's abar is overwritten by then)
     double nsv[E], rsv[E];
+    if (pv) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        const gp<const Y2> y = glob((const Y2*)a.y[c]);
-        const gp<const Y2> yo = glob((const Y2*)a.y[1]);
-        const gp<const double2> abar = glob((const double2*)a.abar[c]);
-        const gp<const double2> abo = glob((const double2*)a.abar[1]);
-        const gp<const T> Vr = glob((const T*)a.Vr[c]);
-        const gp<double2> lam = glob(a.lam[c]), gb_out = glob(a.gb_out[c]);
-        const gp<const double2> lam_in = glob(a.lam_in[c]);
-        const gp<double> vbar = glob(a.vbar[c]);
-        const double gs = a.g4[c][c], go = a.g4[c][1 - c];
+        for (int c = 0; c < NC; ++c) {
+            const gp<const Y2> y = glob((const Y2*)a.y[c]);
+            const gp<const Y2> yo = glob((const Y2*)a.y[1]);
+            const gp<const double2> abar = glob((const double2*)a.abar[c]);
+            const gp<const double2> abo = glob((const double2*)a.abar[1]);
+            const gp<const T> Vr = glob((const T*)a.Vr[c]);
+            const gp<double2> lam = glob(a.lam[c]), gb_out = glob(a.gb_out[c]);
+            const gp<const double2> lam_in = glob(a.lam_in[c]);
+            const gp<double> vbar = glob(a.vbar[c]);
+            const double gs = a.g4[c][c], go = a.g4[c][1 - c];
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = sp[c * a.ntot + row0 + elemB<N, V>(t, q)];
-        fft_inverse<T2, N, V, true>(v, t, my, tw);   // T abar_c
+            for (int q = 0; q < E; ++q) v[q] = sp[c * a.ntot + row0 + elemB<N, V>(t, q)];
+            fft_inverse<T2, N, V, true>(v, t, my, tw);   // T abar_c
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const long long g = row0 + elemA<N, V>(t, q);
-            const Y2 yr = y[g];
-            const double2 yv = double2{(double)yr.x, (double)yr.y};
-            const double2 ab = abar[g];
-            const double n = yv.x * yv.x + yv.y * yv.y;
-            const double r = yv.x * ab.x + yv.y * ab.y;
-            double no = 0.0, ro = 0.0;   // the other component's density and r
-            if (NC == 2) {
-                if (c == 0) {
-                    const Y2 yq = yo[g];
-                    const double2 yw = double2{(double)yq.x, (double)yq.y};
-                    const double2 aq = abo[g];
-                    no = yw.x * yw.x + yw.y * yw.y;
-                    ro = yw.x * aq.x + yw.y * aq.y;
-                    nsv[q] = n;
-                    rsv[q] = r;
-                } else {
-                    no = nsv[q];
-                    ro = rsv[q];
+            for (int q = 0; q < E; ++q) {
+                const long long g = row0 + elemA<N, V>(t, q);
+                const Y2 yr = y[g];
+                const double2 yv = double2{(double)yr.x, (double)yr.y};
+                const double2 ab = abar[g];
+                const double n = yv.x * yv.x + yv.y * yv.y;
+                const double r = yv.x * ab.x + yv.y * ab.y;
+                double no = 0.0, ro = 0.0;   // the other component's density and r
+                if (NC == 2) {
+                    if (c == 0) {
+                        const Y2 yq = yo[g];
+                        const double2 yw = double2{(double)yq.x, (double)yq.y};
+                        const double2 aq = abo[g];
+                        no = yw.x * yw.x + yw.y * yw.y;
+                        ro = yw.x * aq.x + yw.y * aq.y;
+                        nsv[q] = n;
+                        rsv[q] = r;
+                    } else {
+                        no = nsv[q];
+                        ro = rsv[q];
+                    }
                 }
-            }
-            const double pn = DENS ? phin[q] : 0.0;
-            const double pr = DENS ? phir[q] : 0.0;
-            const double w = (double)Vr[g] + gs * n + go * no + pn;
-            const double c2 = 2.0 * (gs * r + go * ro + pr);
-            const double2 G = double2{(double)v[q].x + w * ab.x + c2 * yv.x, (double)v[q].y + w * ab.y + c2 * yv.y};
-            const double2 li = lam_in[g];
-            const double2 base = a.init ? li : lam[g];
-            lam[g] = double2{base.x + G.x, base.y + G.y};
-            if (a.gb_out[0]) {
-                const double2 u = zmul(a.ga, li), wv = zmul(a.gg, G);
-                gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
-            }
-            vbar[g] += r;
-            if constexpr (NC == 1) {
-                s[0] += n * r;
-                s[1] += pn * r;
-            } else {
-                s[c] += n * r;           // a_cc
-                if (c == 0) s[2] += n * ro + no * r;   // a_12: n1 r2 + n2 r1, once
-                s[3] += pn * r;          // the kernel: Phi[n] (r1 + r2)
+                const double pn = DENS ? phin[q] : 0.0;
+                const double pr = DENS ? phir[q] : 0.0;
+                const double w = (double)Vr[g] + gs * n + go * no + pn;
+                const double c2 = 2.0 * (gs * r + go * ro + pr);
+                const double2 G = double2{(double)v[q].x + w * ab.x + c2 * yv.x, (double)v[q].y + w * ab.y + c2 * yv.y};
+                const double2 li = lam_in[g];
+                const double2 base = a.init ? li : lam[g];
+                lam[g] = double2{base.x + G.x, base.y + G.y};
+                if (a.gb_out[0]) {
+                    const double2 u = zmul(a.ga, li), wv = zmul(a.gg, G);
+                    gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
+                }
+                vbar[g] += r;
+                if constexpr (NC == 1) {
+                    s[0] += n * r;
+                    s[1] += pn * r;
+                } else {
+                    s[c] += n * r;           // a_cc
+                    if (c == 0) s[2] += n * ro + no * r;   // a_12: n1 r2 + n2 r1, once
+                    s[3] += pn * r;          // the kernel: Phi[n] (r1 + r2)
+                }
             }
         }
     }

@@ -136,9 +136,7 @@ template <typename T, typename YT, int N, int V, bool DENS, int NC>
 int adj_t(fgpe_ctx* c, const AdjArgs& a, bool epi)
 {
     constexpr int P = px_of<T, N, V>();
-    const long long rows = c->ntot / N;
-    if (rows % P) return fail(FGPE_EUNSUP, "the adjoint needs whole blocks of x rows");
-    dim3 grid((unsigned)(rows / P)), block(P * Plan<N, V>::T);
+    dim3 grid((unsigned)x_groups<P>(c, N)), block(P * Plan<N, V>::T);
     TScope ts(c, FGPE_KCLASS_XSTAGE);
     if (epi) hipLaunchKernelGGL((k_adj_epi<T, YT, N, V, DENS, NC>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));
     else hipLaunchKernelGGL((k_adj_pro<T, YT, N, V, DENS, NC>), grid, block, 0, c->stream, a, tw_of<T>(c, 0));

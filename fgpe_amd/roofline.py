@@ -74,9 +74,10 @@ def adjoint_bytes_per_point(mixed: bool, nsteps: int, every: int, nx: int = 128)
                   y1..y3 to per-step buffers instead of the stage buffer (the
                   same bytes)
       per step    four vector-Jacobian products: k_adj_pro (adjoint seed in, state
-                  in, three spectral channels out), the y / last / y passes on
-                  three complex channels, k_adj_epi (channels, state, seed,
-                  trap, lambda in and out, next seed out, dV accumulated).
+                  in, spectral channels out: the complex adjoint seed and the
+                  real n and r as half spectra), the y / last / y passes on
+                  those channels, k_adj_epi (channels, state, seed, trap,
+                  lambda in and out, next seed out, dV accumulated).
     The adjoint vectors stay FP64 (16 B) in both forms; the state the
     products are taken at is the bulk stage state, or the FP64 step input."""
     sc, sv, s0 = sizes(mixed)
@@ -85,11 +86,12 @@ def adjoint_bytes_per_point(mixed: bool, nsteps: int, every: int, nx: int = 128)
     nseg = (nsteps + every - 1) // every
     fwd = nsteps * step + nseg * 32
     recompute = nseg * (32 + s0 + sc * (1 + h)) + nsteps * (32 + step)
-    passes3 = 3 * 2 * 3 * sc
+    ch = (1 + 2 * h) * sc          # the VJP's spectral channels
+    passes3 = 3 * 2 * ch
     vjp = 0.0
     for s in (3, 2, 1, 0):
         y = s0 if s == 0 else sc
-        pro = (32 if s == 3 else 16) + y + 3 * sc
-        epi = 3 * sc + y + 16 + sv + 16 + (0 if s == 3 else 16) + 16 + (16 if s > 0 else 0) + 16
+        pro = (32 if s == 3 else 16) + y + ch
+        epi = ch + y + 16 + sv + 16 + (0 if s == 3 else 16) + 16 + (16 if s > 0 else 0) + 16
         vjp += pro + passes3 + epi
     return fwd + recompute + nsteps * vjp

@@ -434,3 +434,74 @@ def test_two_component_adjoint_mixed_precision(torch_cuda):
     assert rel < 1e-4, rel
     for k in (0, 2, 3, 4):
         assert abs(s32[k] - s64[k]) <= 1e-4 * abs(s64[k]), (k, s32[k], s64[k])
+
+
+@pytest.mark.parametrize("dims", ["1d", "2d"])
+def test_adjoint_low_dimensional_grids(torch_cuda, dims):
+    """A 1D grid (one x row, fewer rows than a block's pencils) with the 1D
+    soft-core kernel, and a 2D grid with an in-plane dipole: dJ/dV, dJ/da, the
+    kernel's strength and dJ/dpsi_0 against central differences of the HIP
+    forward, one V direction against the oracle's."""
+    if dims == "1d":
+        c = Case(128, 1, 1, L=(12.0, 1.0, 1.0))
+        ham = dict(a=(0.3,), sc_U=2.0, sc_R=1.1, sc_dim=1)
+        kname, gk = "sc_U", 2
+        vt_of = lambda h: c.vtilde(sc=(1, h["sc_R"], h["sc_U"]))
+    else:
+        c = Case(64, 32, 1, L=(12.0, 10.0, 1.0))
+        ham = dict(a=(0.4,), a_dd=0.3, dip=(1.0, 0.0, 0.0))
+        kname, gk = "a_dd", 1
+        vt_of = lambda h: c.vtilde(h["a_dd"], h["dip"])
+    V = configs.harmonic_trap(c.spec, (1.0, 1.2, 1.0)).real
+    psi0 = configs.gaussian_noisy(c.spec, 91, width=1.5, noise=0.02)
+    target = configs.normalise(np.exp(-0.4 * V) * np.exp(0.2j * c.rand_psi(92).real), c.dv)
+    dt, n, every = 5e-4, 30, 7
+
+    def forward(Vx, hx, px=psi0):
+        with c.ctx() as ctx:
+            ctx.set_hamiltonian(**hx)
+            ctx.upload_vext(Vx.astype(np.complex128))
+            ctx.upload_psi(px)
+            ctx.rk4_steps(dt, n)
+            return J_of(ctx.download_psi(), target, c.dv)[0]
+
+    with c.ctx() as ctx:
+        ctx.set_hamiltonian(**ham)
+        ctx.upload_vext(V.astype(np.complex128))
+        ctx.upload_psi(psi0)
+        ctx.adjoint_forward(dt, n, every)
+        J, ov = J_of(ctx.download_psi(), target, c.dv)
+        lamT = torch_cuda.from_numpy(2 * ov * c.dv * target).cuda()
+        gV = torch_cuda.empty(c.shape, dtype=torch_cuda.float64, device="cuda")
+        lam0 = torch_cuda.empty_like(lamT)
+        g = ctx.adjoint_backward(lamT.data_ptr(), lam0.data_ptr(), gV.data_ptr())
+        ctx.sync()
+        gV, lam0 = gV.cpu().numpy(), lam0.cpu().numpy()
+    assert abs(J - forward(V, ham)) < 1e-14
+    h = 1e-4
+    for j in range(2):
+        dV = smooth_field(c, 300 + j)
+        f = (forward(V + h * dV, ham) - forward(V - h * dV, ham)) / (2 * h)
+        ad = float(np.sum(gV * dV))
+        assert abs(ad - f) <= 1e-6 * abs(f), (j, ad, f)
+    for k, key in ((0, "a"), (gk, kname)):
+        hp, hm = dict(ham), dict(ham)
+        if key == "a":
+            hp["a"], hm["a"] = (ham["a"][0] + h,), (ham["a"][0] - h,)
+        else:
+            hp[key], hm[key] = ham[key] + h, ham[key] - h
+        f = (forward(V, hp) - forward(V, hm)) / (2 * h)
+        assert abs(g[k] - f) <= 1e-6 * abs(f), (key, g[k], f)
+    dpsi = c.rand_psi(93, 1e-2)
+    f = (forward(V, ham, psi0 + h * dpsi) - forward(V, ham, psi0 - h * dpsi)) / (2 * h)
+    ad = float(np.sum(np.real(np.conj(lam0) * dpsi)))
+    assert abs(ad - f) <= 1e-6 * abs(f), (ad, f)
+    dV = smooth_field(c, 310)
+    Js = []
+    for sgn in (1, -1):
+        out = orc.rk4(c.grid, orc.Ham(1, ham["a"], vtilde=vt_of(ham)), psi0,
+                      (V + sgn * h * dV).astype(np.complex128), dt, n)
+        Js.append(J_of(out, target, c.dv)[0])
+    f = (Js[0] - Js[1]) / (2 * h)
+    ad = float(np.sum(gV * dV))
+    assert abs(ad - f) <= 1e-6 * abs(f), (ad, f)
