@@ -505,3 +505,28 @@ def test_adjoint_low_dimensional_grids(torch_cuda, dims):
     f = (Js[0] - Js[1]) / (2 * h)
     ad = float(np.sum(gV * dV))
     assert abs(ad - f) <= 1e-6 * abs(f), (ad, f)
+
+
+def test_two_component_adjoint_through_autograd(torch_cuda):
+    """GPEEvolve with a two-component solver (V, psi0 with a leading axis of
+    2, a = (a_11, a_22, a_12)) against the C ABI's gradient, soft-core kernel on."""
+    torch = torch_cuda
+    from fgpe_amd.adjoint import AdjointSolver, GPEEvolve, fidelity
+    c, V, psi0, target, a = two_comp_setup()
+    dt, n, U = 5e-4, 24, 5.0
+    solver = AdjointSolver(c.spec, dt, n, 5, torch=torch, softcore=(3, 1.5), autotune=False, ncomp=2)
+    Vt = torch.tensor(V, dtype=torch.float64, device="cuda", requires_grad=True)
+    at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    Ut = torch.tensor(U, dtype=torch.float64, requires_grad=True)
+    p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda", requires_grad=True)
+    tg = torch.tensor(target, dtype=torch.complex128, device="cuda")
+    loss = fidelity(GPEEvolve().apply(Vt, at, Ut, p0, solver), tg, c.dv)
+    loss.backward()
+    J, gV, g, lam0 = grad2(torch, c, V, a, U, psi0, target, dt, n, 5)
+    assert abs(loss.item() - J) < 1e-14
+    assert np.abs(Vt.grad.cpu().numpy() - gV).max() <= 1e-13 * np.abs(gV).max()
+    ga = at.grad.numpy()
+    for i, k in enumerate((0, 3, 4)):
+        assert abs(ga[i] - g[k]) <= 1e-13 * abs(g[k]), (i, ga[i], g[k])
+    assert abs(Ut.grad.item() - g[2]) <= 1e-13 * abs(g[2])
+    assert np.abs(p0.grad.cpu().numpy() - lam0).max() <= 1e-13 * np.abs(lam0).max()
