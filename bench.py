@@ -265,23 +265,36 @@ def run_adjoint(torch, args):
 def run_sweep(torch, args):
     """Batched imaginary-time ground-state sweep: every instance to
     |dE/E| < 1e-9 (heavy ball, dt = 2e-3, beta = 0.9), mixed precision,
-    groups per grid shape autotuned, iterations replayed from one hipGraph of
-    50 iterations of all groups."""
+    groups per grid shape autotuned, iterations replayed from hipGraphs of 50
+    iterations: one per group, each group on its own stream (default), or one
+    graph of all groups (--sweep-mode shared)."""
     from fgpe_amd import FGPE_PREC_MIXED, configs, roofline
     from fgpe_amd.sweep import Sweep
     torch.cuda.set_device(0)
     insts = configs.sweep_instances(args.sweep_n)
-    side = torch.cuda.Stream()
     t0 = time.perf_counter()
-    sw = Sweep(insts, precision=FGPE_PREC_MIXED, stream=side.cuda_stream)
+    if args.sweep_mode == "per-group":
+        # one stream and one graph per shape group, each group replayed at its own pace
+        streams = []
+
+        def new_stream():
+            streams.append(torch.cuda.Stream())
+            return streams[-1].cuda_stream
+
+        sw = Sweep(insts, precision=FGPE_PREC_MIXED, new_stream=new_stream)
+        sync = torch.cuda.synchronize
+    else:
+        # every group on one stream, the groups on the branches of one graph
+        side = torch.cuda.Stream()
+        sw = Sweep(insts, precision=FGPE_PREC_MIXED, stream=side.cuda_stream)
+        sync = side.synchronize
     setup = time.perf_counter() - t0
     print(f"# sweep: {len(insts)} instances in {len(sw.groups)} groups {sw.shapes}, set-up {setup:.1f} s",
           file=sys.stderr, flush=True)
     tune = sw.autotune(3)
     dt, beta, eps = 2e-3, 0.9, 1e-9
     iters, conv, wall, launches = sw.run(dt, beta, eps, iters_per_graph=50, max_iters=args.sweep_max_iters,
-                                         recapture=args.sweep_recapture,
-                                         sync=side.synchronize)
+                                         recapture=args.sweep_recapture, sync=sync)
     obs = sw.energies()
     pts = np.array([i.grid.npoints for i in insts], dtype=np.float64)
     # bytes per point and iteration: x stage (heavy ball) + y/z passes, mixed precision
@@ -306,7 +319,7 @@ def run_sweep(torch, args):
         "data": "synthetic (seeded Gaussian psi0, harmonic traps)",
         "config": {"workload": f"sweep of {len(insts)} dipolar / soft-core instances",
                    "shapes": [list(s) for s in sw.shapes], "dt": dt, "beta": beta, "eps": eps,
-                   "iters_per_graph": 50},
+                   "iters_per_graph": 50, "mode": args.sweep_mode},
         "converged": int(conv.sum()), "instances": len(insts),
         "iterations": {"min": int(iters.min()), "max": int(iters.max()), "total": int(iters.sum())},
         "wall_s": round(wall, 3), "graph_launches": launches, "graph_captures": sw.captures,
@@ -317,6 +330,12 @@ def run_sweep(torch, args):
                      for k, t in tune.items()},
         "energies": [round(float(e), 10) for e in obs[:, 2]],
     }
+    if hasattr(sw, "group_finish_s"):
+        per = {}
+        for shp, m, fin in zip(sw.shapes, sw.members, sw.group_finish_s):
+            per["x".join(map(str, shp))] = {"instances": len(m), "iterations_max": int(iters[m].max()),
+                                            "iterations_total": int(iters[m].sum()), "finish_s": round(fin, 4)}
+        line["groups"] = per
     sw.close()
     print(json.dumps(line), flush=True)
     return 0
@@ -473,6 +492,8 @@ def main():
     ap.add_argument("--sweep-kernels", action="store_true",
                     help="time the sweep's batched launches per kernel class and shape group (HBM GB/s)")
     ap.add_argument("--sweep-n", type=int, default=64)
+    ap.add_argument("--sweep-mode", choices=("per-group", "shared"), default="per-group",
+                    help="sweep replays: one graph per shape group on its own stream, or one graph of all groups")
     ap.add_argument("--sweep-max-iters", type=int, default=20000)
     ap.add_argument("--sweep-recapture", type=float, default=0.5,
                     help="re-capture the graph over the unconverged instances once this share of the "

@@ -182,3 +182,37 @@ def test_sweep_shapes_against_oracle(torch_cuda, precision):
             assert e_psi < 1e-11 and e_en < 1e-11, (k, e_psi, e_en)
         else:
             assert e_en < 1e-6, (k, e_en)
+
+
+def test_sweep_per_group_streams_match_one_graph(torch_cuda):
+    """The sweep with one stream and one graph per shape group (each group
+    replayed and recaptured at its own pace) gives every instance the same
+    iteration count and the same ground state as one graph of all groups."""
+    from fgpe_amd import configs
+    from fgpe_amd.sweep import Sweep
+    torch = torch_cuda
+    insts = configs.sweep_instances(6, seed=11, shapes=((32, 32, 32), (64, 32, 16), (128, 128, 1)))
+    side = torch.cuda.Stream()
+    runs = []
+    for mode in ("shared", "per-group"):
+        keep = []
+
+        def new_stream():
+            keep.append(torch.cuda.Stream())
+            return keep[-1].cuda_stream
+
+        sw = (Sweep(insts, precision=1, stream=side.cuda_stream) if mode == "shared"
+              else Sweep(insts, precision=1, new_stream=new_stream))
+        try:
+            iters, conv, wall, launches = sw.run(DT, BETA, 1e-9, iters_per_graph=20, max_iters=4000,
+                                                 sync=torch.cuda.synchronize)
+            runs.append((iters, conv, sw.energies(), [c.download_psi() for c in sw.ctxs]))
+            print(mode, iters, conv, f"{wall:.3f} s, {launches} replays", flush=True)
+        finally:
+            sw.close()
+    (i0, c0, e0, p0), (i1, c1, e1, p1) = runs
+    assert c0.all() and c1.all()
+    assert list(i0) == list(i1)
+    assert np.array_equal(e0, e1)
+    for a, b in zip(p0, p1):
+        assert np.array_equal(a, b)
