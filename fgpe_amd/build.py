@@ -102,6 +102,17 @@ def build_lib(verbose: bool = False, variants=("", "user_example"), jobs: int = 
     return LIB
 
 
+def build_probe(verbose: bool = False) -> str:
+    """fgpe_amd/hbm_probe: the achievable HBM bandwidth of the step's access
+    patterns (contiguous streams, the FFT passes' tile pattern) on this GPU."""
+    out = os.path.join(HERE, "hbm_probe")
+    cmd = [HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], os.path.join(CSRC, "hbm_probe.hip"), "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 def build_oracle() -> None:
     subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], check=True)
 
@@ -113,6 +124,7 @@ def build_host() -> None:
 
 def main() -> None:
     build_lib(verbose=True)
+    build_probe(verbose=True)
     build_host()
     build_oracle()
 
