@@ -113,6 +113,10 @@ struct fgpe_ctx {
     double2* lam[3] = {nullptr, nullptr, nullptr};   // LIN, LAM, GB
     double* pacc = nullptr;
     int adj_nc = 0;   // components the adjoint buffers hold (0: none allocated)
+    // the k-space kernels the next spectral multiplies apply (bit 0: dipolar,
+    // bit 1: the soft-core table); the adjoint takes the dipolar one alone
+    // for one channel when both are on
+    int kmask = 3;
     int adj_every = 0, adj_nsteps = 0;
     double adj_dt = 0.0, adj_t0 = 0.0;
     bool adj_ready = false;
@@ -252,8 +256,8 @@ inline LastArgs last_args(const fgpe_ctx* c, void* data, int axis, int nch, int 
     a.dO = d[oaxis];
     a.lap_scale = 19.739208802 / (double)c->ntot;
     a.inv_ntot = 1.0 / (double)c->ntot;
-    a.c_dd = filter ? 12.0 * kPi * c->hp.a_dd : 0.0;
-    a.wtab = filter ? c->wtab : nullptr;
+    a.c_dd = (filter && (c->kmask & 1)) ? 12.0 * kPi * c->hp.a_dd : 0.0;
+    a.wtab = (filter && (c->kmask & 2)) ? c->wtab : nullptr;
     for (int i = 0; i < 2; ++i) a.vpx[i] = vpx ? c->hp.vpx[i] / (double)c->ntot : 0.0;
     return a;
 }

@@ -615,8 +615,6 @@ __global__ void k_accumulate_impl(double* __restrict__ acc, const double* __rest
 int adj_check(fgpe_ctx* c)
 {
     FGPE_RET(ensure_ready(c));
-    if (c->hp.sc_U != 0.0 && c->hp.a_dd != 0.0)
-        return fail(FGPE_EUNSUP, "the adjoint takes one k-space kernel: dipolar or soft-core");
     for (int i = 0; i < c->hp.ncomp; ++i)
         if (c->vcomplex[i]) return fail(FGPE_EUNSUP, "the adjoint needs a real trap");
     if (c->hp.vpx[0] != 0.0 || c->hp.vpx[1] != 0.0) return fail(FGPE_EUNSUP, "the adjoint has no moving-frame term");
@@ -626,9 +624,9 @@ int adj_check(fgpe_ctx* c)
 
 // The adjoint's buffers, sized for nc components: checkpoints and segment
 // states (FP64, nc states each), per-step stage states (allocated by the
-// backward sweep), the VJP's spectral channels (nc + 2), the adjoint vectors
-// and the parameter accumulators.  A context that switches to more
-// components drops the smaller ones.
+// backward sweep), the VJP's spectral channels (nc complex ones and up to
+// three half spectra), the adjoint vectors and the parameter accumulators.
+// A context that switches to more components drops the smaller ones.
 int adj_reserve(fgpe_ctx* c, int nc)
 {
     if (c->adj_nc >= nc) return FGPE_OK;
@@ -640,10 +638,10 @@ int adj_reserve(fgpe_ctx* c, int nc)
     c->segbuf.clear();
     c->ystep.clear();
     for (int i = 0; i < 3; ++i) dfree(c, (void**)&c->lam[i], sizeof(double2) * N * c->adj_nc);
-    dfree(c, &c->spec3, (size_t)(c->adj_nc + 2) * 2 * c->tsz() * N);
+    dfree(c, &c->spec3, (size_t)(c->adj_nc + 3) * 2 * c->tsz() * N);
     c->adj_nc = nc;
     for (int i = 0; i < 3; ++i) FGPE_RET(dalloc(c, (void**)&c->lam[i], sizeof(double2) * N * nc));
-    FGPE_RET(dalloc(c, &c->spec3, (size_t)(nc + 2) * 2 * c->tsz() * N));
+    FGPE_RET(dalloc(c, &c->spec3, (size_t)(nc + 3) * 2 * c->tsz() * N));
     if (!c->pacc) FGPE_RET(dalloc(c, (void**)&c->pacc, sizeof(double) * 4 * (size_t)(c->ntot / 32 + 1024)));
     c->adj_ready = false;
     return FGPE_OK;
@@ -689,12 +687,21 @@ template <typename T> int adj_forward(fgpe_ctx* c, double dt, double t0, int nst
 }
 
 // One VJP: prologue (abar_c, n, r channels), spectral passes, epilogue; the
-// epilogue's per-block parameter sums are added to the accumulators.
+// epilogue's per-block parameter sums are added to the accumulators.  With
+// both kernels on, the passes run once more over the third half channel (n
+// again) with the dipolar kernel alone.
 template <typename T> int vjp(fgpe_ctx* c, AdjArgs a, bool y64, bool dens, double* ptmp, int ns)
 {
     const int nc = c->hp.ncomp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, false));
     FGPE_RET(spectral_passes_on<T>(c, c->spec3, dens ? nc + 2 : nc, nc, nullptr, nullptr, true));
+    if (a.both) {
+        c2_t<T>* const third = (c2_t<T>*)c->spec3 + (long long)nc * c->ntot + 2 * a.hchan;
+        c->kmask = 1;
+        const int rc = spectral_passes_on<T>(c, third, 1, 0, nullptr, nullptr, true);
+        c->kmask = 3;
+        FGPE_RET(rc);
+    }
     a.partials = ptmp;
     FGPE_RET(Launch<T>::adj(c, a, y64, dens, true));
     const long long n = (long long)ns * Launch<T>::xblocks(c);
@@ -709,7 +716,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
     FGPE_RET(adj_check(c));
     if (!c->adj_ready || c->adj_nc < c->hp.ncomp) return fail(FGPE_ESTATE, "fgpe_adjoint_forward has not been run");
     const int nc = c->hp.ncomp;
-    const int ns = nc == 1 ? 2 : 4;   // parameter sums per block
+    const int ns = nc == 1 ? 3 : 5;   // parameter sums per block
     const size_t N = (size_t)c->ntot;
     const size_t b64 = sizeof(double2) * N * nc;   // nc FP64 states
     const size_t bT = 2 * c->tsz() * N;             // one bulk state
@@ -728,6 +735,8 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
         FGPE_RET(dalloc(c, &p, 3 * bT * c->adj_nc));
         c->ystep.push_back(p);
     }
+    if ((size_t)2 * ns * nb > (size_t)4 * (size_t)(c->ntot / 32 + 1024))
+        return fail(FGPE_EUNSUP, "grid too small for the adjoint's partial-sum buffer");
     double* ptmp = c->pacc + ns * nb;
     FGPE_HIPCHK(hipMemsetAsync(c->pacc, 0, sizeof(double) * ns * nb, c->stream));
     FGPE_HIPCHK(hipMemsetAsync(vbar, 0, sizeof(double) * N * nc, c->stream));
@@ -774,6 +783,7 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
             a.ntot = c->ntot;
             a.hpitch = (int)dens_pitch(c);
             a.hchan = dens_pitch(c) * c->n[1] * c->n[2];
+            a.both = (c->hp.a_dd != 0.0 && c->hp.sc_U != 0.0) ? 1 : 0;
             for (int i = 0; i < nc; ++i) {
                 a.Vr[i] = c->Vr[i];
                 a.lam_in[i] = LIN + (size_t)i * N;
@@ -825,16 +835,18 @@ template <typename T> int adj_backward(fgpe_ctx* c, const void* lamT, void* lam0
     std::vector<double> h((size_t)ns * nb);
     FGPE_HIPCHK(hipMemcpyAsync(h.data(), c->pacc, sizeof(double) * ns * nb, hipMemcpyDeviceToHost, c->stream));
     FGPE_HIPCHK(hipStreamSynchronize(c->stream));
-    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (long long b = 0; b < nb; ++b)
         for (int k = 0; k < ns; ++k) sum[k] += h[(size_t)ns * b + k];
     // the contact sums: dJ/da (or, with the a(t) ramp on, the derivative along a
-    // uniform shift of a(t)); the kernel sum <r, Phi[n]>: the kernel's strength
-    // (a_dd or U) scales Phi
-    const double sk = nc == 1 ? sum[1] : sum[3];
+    // uniform shift of a(t)); the kernel sum <r, Phi[n]>: each kernel's strength
+    // (a_dd or U) scales its part of Phi, the dipolar part <r, Phi_dd[n]> summed
+    // on its own when both are on
+    const double sk = nc == 1 ? sum[1] : sum[3], sdd = nc == 1 ? sum[2] : sum[4];
+    const bool both = c->hp.a_dd != 0.0 && c->hp.sc_U != 0.0;
     grads[0] = 4 * kPi * sum[0];
-    grads[1] = c->hp.a_dd != 0.0 ? sk / c->hp.a_dd : 0.0;
-    grads[2] = c->hp.sc_U != 0.0 ? sk / c->hp.sc_U : 0.0;
+    grads[1] = c->hp.a_dd != 0.0 ? (both ? sdd : sk) / c->hp.a_dd : 0.0;
+    grads[2] = c->hp.sc_U != 0.0 ? (both ? sk - sdd : sk) / c->hp.sc_U : 0.0;
     grads[3] = nc == 2 ? 4 * kPi * sum[1] : 0.0;
     grads[4] = nc == 2 ? 4 * kPi * sum[2] : 0.0;
     c->spec_valid = false;

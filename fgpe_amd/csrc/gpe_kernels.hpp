@@ -1025,15 +1025,18 @@ k_xstage(XArgs<T> a0, const c2_t<T>* __restrict__ twg, const XArgs<T>* __restric
 // kernel is linear in a_dd, or in the soft-core U).  One VJP is the prologue
 // below, the spectral passes on NC + 2 channels (abar_c: kinetic; n and r:
 // the k-space kernel, both real fields kept as half spectra like the
-// solver's density), and the epilogue.  Pencils past the last row (grids
-// with fewer rows than a block's pencils, e.g. 1D) do no work: the x
-// exchanges are wave-scoped.
+// solver's density), and the epilogue.  With both kernels on (`both`), n is
+// also written to a third half channel that the passes multiply by the
+// dipolar kernel alone: Phi_dd[n] r separates the two strengths' sums.
+// Pencils past the last row (grids with fewer rows than a block's pencils,
+// e.g. 1D) do no work: the x exchanges are wave-scoped.
 
 struct AdjArgs {
     void* spec;             // channels of T2: abar_c (c < NC), then n, r
     long long ntot;
     long long hchan;        // n and r: half spectra (kx = 0..N/2), n at spec + NC ntot,
-    int hpitch;             // r hchan elements on; row r at r * hpitch
+    int hpitch;             // r hchan elements on (both: n again 2 hchan on); row r at r * hpitch
+    int both;               // dipolar and soft-core kernels on together
     const void* y[2];       // state components the Jacobian is taken at (YT2)
     const double2* src[2];  // prologue: abar_c = alpha * src_c (null: abar read from `abar`)
     double2 alpha;
@@ -1047,7 +1050,8 @@ struct AdjArgs {
     double2* gb_out[2];         // next abar = ga * lam_in + gg * G (null: none; may be abar itself)
     double2 ga, gg;
     double* vbar[2];            // += r_c
-    double* partials;           // per block, NC = 1: (n r, Phi[n] r); NC = 2: (n1 r1, n2 r2, n1 r2 + n2 r1, Phi[n] r)
+    double* partials;           // per block, NC = 1: (n r, Phi[n] r, Phi_dd[n] r); NC = 2: (n1 r1, n2 r2,
+                                // n1 r2 + n2 r1, Phi[n] r, Phi_dd[n] r); Phi_dd[n] r is 0 unless both
 };
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
@@ -1107,6 +1111,14 @@ k_adj_pro(AdjArgs a, const c2_t<T>* __restrict__ twg)
             const int e = elemB<N, V>(t, q);
             if (e <= N / 2) hn[e] = v[q];
         }
+        if (a.both) {
+            const gp<T2> hd = hn + 2 * a.hchan;
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const int e = elemB<N, V>(t, q);
+                if (e <= N / 2) hd[e] = v[q];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = T2{(T)rr[q], (T)0};
         fft_forward<T2, N, V, true>(v, t, my, tw);
@@ -1125,7 +1137,7 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
     using T2 = c2_t<T>;
     using Y2 = c2_t<YT>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = px_of<T, N, V>();
-    constexpr int NS = NC == 1 ? 2 : 4;
+    constexpr int NS = NC == 1 ? 3 : 5;
     __shared__ T2 lds[P * lds_stride<N>()];
     __shared__ T2 twl[N];
     const lds_table<T2> tw = stage_twiddles<T2, N>(twl, twg);
@@ -1168,6 +1180,9 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
     // NC == 2: component 0's n and r, kept for component 1's coupling terms
     // (gb_out may alias abar, so component 0's abar is overwritten by then)
     double nsv[E], rsv[E];
+    double rt[E];   // r summed over the components
+#pragma unroll
+    for (int q = 0; q < E; ++q) rt[q] = 0.0;
     if (pv) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -1219,6 +1234,7 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
                     gb_out[g] = double2{u.x + wv.x, u.y + wv.y};
                 }
                 vbar[g] += r;
+                rt[q] += r;
                 if constexpr (NC == 1) {
                     s[0] += n * r;
                     s[1] += pn * r;
@@ -1229,6 +1245,12 @@ k_adj_epi(AdjArgs a, const c2_t<T>* __restrict__ twg)
                 }
             }
         }
+    }
+    if (DENS && pv && a.both) {
+        load_half(sp + NC * a.ntot + row * a.hpitch + 2 * a.hchan);
+        fft_inverse<T2, N, V, true>(v, t, my, tw);   // Phi_dd[n]
+#pragma unroll
+        for (int q = 0; q < E; ++q) s[NS - 1] += (double)v[q].x * rt[q];
     }
     block_sum_store<NS>(s, glob(a.partials) + NS * (long long)blockIdx.x);
 }

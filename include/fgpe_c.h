@@ -268,8 +268,8 @@ int fgpe_imag_converged(fgpe_ctx* ctx, int* converged);
    Gradients of a real objective J(psi(T)) of nsteps real-time RK4 steps with
    respect to the trap V, the contact lengths and the strength of the k-space
    kernel (a_dd or the soft-core U), by the discrete adjoint of the RK4 step:
-   one or two components, real V, at most one k-space kernel (dipolar or
-   soft-core; its direction, radius and dimension stay fixed).  With the
+   one or two components, real V, the dipolar kernel, the soft-core kernel
+   or both (their direction, radius and dimension stay fixed).  With the
    one-component a(t) ramp on (fgpe_hparams::quench_on) the ramp is part of the
    differentiated dynamics and the derivative in a is along a uniform shift of
    a(t).
