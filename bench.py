@@ -262,6 +262,64 @@ def run_adjoint(torch, args):
     return 0
 
 
+def run_config3(torch, args):
+    """The two-component soft-core workload (256^3, g11 = g22 = 100, g12 = 80,
+    soft-core U0 = 5, R = 1.5 on n1 + n2, dt = 5e-4): real-time RK4 steps/s in
+    both forms after the autotuner, with the byte model's HBM rate and the
+    mixed form's energy error against FP64 after the timed steps."""
+    from fgpe_amd import FGPE_PREC_F64, FGPE_PREC_MIXED, Context, configs, roofline
+    torch.cuda.set_device(0)
+    w = configs.twocomp_softcore_256(nsteps=args.steps)
+    g = w.grid
+    psi, V = w.initial_state(), w.trap()
+    kx, ky, kz = g.kaxes()
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, prec in (("f64", FGPE_PREC_F64), ("mixed", FGPE_PREC_MIXED)):
+        with Context(g.nx, g.ny, g.nz, prec) as ctx:
+            ctx.set_stream(stream.cuda_stream)
+            ctx.set_kspace(kx, ky, kz, g.dv)
+            ctx.set_hamiltonian(ncomp=2, a=w.a, sc_U=w.sc_U, sc_R=w.sc_R, sc_dim=3)
+            ctx.upload_psi(psi)
+            ctx.upload_vext(V)
+            ctx.autotune(5)
+            ctx.rk4_steps(w.dt, args.warmup, 0.0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            ctx.rk4_steps(w.dt, args.steps, args.warmup * w.dt)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            obs = ctx.observables((args.warmup + args.steps) * w.dt)
+            out = ctx.download_psi()
+        bpp = roofline.step_bytes_per_point(name == "mixed", 2, True, g.nx, True, table=True)
+        gbs = bpp * g.npoints / (ms * 1e-3) / 1e9
+        res[name] = {"steps_per_s": round(1e3 / ms, 3), "ms_per_step": round(ms, 4),
+                     "bytes_per_point": round(bpp, 2), "GBs": round(gbs, 1),
+                     "frac_of_peak": round(gbs / roofline.HBM_PEAK_GBS, 4),
+                     "energy": [float(obs[3 * c + 2]) for c in range(2)], "psi": out}
+        print(f"# config 3 {name}: {1e3 / ms:.1f} steps/s, {gbs:.0f} GB/s of {bpp:.0f} B/pt", file=sys.stderr,
+              flush=True)
+    f, m = res["f64"], res["mixed"]
+    e_rel = max(abs(m["energy"][c] - f["energy"][c]) / abs(f["energy"][c]) for c in range(2))
+    l2 = float(np.linalg.norm(m["psi"] - f["psi"]) / np.linalg.norm(f["psi"]))
+    for r in res.values():
+        del r["psi"]
+    line = {
+        "metric": "RK4 steps/s at 256^3 two-component soft-core GPE (FP64 plain form; mixed form beside it)",
+        "value": f["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": f["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded Gaussians offset by +-1 in x, harmonic trap)",
+        "config": {"workload": "two-component soft-core real-time RK4 256x256x256", "box": [16.0, 16.0, 16.0],
+                   "g": [100.0, 100.0, 80.0], "soft_core": {"U0": w.sc_U, "R": w.sc_R}, "dt": w.dt},
+        "f64": f, "mixed": dict(m, speedup_vs_f64=round(m["steps_per_s"] / f["steps_per_s"], 3),
+                                energy_rel_err_vs_f64=e_rel, psi_l2_rel_err_vs_f64=l2),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_sweep(torch, args):
     """Batched imaginary-time ground-state sweep: every instance to
     |dE/E| < 1e-9 (heavy ball, dt = 2e-3, beta = 0.9), mixed precision,
@@ -487,6 +545,8 @@ def main():
     ap.add_argument("--adjoint-dirs", type=int, default=0,
                     help="seeded smooth V directions checked by central differences (default: 4 in the "
                          "default line, 16 with --adjoint)")
+    ap.add_argument("--config3", action="store_true",
+                    help="time the two-component soft-core workload (256^3) in both forms instead")
     ap.add_argument("--init", action="store_true",
                     help="time the device initialisers (state and trap) at --grid instead")
     ap.add_argument("--sweep-kernels", action="store_true",
@@ -527,6 +587,8 @@ def main():
         return run_adjoint(torch, args)
     if args.init:
         return run_init(torch, args)
+    if args.config3:
+        return run_config3(torch, args)
 
     nx, ny, nz = (int(v) for v in args.grid.split(","))
     w = configs.dipolar_real(nx, ny, nz, nsteps=args.steps)
