@@ -351,7 +351,8 @@ def run_sweep(torch, args):
           file=sys.stderr, flush=True)
     tune = sw.autotune(3)
     dt, beta, eps = 2e-3, 0.9, 1e-9
-    iters, conv, wall, launches = sw.run(dt, beta, eps, iters_per_graph=50, max_iters=args.sweep_max_iters,
+    iters, conv, wall, launches = sw.run(dt, beta, eps, iters_per_graph=args.sweep_iters_per_graph,
+                                         max_iters=args.sweep_max_iters,
                                          recapture=args.sweep_recapture, sync=sync)
     obs = sw.energies()
     pts = np.array([i.grid.npoints for i in insts], dtype=np.float64)
@@ -372,12 +373,12 @@ def run_sweep(torch, args):
         "value": round(float(iters.sum()) / wall, 1),
         "unit": "instance-iterations/s",
         "n_gpus": 1, "steps": int(iters.max()), "warmup": 0,
-        "ms_per_step": round(1e3 * wall / max(1, launches * 50), 4),
+        "ms_per_step": round(1e3 * wall / max(1, launches * args.sweep_iters_per_graph), 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 bulk, f64 state",
         "data": "synthetic (seeded Gaussian psi0, harmonic traps)",
         "config": {"workload": f"sweep of {len(insts)} dipolar / soft-core instances",
                    "shapes": [list(s) for s in sw.shapes], "dt": dt, "beta": beta, "eps": eps,
-                   "iters_per_graph": 50, "mode": args.sweep_mode},
+                   "iters_per_graph": args.sweep_iters_per_graph, "mode": args.sweep_mode},
         "converged": int(conv.sum()), "instances": len(insts),
         "iterations": {"min": int(iters.min()), "max": int(iters.max()), "total": int(iters.sum())},
         "wall_s": round(wall, 3), "graph_launches": launches, "graph_captures": sw.captures,
@@ -555,7 +556,8 @@ def main():
     ap.add_argument("--sweep-mode", choices=("per-group", "shared"), default="per-group",
                     help="sweep replays: one graph per shape group on its own stream, or one graph of all groups")
     ap.add_argument("--sweep-max-iters", type=int, default=20000)
-    ap.add_argument("--sweep-recapture", type=float, default=0.5,
+    ap.add_argument("--sweep-iters-per-graph", type=int, default=50)
+    ap.add_argument("--sweep-recapture", type=float, default=0.75,
                     help="re-capture the graph over the unconverged instances once this share of the "
                          "instances in it has converged")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",

@@ -50,7 +50,7 @@ class Sweep:
     def autotune(self, reps=3):
         return {s: g.autotune(reps) for s, g in zip(self.shapes, self.groups)}
 
-    def run(self, dt, beta, eps, iters_per_graph=50, max_iters=20000, sync=None, recapture=0.5):
+    def run(self, dt, beta, eps, iters_per_graph=50, max_iters=20000, sync=None, recapture=0.75):
         """Replays captured iterations until all instances converge (or
         max_iters iterations have been replayed).  Converged instances only
         cost an early exit per block, but their blocks are still launched:
