@@ -387,6 +387,7 @@ def run_sweep(torch, args):
         "hbm_GBs_model": round(gbs, 1),
         "autotune": {"x".join(map(str, k)): {c: [round(x, 4) for x in v] for c, v in t.items()}
                      for k, t in tune.items()},
+        "block_order": {"x".join(map(str, shp)): sw.ctxs[m[0]].order() for shp, m in zip(sw.shapes, sw.members)},
         "energies": [round(float(e), 10) for e in obs[:, 2]],
     }
     if hasattr(sw, "group_finish_s"):

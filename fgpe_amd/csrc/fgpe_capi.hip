@@ -1087,15 +1087,25 @@ template <typename T> GroupTabs group_tabs(fgpe_group* g, double dt, double beta
     t.o_f = align256(t.o_x + n * sizeof(XArgs<T>));
     t.bytes = align256(t.o_f + n * sizeof(FinArgs));
     t.host.assign(t.bytes, 0);
+    // The group's block order (bit k: pass k, y fwd / last / y inv / x, runs
+    // reversed): a reversed pass takes its tiles in reverse (the entries' rev)
+    // and its instances too (its entries stored in reverse instance order, since
+    // blockIdx.y indexes the table), so that it starts on what the pass before it
+    // wrote last.
+    const int ord = g->c[0]->order;
+    auto slot = [&](int bit, size_t i) { return ((ord >> bit) & 1) ? n - 1 - i : i; };
     for (size_t i = 0; i < n; ++i) {
         fgpe_ctx* c = g->c[i];
         const int la = last_axis(c);
         const double* scale = timing ? nullptr : c->scal + SC_SCALE;
         const int* stop = timing ? nullptr : c->flags + FL_STOP;
-        const StridedArgs sf = strided_args(c, c->spec, 1, scale, c->nch, 1, true, stop);
-        const LastArgs l = last_args(c, c->spec, la, c->nch, 1, la == 2 ? nullptr : scale, has_filter(c->hp), stop,
-                                     true, !timing);
-        const StridedArgs si = strided_args(c, c->spec, 1, nullptr, c->nch, 1, true, stop);
+        StridedArgs sf = strided_args(c, c->spec, 1, scale, c->nch, 1, true, stop);
+        LastArgs l = last_args(c, c->spec, la, c->nch, 1, la == 2 ? nullptr : scale, has_filter(c->hp), stop,
+                               true, !timing);
+        StridedArgs si = strided_args(c, c->spec, 1, nullptr, c->nch, 1, true, stop);
+        sf.rev = ord & 1;
+        l.s.rev = (ord >> 1) & 1;
+        si.rev = (ord >> 2) & 1;
         XArgs<T> x;
         if (timing) {
             x = base_xargs<T>(c, 0.0);
@@ -1105,17 +1115,19 @@ template <typename T> GroupTabs group_tabs(fgpe_group* g, double dt, double beta
         } else {
             x = hb_xargs<T>(c, 0.0, dt, beta);
         }
+        x.rev = (ord >> 3) & 1;
         const FinArgs f = fin_args(c, Launch<T>::xblocks(c), FIN_HB, npart, c->scal + SC_TMP, eps);
-        std::memcpy(&t.host[t.o_sf + i * sizeof(StridedArgs)], &sf, sizeof(sf));
-        std::memcpy(&t.host[t.o_l + i * sizeof(LastArgs)], &l, sizeof(l));
-        std::memcpy(&t.host[t.o_si + i * sizeof(StridedArgs)], &si, sizeof(si));
-        std::memcpy(&t.host[t.o_x + i * sizeof(XArgs<T>)], &x, sizeof(x));
+        std::memcpy(&t.host[t.o_sf + slot(0, i) * sizeof(StridedArgs)], &sf, sizeof(sf));
+        std::memcpy(&t.host[t.o_l + slot(1, i) * sizeof(LastArgs)], &l, sizeof(l));
+        std::memcpy(&t.host[t.o_si + slot(2, i) * sizeof(StridedArgs)], &si, sizeof(si));
+        std::memcpy(&t.host[t.o_x + slot(3, i) * sizeof(XArgs<T>)], &x, sizeof(x));
         std::memcpy(&t.host[t.o_f + i * sizeof(FinArgs)], &f, sizeof(f));
     }
     return t;
 }
 
-// Kernel classes of one batched iteration (cls < 0: all, then the finalize).
+// Kernel classes of one batched iteration (cls = -1: all, then the finalize;
+// -2: all but the finalize).
 template <typename T> int group_launch(fgpe_group* g, const GroupTabs& t, const unsigned char* dtab, int cls,
                                        int mode)
 {
@@ -1132,7 +1144,7 @@ template <typename T> int group_launch(fgpe_group* g, const GroupTabs& t, const 
         FGPE_RET(Launch<T>::strided(c0, c0->spec, 1, +1, nch, nc, nullptr, nullptr, true, dtab + t.o_si, n));
     if (cls < 0 || cls == FGPE_KCLASS_XSTAGE)
         FGPE_RET(Launch<T>::xstage(c0, XArgs<T>{}, nc, dens, mode, dtab + t.o_x, n));
-    if (cls < 0) {
+    if (cls == -1) {
         TScope ts(c0, FGPE_KCLASS_OTHER);
         hipLaunchKernelGGL(k_finalize, dim3((unsigned)n), dim3(kFinThreads), 0, c0->stream, FinArgs{},
                            (const FinArgs*)(dtab + t.o_f));
@@ -1234,6 +1246,40 @@ template <typename T> int group_autotune(fgpe_group* g, int reps, double* out)
             }
         }
         for (fgpe_ctx* c : g->c) c->var[cls][axis] = bestv;
+    }
+    // Block order: the 16 combinations of passes run reversed (tiles and
+    // instances), timed as whole batched iterations (at least 10 each: one
+    // iteration of a small group takes about 0.1 ms), so that each pass starts
+    // on what the pass before it wrote last; a reversal has to beat the
+    // forward order by 1%.
+    if (rc == FGPE_OK) {
+        const int oreps = reps > 10 ? reps : 10;
+        double t_fwd = 0.0, best = 1e300;
+        int besto = 0;
+        for (int o = 0; o < 16 && rc == FGPE_OK; ++o) {
+            if (la != 2 && (o & 5)) continue;   // no y passes: bits 0 and 2 unused
+            for (fgpe_ctx* c : g->c) c->order = o;
+            const GroupTabs to = group_tabs<T>(g, 0.0, 0.0, one, 0.0, true);
+            rc = group_upload(c0, &g->tab, &g->cap, to);
+            if (rc != FGPE_OK) break;
+            const unsigned char* dto = (const unsigned char*)g->tab;
+            rc = group_launch<T>(g, to, dto, -2, MODE_RK);   // warm-up
+            if (rc != FGPE_OK) break;
+            (void)hipEventRecord(e0, c0->stream);
+            for (int r = 0; r < oreps && rc == FGPE_OK; ++r) rc = group_launch<T>(g, to, dto, -2, MODE_RK);
+            (void)hipEventRecord(e1, c0->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FGPE_EHIP, "autotune timing failed");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / oreps;
+            if (o == 0) t_fwd = per;
+            if (per < best) {
+                best = per;
+                besto = o;
+            }
+        }
+        if (best > 0.99 * t_fwd) besto = 0;
+        for (fgpe_ctx* c : g->c) c->order = besto;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

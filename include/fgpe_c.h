@@ -360,7 +360,9 @@ int fgpe_group_create(fgpe_ctx* const* ctxs, int n, fgpe_group** out);
 int fgpe_group_destroy(fgpe_group* group);
 /* Times both plan variants of each kernel class as batched launches and sets
    the faster on every context (host_ms: 10 doubles or NULL, as fgpe_autotune;
-   groups run unslabbed, the slab slots stay 0).  A heavy-ball run in progress
+   groups run unslabbed, the slab slots stay 0), then the block order
+   (fgpe_set_order's bits; a batched pass run reversed also takes its
+   instances in reverse) timed as whole batched iterations.  A heavy-ball run in progress
    ends here (its pending renormalisation is applied); the next
    fgpe_group_imag_steps starts a fresh run from the normalised states. */
 int fgpe_group_autotune(fgpe_group* group, int reps, double* host_ms);

@@ -216,3 +216,37 @@ def test_sweep_per_group_streams_match_one_graph(torch_cuda):
     assert np.array_equal(e0, e1)
     for a, b in zip(p0, p1):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("order", [5, 10, 15])
+def test_group_block_order_is_bitwise_neutral(torch_cuda, order):
+    """Batched heavy-ball iterations with passes run reversed (tiles and
+    instances, the order the group autotuner may pick) give the same states,
+    bit for bit, as the forward order."""
+    from fgpe_amd import Group, configs
+    insts = configs.sweep_instances(3, seed=13, shapes=((32, 32, 32),))
+    out = []
+    for o in (0, order):
+        side = torch_cuda.cuda.Stream()
+        ctxs = []
+        for inst in insts:
+            g = inst.grid
+            c = Case(g.nx, g.ny, g.nz, L=(g.lx, g.ly, g.lz)).ctx(1)
+            c.set_stream(side.cuda_stream)
+            c.set_hamiltonian(a=(inst.a,), a_dd=inst.a_dd, dip=inst.dip, sc_U=inst.sc_U, sc_R=inst.sc_R,
+                              sc_dim=inst.sc_dim)
+            c.upload_psi(inst.initial_state())
+            c.upload_vext(inst.trap())
+            c.set_order(o)
+            ctxs.append(c)
+        grp = Group(ctxs)
+        try:
+            grp.imag_steps(DT, 12, beta=BETA, npart=1.0, eps=0.0)
+            grp.imag_finish()
+            out.append([c.download_psi() for c in ctxs])
+        finally:
+            grp.close()
+            for c in ctxs:
+                c.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
