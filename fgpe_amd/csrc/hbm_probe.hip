@@ -13,6 +13,16 @@
 //                    W consecutive x columns of all 256 points along z (pencils
 //                    1 MiB apart) or along y (4 KiB apart), holds them in
 //                    registers and writes them back scaled
+//   chain            one RK stage's pass sequence without the FFTs, on the
+//                    step's own footprint (a complex channel and a half channel
+//                    of 128 stored columns, row pitch 144: 419 MB): a
+//                    contiguous in-place pass over both (the x stage's spectral
+//                    traffic), then y, z and y tile passes (16 columns, 512
+//                    threads), each pass's blocks forward or reversed (the 16
+//                    block orders) and the tiles enumerated outer index by outer
+//                    index (as the step does) or column group by column group;
+//                    every pass starts on what the one before it left in the
+//                    memory-side cache
 //
 // Build: python -m fgpe_amd.build (fgpe_amd/hbm_probe).  Run: fgpe_amd/hbm_probe
 #include <hip/hip_runtime.h>
@@ -88,6 +98,67 @@ template <int W, int THREADS, bool ZAXIS> __global__ void __launch_bounds__(THRE
     for (int q = 0; q < PER; ++q) d[base + (long long)(r0 + q * R) * st] = double2{v[q].x * 0.5, v[q].y * 0.5};
 }
 
+// ---- chain: the passes of one RK stage without the FFTs ---------------------
+constexpr int CH_W = 16, CH_TH = 512, CH_R = CH_TH / CH_W, CH_PER = 256 / CH_R;
+constexpr int CH_HP = 144, CH_HC = 128;                       // half channel: pitch, stored columns used
+constexpr long long CH_HALF = (long long)NX * NY * CH_HP;     // half-channel elements
+constexpr int CH_NTF = NX / CH_W, CH_NHF = CH_HC / CH_W, CH_PERO = CH_NTF + CH_NHF;
+constexpr int CH_TILES = 256 * CH_PERO;
+
+// tile bx -> (outer index, column group, channel): outer index by outer index
+// (all column groups of both channels of one outer index together) or column
+// group by column group (all outer indices of one group, both channels, together)
+__device__ __forceinline__ void chain_tile(int bx, bool colmaj, int& outer, int& g, int& ch)
+{
+    if (!colmaj) {
+        outer = bx / CH_PERO;
+        const int w = bx % CH_PERO;
+        ch = w >= CH_NTF;
+        g = ch ? w - CH_NTF : w;
+    } else if (bx < CH_NHF * 2 * 256) {
+        g = bx / 512;
+        outer = (bx % 512) / 2;
+        ch = bx % 2;
+    } else {
+        const int b2 = bx - CH_NHF * 512;
+        g = CH_NHF + b2 / 256;
+        outer = b2 % 256;
+        ch = 0;
+    }
+}
+
+template <bool ZAXIS> __global__ void __launch_bounds__(CH_TH)
+k_chain_tile(double2* __restrict__ psi, double2* __restrict__ half, int rev, int colmaj, double sc)
+{
+    const int bx = rev ? CH_TILES - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    int outer, g, ch;
+    chain_tile(bx, colmaj != 0, outer, g, ch);
+    const int col = threadIdx.x % CH_W, r0 = threadIdx.x / CH_W;
+    const long long row = ch ? CH_HP : NX, plane = row * NY;
+    double2* d = ch ? half : psi;
+    const long long base = (ZAXIS ? (long long)outer * row : (long long)outer * plane) + g * CH_W + col;
+    const long long st = ZAXIS ? plane : row;
+    double2 v[CH_PER];
+#pragma unroll
+    for (int q = 0; q < CH_PER; ++q) v[q] = d[base + (long long)(r0 + q * CH_R) * st];
+#pragma unroll
+    for (int q = 0; q < CH_PER; ++q) d[base + (long long)(r0 + q * CH_R) * st] = double2{v[q].x * sc, v[q].y * sc};
+}
+
+// contiguous in-place pass over the complex channel, then the half channel
+__global__ void __launch_bounds__(256) k_chain_x(double2* __restrict__ psi, double2* __restrict__ half, int rev,
+                                                 double sc)
+{
+    const long long nb_psi = NTOT / 1024, nb = nb_psi + CH_HALF / 1024;
+    const long long b = rev ? nb - 1 - blockIdx.x : blockIdx.x;
+    double2* d = b < nb_psi ? psi + b * 1024 : half + (b - nb_psi) * 1024;
+    double2 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = d[threadIdx.x + q * 256];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[threadIdx.x + q * 256] = double2{v[q].x * sc, v[q].y * sc};
+}
+
 struct Timer {
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -116,6 +187,62 @@ template <int W, int THREADS, bool Z> int tile(double2* d, Timer& tm)
     char name[64];
     std::snprintf(name, sizeof name, "tile %s, %d columns, %d threads", Z ? "z" : "y", W, THREADS);
     return timed(name, 2 * BYTES, tm, [&] { hipLaunchKernelGGL((k_tile<W, THREADS, Z>), dim3(blocks), dim3(THREADS), 0, 0, d); });
+}
+
+// All 16 block orders (bit 0: y forward, 1: z, 2: y inverse, 3: x pass reversed)
+// of one enumeration; prints the best order's per-pass times (10 sequences each,
+// events between the passes).
+int chain(double2* psi, double2* half, int colmaj)
+{
+    static_assert(NTOT % 1024 == 0 && CH_HALF % 1024 == 0, "x-pass blocks");
+    hipEvent_t e[5];
+    for (auto& x : e) PROBE_CHK(hipEventCreate(&x));
+    const unsigned xblocks = (unsigned)((NTOT + CH_HALF) / 1024);
+    const double tile_bytes = 2.0 * sizeof(double2) * ((double)NTOT + (double)NX * NY * CH_HC);
+    const double x_bytes = 2.0 * sizeof(double2) * ((double)NTOT + (double)CH_HALF);
+    float best = 1e30f, bt[4] = {};
+    int besto = 0;
+    for (int o = 0; o < 16; ++o) {
+        auto seq = [&](bool ev) {
+            if (ev) (void)hipEventRecord(e[0], 0);
+            hipLaunchKernelGGL(k_chain_x, dim3(xblocks), dim3(256), 0, 0, psi, half, (o >> 3) & 1, 1.0);
+            if (ev) (void)hipEventRecord(e[1], 0);
+            hipLaunchKernelGGL(k_chain_tile<false>, dim3(CH_TILES), dim3(CH_TH), 0, 0, psi, half, o & 1, colmaj, 1.0);
+            if (ev) (void)hipEventRecord(e[2], 0);
+            hipLaunchKernelGGL(k_chain_tile<true>, dim3(CH_TILES), dim3(CH_TH), 0, 0, psi, half, (o >> 1) & 1, colmaj,
+                               1.0);
+            if (ev) (void)hipEventRecord(e[3], 0);
+            hipLaunchKernelGGL(k_chain_tile<false>, dim3(CH_TILES), dim3(CH_TH), 0, 0, psi, half, (o >> 2) & 1, colmaj,
+                               1.0);
+            if (ev) (void)hipEventRecord(e[4], 0);
+        };
+        for (int w = 0; w < 3; ++w) seq(false);
+        PROBE_CHK(hipGetLastError());
+        PROBE_CHK(hipDeviceSynchronize());
+        float tot = 0.f, part[4] = {};
+        for (int r = 0; r < 10; ++r) {
+            seq(true);
+            PROBE_CHK(hipEventSynchronize(e[4]));
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0.f;
+                PROBE_CHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+                part[k] += ms / 10;
+                tot += ms / 10;
+            }
+        }
+        if (tot < best) {
+            best = tot;
+            besto = o;
+            for (int k = 0; k < 4; ++k) bt[k] = part[k];
+        }
+    }
+    std::printf("chain, %s tiles, best block order %2d: x %.1f us (%.0f GB/s), y fwd %.1f us (%.0f GB/s), "
+                "z %.1f us (%.0f GB/s), y inv %.1f us (%.0f GB/s)\n",
+                colmaj ? "column-group-major" : "outer-major", besto, 1e3 * bt[0], x_bytes / bt[0] / 1e6, 1e3 * bt[1],
+                tile_bytes / bt[1] / 1e6, 1e3 * bt[2], tile_bytes / bt[2] / 1e6, 1e3 * bt[3], tile_bytes / bt[3] / 1e6);
+    std::fflush(stdout);
+    for (auto& x : e) PROBE_CHK(hipEventDestroy(x));
+    return 0;
 }
 
 }  // namespace
@@ -157,6 +284,10 @@ int main()
     rc |= tile<16, 256, false>(a, tm);
     rc |= tile<16, 1024, false>(a, tm);
     rc |= tile<32, 1024, false>(a, tm);
+    // the chain runs on the first 268 MB of a (complex channel) and of b (half channel)
+    static_assert(CH_HALF <= NELEM, "half channel fits in b");
+    rc |= chain(a, b, 0);
+    rc |= chain(a, b, 1);
     PROBE_CHK(hipFree(a));
     PROBE_CHK(hipFree(b));
     return rc;
