@@ -1,7 +1,8 @@
 """Batched imaginary-time ground-state sweep: many independent instances of
-uneven grid sizes, one group (batched launches) per grid shape, all groups'
-heavy-ball iterations captured in one hipGraph and replayed until every
-instance has met |E - E_prev| / E_prev < eps (the reference's test,
+uneven grid sizes, one group (batched launches) per grid shape, the groups'
+heavy-ball iterations captured as hipGraphs (one per group, each on a stream
+of its own, or one graph of all groups on a shared stream) and replayed until
+every instance has met |E - E_prev| / E_prev < eps (the reference's test,
 src/oneComponentGPSolver.cu:84-88) or max_iters is reached."""
 from __future__ import annotations
 
