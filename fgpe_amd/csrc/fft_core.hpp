@@ -309,8 +309,46 @@ __device__ __forceinline__ void exchange(T2* v, int t, T2* lds)
     exchange_sync<WAVE>();
 }
 
-template <typename T2, int N, int V, int DIR, int S, bool WAVE, typename TP>
-__device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, TP tw)
+// The same exchange through a scalar LDS buffer of the same length, half the
+// bytes: the real parts go through it first, then the imaginary parts, each
+// with its own write / sync / read / sync.  A kernel whose LDS limits its
+// blocks per CU fits more of them this way, at twice the barriers.
+template <typename T2, int N, int V, int DIR, int S, bool WAVE>
+__device__ __forceinline__ void exchange(T2* v, int t, decltype(T2{}.x)* lds)
+{
+    constexpr int TT = Plan<N, V>::T, E = N / TT;
+    constexpr int r = radix_of<N, V, DIR, S>();
+    constexpr int Ns = ns_of<N, V, DIR, S>();
+    constexpr int M = E / r;
+    constexpr int r2 = radix_of<N, V, DIR, S + 1>();
+    constexpr int M2 = E / r2;
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int j = t + TT * m;
+            const int k = j % Ns;
+            const int base = (j / Ns) * Ns * r + k;
+#pragma unroll
+            for (int i = 0; i < r; ++i) lds[lds_phys(base + i * Ns)] = part ? v[m * r + i].y : v[m * r + i].x;
+        }
+        exchange_sync<WAVE>();
+#pragma unroll
+        for (int m = 0; m < M2; ++m) {
+            const int j = t + TT * m;
+#pragma unroll
+            for (int i = 0; i < r2; ++i) {
+                const auto w = lds[lds_phys(j + i * (N / r2))];
+                if (part) v[m * r2 + i].y = w;
+                else v[m * r2 + i].x = w;
+            }
+        }
+        exchange_sync<WAVE>();
+    }
+}
+
+template <typename T2, int N, int V, int DIR, int S, bool WAVE, typename TP, typename LT>
+__device__ __forceinline__ void fft_stages(T2* v, int t, LT* lds, TP tw)
 {
     constexpr int L = Plan<N, V>::L, TT = Plan<N, V>::T, E = N / TT;
     constexpr int r = radix_of<N, V, DIR, S>();
@@ -330,7 +368,7 @@ __device__ __forceinline__ void fft_stages(T2* v, int t, T2* lds, TP tw)
     }
     if constexpr (S + 1 < L) {
         exchange<T2, N, V, DIR, S, WAVE>(v, t, lds);
-        fft_stages<T2, N, V, DIR, S + 1, WAVE, TP>(v, t, lds, tw);
+        fft_stages<T2, N, V, DIR, S + 1, WAVE, TP, LT>(v, t, lds, tw);
     }
 }
 
@@ -348,18 +386,19 @@ template <typename T2, typename TP> __device__ __forceinline__ TP opaque_table(T
 }
 
 // Forward: v in layout A -> layout B.  Inverse: layout B -> layout A.
-// lds: this pencil's lds_stride<N>() elements.  WAVE: the pencil's T threads
+// lds: this pencil's lds_stride<N>() elements, complex (T2) or, for the
+// split exchange, scalar.  WAVE: the pencil's T threads
 // are consecutive lanes of one wave (then only that wave must call);
 // otherwise every thread of the block must call (block barriers).  tw: the
 // twiddle table of length N, in global memory (gp) or staged in LDS.
-template <typename T2, int N, int V, bool WAVE = false, typename TP>
-__device__ __forceinline__ void fft_forward(T2* v, int t, T2* lds, TP tw)
+template <typename T2, int N, int V, bool WAVE = false, typename TP, typename LT>
+__device__ __forceinline__ void fft_forward(T2* v, int t, LT* lds, TP tw)
 {
     static_assert(!WAVE || (Plan<N, V>::T <= 64 && 64 % Plan<N, V>::T == 0), "pencil must fit in a wave");
     fft_stages<T2, N, V, -1, 0, WAVE>(v, t, lds, opaque_table<T2>(tw));
 }
-template <typename T2, int N, int V, bool WAVE = false, typename TP>
-__device__ __forceinline__ void fft_inverse(T2* v, int t, T2* lds, TP tw)
+template <typename T2, int N, int V, bool WAVE = false, typename TP, typename LT>
+__device__ __forceinline__ void fft_inverse(T2* v, int t, LT* lds, TP tw)
 {
     static_assert(!WAVE || (Plan<N, V>::T <= 64 && 64 % Plan<N, V>::T == 0), "pencil must fit in a wave");
     fft_stages<T2, N, V, +1, 0, WAVE>(v, t, lds, opaque_table<T2>(tw));

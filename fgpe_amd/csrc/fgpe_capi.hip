@@ -894,8 +894,17 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         *per = ms / reps;
         return FGPE_OK;
     };
+    // one RK stage: the spectral passes and an x stage
+    auto stage = [&]() -> int {
+        FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
+        return run(FGPE_KCLASS_XSTAGE);
+    };
     // the plan variants of one kernel class on one axis (the x class has a
-    // third where its length has one); keeps the fastest
+    // third where its length has one); keeps the fastest.  The x stage is
+    // timed alone; the passes in whole RK stages, since a pass's speed
+    // depends on what the pass before it left in the memory-side cache (at
+    // 256^3 FP64 the two last-axis plans tie alone and differ by 7% in a
+    // stage).
     double tt[7] = {};
     auto tune = [&](int cls, int axis, int ci) -> int {
         const int nv = cls == FGPE_KCLASS_XSTAGE && x_variant2(c->n[0]) ? 3 : 2;
@@ -904,7 +913,8 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         for (int v = 0; v < nv; ++v) {
             c->var[cls][axis] = v;
             double per = 0.0;
-            FGPE_RET(timed([&] { return run(cls); }, &per));
+            if (cls == FGPE_KCLASS_XSTAGE) FGPE_RET(timed([&] { return run(cls); }, &per));
+            else FGPE_RET(timed(stage, &per));
             tt[v < 2 ? 2 * ci + v : 6] = per;
             if (per < best) {
                 best = per;
@@ -913,11 +923,6 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         }
         c->var[cls][axis] = bestv;
         return FGPE_OK;
-    };
-    // one RK stage: the spectral passes and an x stage
-    auto stage = [&]() -> int {
-        FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
-        return run(FGPE_KCLASS_XSTAGE);
     };
     // block order of the four passes of a stage (reversed or not per pass),
     // timed as whole stages; keeps the fastest of the 16 combinations

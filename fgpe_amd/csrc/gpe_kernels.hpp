@@ -31,7 +31,10 @@ constexpr double kPi = 3.141592653589793;  // hpp/header.hpp:28
 // Variant 0: 128-thread x blocks, 128-byte row segments in the strided
 // passes; variant 1 (with its plan of more threads per pencil): 256-thread x
 // blocks and 256-byte row segments; variant 2 (x kernels only): 128-thread
-// x blocks.
+// x blocks.  FP64 256-point pencils in variant 0 (16 elements per thread,
+// one exchange per transform) take 16-column tiles, 256-byte rows, as in
+// variant 1: the last-axis pass fits three such tiles in a CU by splitting
+// its exchanges (last_split below).
 template <typename T, int N, int V = 0> __host__ __device__ constexpr int px_of()
 {
     constexpr int th = V == 1 ? 256 : 128;
@@ -42,6 +45,7 @@ template <typename T, int N, int V = 0> __host__ __device__ constexpr int ps_of(
     constexpr int base = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
     // a pencil of 1024 takes 17 KB (FP64) of exchange space per column: a
     // wider tile would not fit the last-axis pass's LDS
+    if (sizeof(T) == 8 && N == 256 && V == 0) return 16;
     return (V == 0 || N >= 1024) ? base : 2 * base;
 }
 
@@ -471,11 +475,24 @@ __device__ __forceinline__ void last_multiply_f32(float2* v, int t, const LastAr
     }
 }
 
-// Waves per SIMD the FP32 radix-16 last-axis pass is compiled for: four
-// (at most 128 VGPRs) keeps four blocks per CU in flight while the
-// register-resident pencil waits on HBM.
+// The FP64 radix-16 last-axis pass (16 pencils of 256, 256 threads) exchanges
+// the real and the imaginary parts in turn through a scalar LDS buffer: 43 KB
+// per block instead of 78, so three blocks share a CU and one block's
+// transforms overlap the others' HBM traffic (the pass went from 166 to
+// 154 us at 256^3; its access pattern alone, chained as in the step, takes
+// about 122 us).
+template <typename T, int N, int V> constexpr bool last_split()
+{
+    return sizeof(T) == 8 && N == 256 && V == 0;
+}
+
+// Waves per SIMD the radix-16 last-axis passes are compiled for: four for
+// FP32 (at most 128 VGPRs) keeps four blocks per CU in flight while the
+// register-resident pencil waits on HBM; three for the split FP64 pass (at
+// most 168 VGPRs, three 4-wave blocks).
 template <typename T, int N, int V> constexpr int last_waves()
 {
+    if constexpr (last_split<T, N, V>()) return 3;
     return (sizeof(T) == 4 && N == 256 && V == 0) ? 4 : 1;
 }
 
@@ -486,7 +503,8 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
-    __shared__ T2 lds[P * lds_stride<N>()];
+    using LT = std::conditional_t<last_split<T, N, V>(), T, T2>;   // exchange buffer element
+    __shared__ LT lds[P * lds_stride<N>()];
     const LastArgs a = args_of(a0, tab, blockIdx.y);
     if (a.s.stop && *glob(a.s.stop)) return;
     // the last axis's k pairs, read once per element by the multiply, are
@@ -503,7 +521,7 @@ k_last(const c2_t<T>* __restrict__ twg, LastArgs a0, const LastArgs* __restrict_
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a.s, p, blockIdx.x, gridDim.x);
     const gp<T2> base = data + at.off;
-    T2* my = lds + p * lds_stride<N>();
+    LT* my = lds + p * lds_stride<N>();
     // this column's k_x and outer-axis k pairs, loaded before the pencil so
     // that their latency hides behind it rather than after the transform
     const T2 kxp2 = glob((const T2*)(sizeof(T) == 8 ? (const void*)a.kX64 : (const void*)a.kX32))[at.ix];
