@@ -900,11 +900,12 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         return run(FGPE_KCLASS_XSTAGE);
     };
     // the plan variants of one kernel class on one axis (the x class has a
-    // third where its length has one); keeps the fastest.  The x stage is
-    // timed alone; the passes in whole RK stages, since a pass's speed
-    // depends on what the pass before it left in the memory-side cache (at
-    // 256^3 FP64 the two last-axis plans tie alone and differ by 7% in a
-    // stage).
+    // third where its length has one); keeps the fastest.  The x stage and
+    // the y passes are timed alone; the last-axis pass in whole RK stages,
+    // since it runs on what the y pass before it left in the memory-side
+    // cache (at 256^3 FP64 its two plans tie alone and differ by 7% in a
+    // stage).  The y passes' plans differ by 8% alone in FP32, where a whole
+    // stage would drown that in its own noise.
     double tt[7] = {};
     auto tune = [&](int cls, int axis, int ci) -> int {
         const int nv = cls == FGPE_KCLASS_XSTAGE && x_variant2(c->n[0]) ? 3 : 2;
@@ -913,7 +914,7 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         for (int v = 0; v < nv; ++v) {
             c->var[cls][axis] = v;
             double per = 0.0;
-            if (cls == FGPE_KCLASS_XSTAGE) FGPE_RET(timed([&] { return run(cls); }, &per));
+            if (cls != FGPE_KCLASS_LAST) FGPE_RET(timed([&] { return run(cls); }, &per));
             else FGPE_RET(timed(stage, &per));
             tt[v < 2 ? 2 * ci + v : 6] = per;
             if (per < best) {

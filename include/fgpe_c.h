@@ -312,8 +312,9 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
    exists, 256-thread x blocks, 256-byte row segments).  fgpe_autotune times
    both for the x-stage, strided and last-axis classes on the context's own
    buffers (reps launches each; the state is preserved) and keeps the faster:
-   the x stage alone, the passes in whole RK stages (a pass's speed depends on
-   what the pass before it left in the memory-side cache).
+   the x stage and the y passes alone, the last-axis pass in whole RK stages
+   (its speed depends on what the pass before it left in the memory-side
+   cache).
    It then times whole RK stages: the 16 block orders (fgpe_set_order), on 3D
    grids for each choice of the k-space multiply axis (fgpe_set_kmul_axis,
    with that choice's pass variants tuned first), keeping the fastest, and
@@ -322,7 +323,7 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
    The x-stage class has a third plan, variant 2 (4 elements per thread in
    radix-4 stages), on nx = 128 and 256; elsewhere variant 2 runs variant 1.
    host_ms (10 doubles or NULL) gets the times per class and variant 0/1 of
-   the chosen configuration (6: strided, last-axis, x stage; the passes'
+   the chosen configuration (6: strided, last-axis, x stage; the last-axis
    times are those of whole RK stages with that variant), then per slab
    count (3; 0 where not applicable), then the x stage's variant 2 (0 where
    the length has none).  Needs the Hamiltonian, trap and state set.
