@@ -36,11 +36,13 @@ def make_ctx(case, inst, precision, stream=None):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 1)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 1), (32, 16, 256), (32, 256, 1)])
 def test_group_equals_single_contexts(torch_cuda, precision, shape):
+    """The 256-long last axes run the FP64 last-axis pass's split-exchange
+    plan (plan 0, the default) in batched launches."""
     from fgpe_amd import FGPE_IMAG_HEAVY_BALL, Group
     side = torch_cuda.cuda.Stream()
-    case = Case(*shape, L=(12.0, 12.0, 12.0))
+    case = Case(*shape, L=tuple(12.0 if n <= 64 else 24.0 for n in shape))
     singles = [make_ctx(case, inst, precision) for inst in INSTANCES]
     batch = [make_ctx(case, inst, precision, side.cuda_stream) for inst in INSTANCES]
     group = Group(batch)
