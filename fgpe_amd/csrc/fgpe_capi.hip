@@ -882,30 +882,31 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         a.stage = 2;
         return Launch<T>::xstage(c, a, nc, dens, MODE_RK);
     };
-    // average time of reps calls of f after one untimed call
-    auto timed = [&](auto&& f, double* per) -> int {
+    // average time of n calls of f after one untimed call
+    auto timed_n = [&](auto&& f, int n, double* per) -> int {
         FGPE_RET(f());
         (void)hipEventRecord(e0, c->stream);
-        for (int r = 0; r < reps; ++r) FGPE_RET(f());
+        for (int r = 0; r < n; ++r) FGPE_RET(f());
         (void)hipEventRecord(e1, c->stream);
         if (hipEventSynchronize(e1) != hipSuccess) return fail(FGPE_EHIP, "autotune timing failed");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        *per = ms / reps;
+        *per = ms / n;
         return FGPE_OK;
     };
+    auto timed = [&](auto&& f, double* per) -> int { return timed_n(f, reps, per); };
     // one RK stage: the spectral passes and an x stage
     auto stage = [&]() -> int {
         FGPE_RET(spectral_passes_on<T>(c, c->spec, c->nch, nc, nullptr, nullptr, true));
         return run(FGPE_KCLASS_XSTAGE);
     };
     // the plan variants of one kernel class on one axis (the x class has a
-    // third where its length has one); keeps the fastest.  The x stage and
-    // the y passes are timed alone; the last-axis pass in whole RK stages,
-    // since it runs on what the y pass before it left in the memory-side
-    // cache (at 256^3 FP64 its two plans tie alone and differ by 7% in a
-    // stage).  The y passes' plans differ by 8% alone in FP32, where a whole
-    // stage would drown that in its own noise.
+    // third where its length has one); keeps the fastest.  The x stage is
+    // timed alone; the passes in whole RK stages (twice the repetitions),
+    // since a pass runs on what the pass before it left in the memory-side
+    // cache: at 256^3 FP64 the two last-axis plans tie alone and differ by 7%
+    // in a stage, and the y passes' split plan loses alone and wins by 3% in
+    // a stage.
     double tt[7] = {};
     auto tune = [&](int cls, int axis, int ci) -> int {
         const int nv = cls == FGPE_KCLASS_XSTAGE && x_variant2(c->n[0]) ? 3 : 2;
@@ -914,8 +915,8 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
         for (int v = 0; v < nv; ++v) {
             c->var[cls][axis] = v;
             double per = 0.0;
-            if (cls != FGPE_KCLASS_LAST) FGPE_RET(timed([&] { return run(cls); }, &per));
-            else FGPE_RET(timed(stage, &per));
+            if (cls == FGPE_KCLASS_XSTAGE) FGPE_RET(timed([&] { return run(cls); }, &per));
+            else FGPE_RET(timed_n(stage, 2 * reps, &per));
             tt[v < 2 ? 2 * ci + v : 6] = per;
             if (per < best) {
                 best = per;
@@ -965,6 +966,15 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
                 FGPE_RET(tune(FGPE_KCLASS_LAST, la, 1));
                 double per = 0.0;
                 FGPE_RET(tune_order(&per));
+                // the pass plans again in the chosen block order (what a pass
+                // finds in the memory-side cache depends on it: at 256^3 the
+                // FP64 y passes' split plan loses in the forward order and wins
+                // by 3% in the chosen one), and the order again if one changed
+                const int vs = c->var[FGPE_KCLASS_STRIDED][3 - la], vl = c->var[FGPE_KCLASS_LAST][la];
+                FGPE_RET(tune(FGPE_KCLASS_STRIDED, 3 - la, 0));
+                FGPE_RET(tune(FGPE_KCLASS_LAST, la, 1));
+                if (c->var[FGPE_KCLASS_STRIDED][3 - la] != vs || c->var[FGPE_KCLASS_LAST][la] != vl)
+                    FGPE_RET(tune_order(&per));
                 if (per < best * (yl ? 0.99 : 1.0)) {
                     best = per;
                     besty = yl != 0;
@@ -977,6 +987,11 @@ template <typename T> int autotune(fgpe_ctx* c, int reps, double* out)
             std::memcpy(tt, keep, sizeof(tt));
         } else {
             FGPE_RET(tune_order(&best));
+            if (la0 == 1) {   // 2D: the last-axis plan again in the chosen order
+                const int vl = c->var[FGPE_KCLASS_LAST][1];
+                FGPE_RET(tune(FGPE_KCLASS_LAST, 1, 1));
+                if (c->var[FGPE_KCLASS_LAST][1] != vl) FGPE_RET(tune_order(&best));
+            }
         }
         if (out) {
             for (int k = 0; k < 6; ++k) out[k] = tt[k];

@@ -33,8 +33,8 @@ constexpr double kPi = 3.141592653589793;  // hpp/header.hpp:28
 // blocks and 256-byte row segments; variant 2 (x kernels only): 128-thread
 // x blocks.  FP64 256-point pencils in variant 0 (16 elements per thread,
 // one exchange per transform) take 16-column tiles, 256-byte rows, as in
-// variant 1: the last-axis pass fits three such tiles in a CU by splitting
-// its exchanges (last_split below).
+// variant 1: the y and last-axis passes fit three such tiles in a CU by
+// splitting their exchanges (strided_split, last_split below).
 template <typename T, int N, int V = 0> __host__ __device__ constexpr int px_of()
 {
     constexpr int th = V == 1 ? 256 : 128;
@@ -277,13 +277,32 @@ template <int P> __device__ __forceinline__ TileAt strided_tile(const StridedArg
     return r;
 }
 
+// The FP64 radix-16 y pass (plan 0 at length 256, 16 pencils, 256 threads)
+// splits its exchange like the last-axis pass below: 39 KB of LDS instead of
+// 74, so three blocks share a CU at its register budget of 3 waves per SIMD
+// (132 VGPRs, no spill); in a running step at 256^3 it then takes 127 us
+// against 131 unsplit and 133 for plan 1.
+template <typename T, int N, int V> constexpr bool strided_split()
+{
+    return sizeof(T) == 8 && N == 256 && V == 0;
+}
+#ifndef FGPE_STRIDED_WAVES
+#define FGPE_STRIDED_WAVES 3
+#endif
+template <typename T, int N, int V> constexpr int strided_waves()
+{
+    return strided_split<T, N, V>() ? FGPE_STRIDED_WAVES : 1;
+}
+
 template <typename T, int N, int V, int DIR>
 __global__ void __launch_bounds__((ps_of<T, N, V>() * Plan<N, V>::T))
+__attribute__((amdgpu_waves_per_eu(strided_waves<T, N, V>())))
 k_strided(const c2_t<T>* __restrict__ twg, StridedArgs a0, const StridedArgs* __restrict__ tab)
 {
     using T2 = c2_t<T>;
     constexpr int TT = Plan<N, V>::T, E = Plan<N, V>::E, P = ps_of<T, N, V>();
-    __shared__ T2 lds[P * lds_stride<N>()];
+    using LT = std::conditional_t<strided_split<T, N, V>(), T, T2>;   // exchange buffer element
+    __shared__ LT lds[P * lds_stride<N>()];
     const StridedArgs a = args_of(a0, tab, blockIdx.y);
     if (a.stop && *glob(a.stop)) return;
     __shared__ T2 twl[N];
@@ -292,7 +311,7 @@ k_strided(const c2_t<T>* __restrict__ twg, StridedArgs a0, const StridedArgs* __
     const int p = threadIdx.x % P, t = threadIdx.x / P;
     const TileAt at = strided_tile<P>(a, p, blockIdx.x, gridDim.x);
     const gp<T2> base = data + at.off;
-    T2* my = lds + p * lds_stride<N>();
+    LT* my = lds + p * lds_stride<N>();
     T2 v[E];
     if (DIR < 0) {
 #pragma unroll

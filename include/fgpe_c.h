@@ -312,18 +312,19 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
    exists, 256-thread x blocks, 256-byte row segments).  fgpe_autotune times
    both for the x-stage, strided and last-axis classes on the context's own
    buffers (reps launches each; the state is preserved) and keeps the faster:
-   the x stage and the y passes alone, the last-axis pass in whole RK stages
-   (its speed depends on what the pass before it left in the memory-side
+   the x stage alone, the y and last-axis passes in whole RK stages (2 * reps;
+   a pass's speed depends on what the pass before it left in the memory-side
    cache).
    It then times whole RK stages: the 16 block orders (fgpe_set_order), on 3D
    grids for each choice of the k-space multiply axis (fgpe_set_kmul_axis,
-   with that choice's pass variants tuned first), keeping the fastest, and
+   with that choice's pass variants tuned first and once more in the chosen
+   order, the order again if a variant changed), keeping the fastest, and
    last the y / last / y sequence split into 1, 2 or 4 column slabs (a slab's
    three passes run back to back while it is in the memory-side cache).
    The x-stage class has a third plan, variant 2 (4 elements per thread in
    radix-4 stages), on nx = 128 and 256; elsewhere variant 2 runs variant 1.
    host_ms (10 doubles or NULL) gets the times per class and variant 0/1 of
-   the chosen configuration (6: strided, last-axis, x stage; the last-axis
+   the chosen configuration (6: strided, last-axis, x stage; the passes'
    times are those of whole RK stages with that variant), then per slab
    count (3; 0 where not applicable), then the x stage's variant 2 (0 where
    the length has none).  Needs the Hamiltonian, trap and state set.
