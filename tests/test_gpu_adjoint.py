@@ -306,6 +306,53 @@ def test_adjoint_x_stage_third_plan(torch_cuda):
     assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
 
 
+def test_adjoint_split_pass_plans(torch_cuda):
+    """The adjoint's y and last-axis passes on 256-long axes, where plan 0 of
+    the FP64 passes exchanges real and imaginary parts in turn: the same
+    gradient as with plan 1 (to rounding) and central differences."""
+    c = Case(16, 256, 256, L=(8.0, 16.0, 24.0))
+    V = configs.harmonic_trap(c.spec, (1.0, 0.8, 0.5)).real
+    psi0 = configs.normalise(np.exp(-0.5 * configs.harmonic_trap(c.spec, (1, 1, 0.5)).real) +
+                             0.01 * c.rand_psi(45), c.dv)
+    target = configs.normalise(np.exp(-0.6 * configs.harmonic_trap(c.spec, (1, 1, 0.4)).real) *
+                               np.exp(0.2j * c.rand_psi(46).real), c.dv)
+    a, a_dd, dt, n, every = 0.6, 0.5, 5e-4, 24, 5
+    grads = []
+    for variant in (0, 1):
+        with c.ctx() as ctx:
+            for k in (1, 2):   # strided and last-axis classes, every axis
+                for axis in range(3):
+                    ctx.set_variant(k, axis, variant)
+            ctx.set_hamiltonian(a=(a,), a_dd=a_dd, dip=(0.0, 0.0, 1.0))
+            ctx.upload_vext(V.astype(np.complex128))
+            ctx.upload_psi(psi0)
+            ctx.adjoint_forward(dt, n, every)
+            J, ov = J_of(ctx.download_psi(), target, c.dv)
+            lamT = torch_cuda.from_numpy(2 * ov * c.dv * target).cuda()
+            gV = torch_cuda.empty(c.shape, dtype=torch_cuda.float64, device="cuda")
+            g = ctx.adjoint_backward(lamT.data_ptr(), 0, gV.data_ptr())
+            ctx.sync()
+            grads.append((gV.cpu().numpy(), g))
+    (g0, s0), (g1, s1) = grads
+    assert np.abs(g0 - g1).max() <= 1e-10 * np.abs(g1).max()
+    assert abs(s0[0] - s1[0]) <= 1e-10 * abs(s1[0]) and abs(s0[1] - s1[1]) <= 1e-10 * abs(s1[1])
+    # central differences at h and h/2, Richardson-extrapolated, along the
+    # gradient itself: on this grid a smooth random direction's derivative is
+    # about 1e-6 of J's scale, where the forward's rounding decides the
+    # differences' last digits
+    dV = g0 / np.abs(g0).max()
+
+    def fd(h):
+        Jp = J_of(gpu_forward(c, V + h * dV, a, a_dd, psi0, dt, n), target, c.dv)[0]
+        Jm = J_of(gpu_forward(c, V - h * dV, a, a_dd, psi0, dt, n), target, c.dv)[0]
+        return (Jp - Jm) / (2 * h)
+
+    f1, f2 = fd(2e-4), fd(1e-4)
+    fr = (4 * f2 - f1) / 3
+    ad = float(np.sum(g0 * dV))
+    assert abs(ad - fr) <= 1e-6 * abs(fr), (ad, fr, f1, f2)
+
+
 def test_adjoint_solver_autotuned(torch_cuda):
     """AdjointSolver's default: the plan autotuner runs at the first forward;
     the gradient then agrees with the untuned one to rounding."""
