@@ -672,7 +672,9 @@ template <typename T> struct XArgs {
 // forcing it spills heavily.
 template <typename T, int N, int V, int NC, int MODE> constexpr int xstage_waves()
 {
-    if constexpr (V == 2 && NC == 1 && MODE == MODE_RK) return sizeof(T) == 4 ? 7 : 5;
+    // FP32 at nx = 256: 8 waves (64 VGPRs, a 12-byte spill) ran 178.5 us
+    // against 181-182 at 7 (70 VGPRs, no spill), the mixed step +0.8%
+    if constexpr (V == 2 && NC == 1 && MODE == MODE_RK) return sizeof(T) == 4 ? (N == 256 ? 8 : 7) : 5;
     return (sizeof(T) == 4 && V == 1 && NC == 1 && MODE == MODE_RK && N <= 256 && Plan<N, V>::E <= 8) ? 4 : 1;
 }
 
