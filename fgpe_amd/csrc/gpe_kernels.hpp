@@ -31,12 +31,15 @@ constexpr double kPi = 3.141592653589793;  // hpp/header.hpp:28
 // Variant 0: 128-thread x blocks, 128-byte row segments in the strided
 // passes; variant 1 (with its plan of more threads per pencil): 256-thread x
 // blocks and 256-byte row segments; variant 2 (x kernels only): 128-thread
-// x blocks.  FP64 256-point pencils in variant 0 (16 elements per thread,
+// x blocks (256-thread blocks of four one-wave pencils at nx = 256, which
+// halves the blocks that stage a twiddle table: x stage 337 -> 334 us FP64,
+// 179 -> 174 us FP32 at 256^3).  FP64 256-point pencils in variant 0 (16 elements per thread,
 // one exchange per transform) take 16-column tiles, 256-byte rows, as in
 // variant 1: the y and last-axis passes fit three such tiles in a CU by
 // splitting their exchanges (strided_split, last_split below).
 template <typename T, int N, int V = 0> __host__ __device__ constexpr int px_of()
 {
+    if (V == 2 && N == 256) return 4;
     constexpr int th = V == 1 ? 256 : 128;
     return Plan<N, V>::T >= th ? 1 : th / Plan<N, V>::T;
 }

@@ -270,10 +270,12 @@ def test_adjoint_with_both_kernels(torch_cuda, dims):
     assert abs(ad - f) <= 1e-6 * abs(f), (ad, f)
 
 
-def test_adjoint_x_stage_third_plan(torch_cuda):
-    """The adjoint's x kernels with the x class's third plan (nx = 128):
-    the same gradient as with variant 1 (to rounding) and central differences."""
-    c = Case(128, 16, 32, L=(16.0, 8.0, 10.0))
+@pytest.mark.parametrize("nx", [128, 256])
+def test_adjoint_x_stage_third_plan(torch_cuda, nx):
+    """The adjoint's x kernels with the x class's third plan (four pencils
+    per block: half-wave pencils at nx = 128, one wave each at nx = 256): the
+    same gradient as with variant 1 (to rounding) and central differences."""
+    c = Case(nx, 16, 32, L=(16.0, 8.0, 10.0))
     V = configs.harmonic_trap(c.spec, (1.0, 1.0, 0.25)).real
     psi0 = configs.normalise(np.exp(-0.5 * configs.harmonic_trap(c.spec, (1, 1, 0.5)).real) +
                              0.01 * c.rand_psi(43), c.dv)
