@@ -308,8 +308,10 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
 /* ---- plan variants and the autotuner (no counterpart in the reference) ----
    Every kernel class has two compiled plans per pencil length: variant 0
    (16 elements per thread on long pencils, 128-thread x blocks, 128-byte row
-   segments) and variant 1 (8 elements per thread where a second radix split
-   exists, 256-thread x blocks, 256-byte row segments).  fgpe_autotune times
+   segments; FP64 y and last-axis passes of length 256: 256-byte rows and
+   exchanges split into real and imaginary halves) and variant 1 (8 elements
+   per thread where a second radix split exists, 256-thread x blocks,
+   256-byte row segments).  fgpe_autotune times
    both for the x-stage, strided and last-axis classes on the context's own
    buffers (reps launches each; the state is preserved) and keeps the faster:
    the x stage alone, the y and last-axis passes in whole RK stages (2 * reps;
@@ -322,7 +324,8 @@ int fgpe_timing_read(fgpe_ctx* ctx, double* host_out);
    last the y / last / y sequence split into 1, 2 or 4 column slabs (a slab's
    three passes run back to back while it is in the memory-side cache).
    The x-stage class has a third plan, variant 2 (4 elements per thread in
-   radix-4 stages), on nx = 128 and 256; elsewhere variant 2 runs variant 1.
+   radix-4 stages, four pencils per block), on nx = 128 and 256; elsewhere
+   variant 2 runs variant 1.
    host_ms (10 doubles or NULL) gets the times per class and variant 0/1 of
    the chosen configuration (6: strided, last-axis, x stage; the passes'
    times are those of whole RK stages with that variant), then per slab
