@@ -15,7 +15,8 @@ def dev(torch, arr):
 
 
 FFT_SHAPES = [(32, 32, 32), (64, 32, 96), (128, 64, 32), (96, 192, 32), (512, 32, 32),
-              (256, 32, 64), (64, 64, 1), (256, 128, 1)]
+              (256, 32, 64), (64, 64, 1), (256, 128, 1),
+              (32, 256, 64), (16, 64, 256)]   # 256-long y and z: the split-exchange FP64 pass
 
 
 @pytest.mark.parametrize("shape", FFT_SHAPES)
